@@ -1,0 +1,206 @@
+/*
+ * rbfx.h -- C ABI of the MI355X (gfx950) dense RBF-collocation library.
+ *
+ * Drop-in boundary for the hot path of Updes' pde_solver / pde_solver_jit
+ * (updes/operators.py:559-683).  The reference has no FFI of its own: its
+ * boundary is the Python API, and every arithmetic step runs inside JAX/XLA
+ * or lineax.  Each entry point below replaces one of those call sites (cited
+ * per function).  Conventions:
+ *   - every pointer argument named d_* is DEVICE memory owned by the caller;
+ *     the library allocates no device memory and keeps no pointer after return;
+ *   - matrices are ROW-MAJOR with a leading dimension ld (elements);
+ *   - every call is asynchronous on the caller's stream (hipStream_t passed as
+ *     void*; NULL = default stream) unless documented as synchronising;
+ *   - the return value is RBFX_OK (0) or a negative error code; the message of
+ *     the last error of the calling thread is available from rbfx_last_error();
+ *   - no C++ exception crosses the ABI.
+ */
+#ifndef RBFX_H
+#define RBFX_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define RBFX_ABI_VERSION 1
+
+enum rbfx_status {
+  RBFX_OK = 0,
+  RBFX_ERR_ARG = -1,       /* invalid argument / shape                       */
+  RBFX_ERR_HIP = -2,       /* a HIP runtime call failed                      */
+  RBFX_ERR_SINGULAR = -3,  /* exact zero pivot met (factors still written)   */
+  RBFX_ERR_TIMEOUT = -4,   /* a bounded device-side wait expired             */
+  RBFX_ERR_STALL = -5,     /* iterative refinement stalled                   */
+};
+
+/* RBF kernels (updes/utils.py:30-69). */
+enum rbfx_kernel {
+  RBFX_POLYHARMONIC = 0,      /* r^(2a+1), param = a            */
+  RBFX_GAUSSIAN = 1,          /* exp(-(eps r)^2), param = eps   */
+  RBFX_THIN_PLATE = 2,        /* r^(2a) log r, param = a        */
+  RBFX_MULTIQUADRIC = 3,      /* sqrt(1+(eps r)^2)              */
+  RBFX_INV_MULTIQUADRIC = 4,  /* 1/sqrt(1+(eps r)^2)            */
+};
+
+/* Operator basis of a lowered nodal operator. */
+enum rbfx_op { RBFX_VAL = 0, RBFX_DX, RBFX_DY, RBFX_LAP, RBFX_DXX, RBFX_DYY, RBFX_DXY, RBFX_NOPS };
+
+/* Which matrix rbfx_assemble / rbfx_matvec describe. */
+enum rbfx_matrix {
+  RBFX_MAT_K = 0,  /* collocation matrix [[opPhi opP];[bdPhi bdP];[P^T 0]]  */
+  RBFX_MAT_A = 1,  /* interpolation matrix [[Phi P];[P^T 0]]                */
+};
+
+/*
+ * A collocation problem on N sorted nodes (updes/cloud.py:403-408 ordering:
+ * internal | dirichlet | neumann | robin | periodic groups).
+ * Row classes of K (updes/assembly.py:141-362, 366-401):
+ *   [0, Ni)                 interior rows, operator descriptor op_rbf/op_mon
+ *   [Ni, Ni+Nd)             Dirichlet rows      phi
+ *   [.., +Nn)               Neumann rows        grad phi . n
+ *   [.., +Nr)               Robin rows          beta phi + grad phi . n
+ *   [.., +Npd)              periodic value rows phi(x_i1,.) - phi(x_i2,.)
+ *   [.., +Npd)              periodic flux rows  grad phi(x_i1,.).n1 + grad phi(x_i2,.).n2
+ *   [N, N+M)                P^T rows
+ * The descriptor coefficients are op_*_const[t] + (op_*_row[t] ? op_*_row[t][i] : 0).
+ */
+typedef struct rbfx_problem {
+  int64_t N;            /* number of nodes                                   */
+  int64_t M;            /* number of monomials, 0..15                        */
+  int32_t kernel;       /* enum rbfx_kernel                                  */
+  int32_t flags;        /* reserved, 0                                       */
+  double param;         /* a (polyharmonic / thin plate) or eps              */
+  const double* d_x;    /* (N) node x, sorted order                          */
+  const double* d_y;    /* (N) node y                                        */
+  int64_t Ni, Nd, Nn, Nr, Npd;  /* row-class sizes; Npd = sum(Np)/2          */
+  const double* d_nx;   /* (Nn+Nr+2*Npd) outward normals, sorted node order  */
+  const double* d_ny;
+  const double* d_robin_normal_x; /* (Nr) normal used by the Robin phi rows   */
+  const double* d_robin_normal_y; /*      (updes/assembly.py:206 indexing)   */
+  const double* d_beta; /* (Nr) Robin coefficients, sorted node order        */
+  const int64_t* d_pd_i1; /* (Npd) first node of each periodic pair          */
+  const int64_t* d_pd_i2; /* (Npd) partner node                               */
+  double op_rbf_const[RBFX_NOPS];
+  double op_mon_const[RBFX_NOPS];
+  const double* d_op_rbf_row[RBFX_NOPS]; /* (Ni) per-row coefficients or NULL */
+  const double* d_op_mon_row[RBFX_NOPS];
+} rbfx_problem;
+
+/* ---- library ---------------------------------------------------------- */
+int rbfx_abi_version(void);
+const char* rbfx_last_error(void);
+/* Queries the current device (CU count, LDS size).  Called implicitly. */
+int rbfx_init(void);
+
+/* ---- K1 assembly ---------------------------------------------------------
+ * Writes rows [row0, row0+nrows) of K or A (n = N+M columns) in FP64 and/or
+ * FP32 (either output may be NULL).  Row r of the matrix goes to out[(r-row0)*ld].
+ * Replaces updes/assembly.py:10-36 (assemble_Phi), 39-59 (assemble_P),
+ * 62-85 (assemble_A), 93-137 (assemble_op_Phi_P), 141-362
+ * (assemble_bd_Phi_P) and the concatenation in 384-396 (assemble_B).       */
+int rbfx_assemble(const rbfx_problem* p, int matrix, int64_t row0, int64_t nrows,
+                  double* d_out64, int64_t ld64, float* d_out32, int64_t ld32,
+                  void* stream);
+
+/* Matrix-free product y = K x (trans=0) or y = K^T x (trans=1) over all
+ * n = N+M rows, recomputing every entry from the node coordinates in FP64.
+ * Replaces the products with B / A implied by updes/assembly.py:399,410,430. */
+size_t rbfx_matvec_workspace(const rbfx_problem* p, int trans);
+int rbfx_matvec(const rbfx_problem* p, int matrix, int trans, const double* d_x,
+                double* d_y, void* d_work, size_t work_bytes, void* stream);
+
+/* ---- K6 field evaluation -------------------------------------------------
+ * out[t*npts + i] = sum_j lambda_j op_t phi(p_i, c_j) + sum_k gamma_k op_t p_k(p_i)
+ * for every t set in ops_mask (bit t = enum rbfx_op), coefficients
+ * coeffs = [lambda (N); gamma (M)].  All N centres are summed, the evaluation
+ * point included (phi(0) for values, 0 for derivatives at r = 0).
+ * Replaces updes/operators.py:118-150 (value/value_vec), 156-187
+ * (gradient/gradient_vec) and 319-351 (laplacian/laplacian_vec).          */
+int rbfx_eval(int kernel, double param, int64_t N, int64_t M, const double* d_cx,
+              const double* d_cy, const double* d_coeffs, int64_t npts,
+              const double* d_px, const double* d_py, uint32_t ops_mask,
+              double* d_out, void* stream);
+
+/* Transpose of rbfx_eval for one operator t (adjoint of the evaluation):
+ * out[j] = sum_i w_i op_t phi(p_i, c_j)  (j < N),
+ * out[N+k] = sum_i w_i op_t p_k(p_i)     (k < M).  No reference counterpart
+ * (the reference differentiates value() with jax.grad).                   */
+int rbfx_eval_t(int kernel, double param, int64_t N, int64_t M, const double* d_cx,
+                const double* d_cy, int64_t npts, const double* d_px,
+                const double* d_py, int op, const double* d_w, double* d_out,
+                void* stream);
+
+/* ---- K2/K3 dense LU with partial pivoting ---------------------------------
+ * In-place P A = L U of an n x n row-major matrix (unit lower L), LAPACK
+ * getrf semantics with 0-based ipiv (row i was swapped with ipiv[i]).
+ * Replaces the factorisations inside jnp.linalg.inv (updes/assembly.py:90)
+ * and lineax QR (updes/operators.py:612-613).  *info = 0 or 1+index of the
+ * first exactly-zero pivot.  Workspace size from rbfx_getrf_workspace.     */
+size_t rbfx_getrf_workspace(int64_t n, int dtype_bytes);
+int rbfx_getrf_f64(int64_t n, double* d_A, int64_t lda, int64_t* d_ipiv,
+                   void* d_work, size_t work_bytes, int32_t* d_info, void* stream);
+int rbfx_getrf_f32(int64_t n, float* d_A, int64_t lda, int64_t* d_ipiv,
+                   void* d_work, size_t work_bytes, int32_t* d_info, void* stream);
+
+/* Prepares the solve data of a factorisation: d_perm (n, int64) the row
+ * permutation (row i of P A is row perm[i] of A) and d_dinv (the inverses of
+ * the 64x64 diagonal blocks of L and U, rbfx_getrs_dinv_elems(n) elements).
+ * Synchronises the stream (copies ipiv to the host once).                   */
+int64_t rbfx_getrs_dinv_elems(int64_t n);
+int rbfx_getrs_prepare_f64(int64_t n, const double* d_LU, int64_t lda,
+                           const int64_t* d_ipiv, int64_t* d_perm, double* d_dinv,
+                           void* d_work, size_t work_bytes, void* stream);
+int rbfx_getrs_prepare_f32(int64_t n, const float* d_LU, int64_t lda,
+                           const int64_t* d_ipiv, int64_t* d_perm, double* d_dinv,
+                           void* d_work, size_t work_bytes, void* stream);
+
+/* Solves A x = b (trans=0) or A^T x = b (trans=1) in place on d_b (FP64
+ * vector) with FP64 or FP32 factors; the arithmetic is FP64 in both cases
+ * (FP32 factors are applied exactly as stored).  Work size from
+ * rbfx_getrs_workspace.  Replaces the triangular solves of lineax QR and the
+ * inv(A) @ v products (updes/assembly.py:410,430; operators.py:612-616).   */
+size_t rbfx_getrs_workspace(int64_t n);
+int rbfx_getrs_f64(int trans, int64_t n, const double* d_LU, int64_t lda,
+                   const int64_t* d_perm, const double* d_dinv, double* d_b,
+                   void* d_work, size_t work_bytes, void* stream);
+int rbfx_getrs_f32(int trans, int64_t n, const float* d_LU, int64_t lda,
+                   const int64_t* d_perm, const double* d_dinv, double* d_b,
+                   void* d_work, size_t work_bytes, void* stream);
+
+/* ---- K5 FP64 GEMV ---------------------------------------------------------
+ * y = alpha op(A) x + beta y, op = A (trans=0) or A^T (trans=1); A m x n.   */
+size_t rbfx_gemv_workspace(int trans, int64_t m, int64_t n);
+int rbfx_gemv_f64(int trans, int64_t m, int64_t n, double alpha, const double* d_A,
+                  int64_t lda, const double* d_x, double beta, double* d_y,
+                  void* d_work, size_t work_bytes, void* stream);
+
+/* ---- GMRES-based iterative refinement -------------------------------------
+ * Solves op(K) x = b (trans 0/1) in FP64 accuracy using FP32 LU factors of K
+ * as a right preconditioner and FP64 residuals computed either from a stored
+ * FP64 K (d_K64 != NULL) or matrix-free from the problem (d_K64 == NULL).
+ * x holds the result.  Returns RBFX_ERR_STALL (x = best iterate) when the
+ * residual does not reach tol * ||b|| within the iteration limits; the caller
+ * then falls back to the FP64 factorisation.  Synchronises the stream.
+ * No reference counterpart (the reference solves in FP64 only).            */
+typedef struct rbfx_ir_stats {
+  int32_t outer_iters;
+  int32_t inner_iters;
+  double final_relres;   /* ||b - K x|| / ||b||  (FP64)                     */
+  int32_t stalled;
+  int32_t reserved;
+} rbfx_ir_stats;
+size_t rbfx_gmres_ir_workspace(int64_t n, int32_t restart);
+int rbfx_gmres_ir(int trans, int64_t n, const rbfx_problem* p, const double* d_K64,
+                  int64_t ld64, const float* d_LU32, int64_t ld32,
+                  const int64_t* d_perm32, const double* d_dinv32,
+                  const double* d_b, double* d_x, double tol, int32_t restart,
+                  int32_t max_inner, void* d_work, size_t work_bytes,
+                  rbfx_ir_stats* stats, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* RBFX_H */

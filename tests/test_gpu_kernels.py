@@ -1,0 +1,88 @@
+"""GPU tests of the dense linear-algebra kernels through the C ABI."""
+import ctypes
+
+import numpy as np
+import pytest
+import scipy.linalg as sla
+import torch
+
+pytestmark = pytest.mark.gpu
+
+
+def _lib():
+    from updes_amd import _lib as L
+    L.require_gpu()
+    return L
+
+
+@pytest.mark.parametrize("dtype", [torch.float64, torch.float32])
+@pytest.mark.parametrize("m,n,k", [(1, 1, 1), (130, 70, 17), (300, 257, 64), (512, 384, 256)])
+def test_gemm_matches_torch(dtype, m, n, k):
+    L = _lib()
+    g = torch.Generator().manual_seed(m * 7 + n + k)
+    A = torch.randn(m, k, generator=g, dtype=torch.float64)
+    B = torch.randn(k, n, generator=g, dtype=torch.float64)
+    C = torch.randn(m, n, generator=g, dtype=torch.float64)
+    ref = C - A @ B
+    dA, dB, dC = (t.to(dtype).cuda().contiguous() for t in (A, B, C))
+    fn = L.lib().rbfx_gemm_f64 if dtype == torch.float64 else L.lib().rbfx_gemm_f32
+    L.check(fn(m, n, k, -1.0, L.ptr(dA), k, L.ptr(dB), n, 1.0, L.ptr(dC), n, L.stream()))
+    torch.cuda.synchronize()
+    tol = 1e-12 if dtype == torch.float64 else 2e-4
+    err = (dC.double().cpu() - ref).abs().max().item() / max(1.0, ref.abs().max().item())
+    assert err < tol, err
+
+
+def _random_well(n, seed):
+    rng = np.random.default_rng(seed)
+    a = rng.standard_normal((n, n))
+    return a
+
+
+@pytest.mark.parametrize("dtype", [torch.float64, torch.float32])
+@pytest.mark.parametrize("n", [1, 7, 64, 200, 600, 1301])
+def test_getrf_getrs(dtype, n):
+    from updes_amd import engine as E
+    _lib()
+    a = _random_well(n, n)
+    ld = E._ld(n)
+    m = torch.zeros((n, ld), dtype=dtype)
+    m[:, :n] = torch.as_tensor(a).to(dtype)
+    m = m.cuda()
+    lu = E.LU(m, n)
+    assert lu.singular == 0
+    ipiv = lu.ipiv.cpu().numpy()
+    LU = lu.a[:, :n].double().cpu().numpy()
+    Lm = np.tril(LU, -1) + np.eye(n)
+    Um = np.triu(LU)
+    pa = a.astype(np.float64 if dtype == torch.float64 else np.float32).astype(np.float64).copy()
+    for i in range(n):
+        p = ipiv[i]
+        assert i <= p < n
+        pa[[i, p]] = pa[[p, i]]
+    rel = np.abs(Lm @ Um - pa).max() / np.abs(a).max()
+    tol = 1e-13 * n if dtype == torch.float64 else 1e-6 * n
+    assert rel < tol, rel
+    assert np.abs(np.tril(LU, -1)).max() <= 1.0 + 1e-12   # partial pivoting bound
+    b = np.random.default_rng(1).standard_normal(n)
+    for trans in (False, True):
+        x = lu.solve(torch.as_tensor(b).cuda(), trans=trans).cpu().numpy()
+        aa = a if dtype == torch.float64 else a.astype(np.float32).astype(np.float64)
+        res = (aa.T if trans else aa) @ x - b
+        bound = (1e-11 if dtype == torch.float64 else 1e-3) * np.linalg.norm(aa, 1) * np.linalg.norm(x, 1) * max(1, n / 100)
+        assert np.abs(res).max() < bound + 1e-12, (trans, np.abs(res).max(), bound)
+
+
+def test_getrf_matches_lapack_pivots():
+    """Same pivot sequence as LAPACK getrf on a matrix without ties."""
+    from updes_amd import engine as E
+    _lib()
+    n = 300
+    a = _random_well(n, 5)
+    ld = E._ld(n)
+    m = torch.zeros((n, ld), dtype=torch.float64)
+    m[:, :n] = torch.as_tensor(a)
+    lu = E.LU(m.cuda(), n)
+    lu_ref, piv_ref = sla.lu_factor(a)
+    assert np.array_equal(lu.ipiv.cpu().numpy(), piv_ref)
+    assert np.abs(lu.a[:, :n].cpu().numpy() - lu_ref).max() < 1e-10

@@ -1,0 +1,129 @@
+"""GPU parity of assembly, solve and evaluation against the CPU oracle."""
+import numpy as np
+import pytest
+import torch
+
+from oracle import assembly as OA, cloud as OC, rbf as R, solver as OS
+from tests import _cases as C
+
+pytestmark = pytest.mark.gpu
+
+
+def _laplace_op(x, center=None, rbf=None, monomial=None, fields=None):
+    import updes_amd as u
+    return u.nodal_laplacian(x, center, rbf, monomial)
+
+
+def _value_op(x, center=None, rbf=None, monomial=None, fields=None):
+    import updes_amd as u
+    return u.nodal_value(x, center, rbf, monomial)
+
+
+def _rowscaled_err(a, b):
+    scale = np.maximum(np.abs(b).max(axis=1, keepdims=True), 1e-300)
+    return float((np.abs(a - b) / scale).max())
+
+
+@pytest.mark.parametrize("kind,param", C.KINDS)
+@pytest.mark.parametrize("fs", range(len(C.FACET_SETS)))
+def test_assembly_K_matches_oracle(kind, param, fs):
+    import updes_amd as u
+    facets = C.FACET_SETS[fs]
+    nx, ny = 9, 8
+    J = C.jitter(nx, ny, 0.2, seed=fs)
+    oc = OC.SquareCloud(nx, ny, facets, jitter=J)
+    uc = u.SquareCloud(Nx=nx, Ny=ny, facet_types=facets, jitter=J)
+    M = 6
+    betas = {f: np.linspace(0.5, 1.5, len(oc.facet_nodes[f])) for f, t in oc.facet_types.items() if t == "r"}
+    rb = OA.robin_coefficients(oc, betas)
+    Kref = OA.assemble_K(C.adv_diff_desc(0.1, 0.3, (1.0, -0.5)), oc, kind, param, M, rb)
+    rc, _ = u.duplicate_robin_coeffs({f: (0.0, betas[f]) if t == "r" else 0.0 for f, t in uc.facet_types.items()}, uc)
+    dt, nu, vel = 0.1, 0.3, (1.0, -0.5)
+
+    def op(x, center=None, rbf=None, monomial=None, fields=None):
+        val = u.nodal_value(x, center, rbf, monomial)
+        grad = u.nodal_gradient(x, center, rbf, monomial)
+        lap = u.nodal_laplacian(x, center, rbf, monomial)
+        return val + dt * torch.dot(torch.tensor(vel, dtype=torch.float64), grad) - dt * nu * lap
+
+    K = u.assemble_K(op, uc, C.torch_rbf(kind, param), M, None, rc).cpu().numpy()
+    assert K.shape == Kref.shape
+    assert _rowscaled_err(K, Kref) < 1e-12
+
+
+@pytest.mark.parametrize("kind,param", C.KINDS[:3])
+def test_assembly_A_matches_oracle(kind, param):
+    import updes_amd as u
+    oc = OC.SquareCloud(12, 10, C.README_FACETS)
+    uc = u.SquareCloud(Nx=12, Ny=10, facet_types=C.README_FACETS)
+    A = u.assemble_A(uc, C.torch_rbf(kind, param), 3).cpu().numpy()
+    Aref = OA.assemble_A(oc, kind, param, 3)
+    assert _rowscaled_err(A, Aref) < 1e-12
+
+
+def test_readme_laplace_matches_reference_algorithm():
+    """README example (N=600): vals within 1e-8 rel-L2 of the restated reference
+    (inv(A), D inv(A), QR); coefficients agree to the conditioning of A."""
+    import updes_amd as u
+    ft = C.README_FACETS
+    oc = OC.SquareCloud(30, 20, ft)
+    cloud = u.SquareCloud(Nx=30, Ny=20, facet_types=ft)
+    sine = lambda coord: torch.sin(torch.pi * coord[0])
+    zero = lambda coord: 0.0
+    bcs = {"South": zero, "West": zero, "North": sine, "East": zero}
+    sol = u.pde_solver_jit(diff_operator=_laplace_op, rhs_operator=lambda x, centers, rbf, fields: 0.0,
+                           cloud=cloud, boundary_conditions=bcs, rbf=u.polyharmonic, max_degree=1)
+    bc_arr = {f: np.sin(np.pi * oc.sorted_nodes[oc.facet_nodes[f], 0]) if f == "North" else
+              np.zeros(len(oc.facet_nodes[f])) for f in ft}
+    q = OA.assemble_q(oc, 0.0, bc_arr)
+    u_ref, c_ref, B = OS.pde_solver_bpath(C.LAPLACIAN, oc, R.POLYHARMONIC, 1.0, 3, q)
+    vals = sol.vals.cpu().numpy()
+    rel = np.linalg.norm(vals - u_ref) / np.linalg.norm(u_ref)
+    assert rel < 1e-8, rel
+    # coefficients: ill-determined (kappa(A) ~ 2e7); compare through the values they produce
+    A = OA.assemble_A(oc, R.POLYHARMONIC, 1.0, 3)
+    assert np.linalg.norm(A[:600] @ sol.coeffs.cpu().numpy() - u_ref) / np.linalg.norm(u_ref) < 1e-8
+    assert np.linalg.norm(sol.coeffs.cpu().numpy() - c_ref) / np.linalg.norm(c_ref) < 1e-4
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32_ir"])
+def test_poisson_small_both_precisions(precision):
+    import updes_amd as u
+    nx = 30
+    ft = {"North": "d", "South": "d", "East": "d", "West": "d"}
+    J = C.jitter(nx, nx, 0.2, 0)
+    oc = OC.SquareCloud(nx, nx, ft, jitter=J)
+    cloud = u.SquareCloud(Nx=nx, Ny=nx, facet_types=ft, jitter=J)
+    ex = lambda x: torch.sin(torch.pi * x[0]) * torch.sin(torch.pi * x[1])
+    rhs = lambda x, centers, rbf, fields: -2 * torch.pi ** 2 * torch.sin(torch.pi * x[0]) * torch.sin(torch.pi * x[1])
+    sol = u.pde_solver_jit(_laplace_op, rhs, cloud, {f: ex for f in ft}, u.polyharmonic, 2, precision=precision)
+    x, y = oc.sorted_nodes.T
+    q = np.where(np.arange(oc.N) < oc.Ni, -2 * np.pi ** 2 * np.sin(np.pi * x) * np.sin(np.pi * y),
+                 np.sin(np.pi * x) * np.sin(np.pi * y))
+    u_ref, _, _ = OS.pde_solver_bpath(C.LAPLACIAN, oc, R.POLYHARMONIC, 1.0, 6, q)
+    vals = sol.vals.cpu().numpy()
+    assert np.linalg.norm(vals - u_ref) / np.linalg.norm(u_ref) < 1e-8
+    st = u.last_solve_stats()
+    assert st["precision"] == precision
+
+
+def test_eval_value_gradient_laplacian():
+    import updes_amd as u
+    oc = OC.SquareCloud(10, 9, C.README_FACETS, jitter=C.jitter(10, 9, 0.1, 3))
+    for kind, param in C.KINDS:
+        rng = np.random.default_rng(kind)
+        coeffs = rng.standard_normal(oc.N + 6)
+        pts = np.concatenate([rng.uniform(0, 1, (17, 2)), oc.sorted_nodes[:5]])
+        ref = OS.eval_ops(pts, coeffs, oc.sorted_nodes, kind, param)
+        rbf = C.torch_rbf(kind, param)
+        c = torch.as_tensor(coeffs).cuda()
+        ctr = torch.as_tensor(oc.sorted_nodes).cuda()
+        p = torch.as_tensor(pts).cuda()
+        v = u.value_vec(p, c, ctr, rbf).cpu().numpy()
+        g = u.gradient_vec(p, c, ctr, rbf).cpu().numpy()
+        l = u.laplacian_vec(p, c, ctr, rbf).cpu().numpy()
+        sc = np.abs(ref).max(axis=1)
+        assert np.abs(v - ref[R.VAL]).max() <= 1e-12 * max(sc[R.VAL], 1)
+        assert np.abs(g[:, 0] - ref[R.DX]).max() <= 1e-12 * max(sc[R.DX], 1)
+        assert np.abs(g[:, 1] - ref[R.DY]).max() <= 1e-12 * max(sc[R.DY], 1)
+        assert np.abs(l - ref[R.LAP]).max() <= 1e-12 * max(sc[R.LAP], 1)

@@ -1,0 +1,466 @@
+// K1: dense collocation assembly and the matrix-free products built on it.
+//
+// Reference: updes/assembly.py:10-137 (Phi, P, A, op(Phi, P)), 141-362
+// (boundary rows), 366-401 (D = [[opPhi opP];[bdPhi bdP]]).  The reference
+// assembles row by row with fori_loop + scatter over a support table that
+// omits the node itself (updes/cloud.py:110-111), evaluating each entry by
+// autodiff.  Here every entry is a closed form (rbf_device.h) and one launch
+// writes the whole row-major (N+M) x (N+M) matrix: each thread owns two
+// adjacent columns (16-byte non-temporal stores), loops over a tile of rows,
+// and keeps its two centres in registers; the row context (node, normal,
+// operator coefficients) is wave-uniform.  The kernel is bound by HBM writes.
+#include <stdarg.h>
+
+#include "common.h"
+#include "rbf_device.h"
+
+namespace rbfx {
+
+// Device copy of rbfx_problem (by value in the kernel arguments).
+struct ProbDev {
+  int64_t N, M, n;
+  int64_t Ni, Nd, Nn, Nr, Npd;
+  Kern k;
+  const double* x;
+  const double* y;
+  const double* nx;
+  const double* ny;
+  const double* rnx;
+  const double* rny;
+  const double* beta;
+  const int64_t* pd1;
+  const int64_t* pd2;
+  double rc[RBFX_NOPS];
+  double mc[RBFX_NOPS];
+  const double* rrow[RBFX_NOPS];
+  const double* mrow[RBFX_NOPS];
+  int matrix;
+};
+
+enum RowClass { RC_INT = 0, RC_D, RC_N, RC_R, RC_PD, RC_PN, RC_PT, RC_VAL };
+
+struct RowCtx {
+  int cls;
+  int mono;       // RC_PT: monomial id
+  int64_t i;      // node of the row (self column excluded for INT/D/N/R/VAL)
+  double xi, yi;
+  double x2, y2;  // periodic partner
+  double nx, ny;  // normal used by the RBF part (Neumann / Robin(bug) / pn first)
+  double mnx, mny;  // normal used by the monomial part
+  double n2x, n2y;  // pn second normal
+  double beta;
+  OpCoef rc, mc;
+};
+
+__device__ inline OpCoef make_coef(const double* c, const double* const* rows, int64_t i) {
+  OpCoef o;
+  double v[RBFX_NOPS];
+#pragma unroll
+  for (int t = 0; t < RBFX_NOPS; ++t) v[t] = c[t] + (rows[t] ? rows[t][i] : 0.0);
+  o.v = v[RBFX_VAL];
+  o.dx = v[RBFX_DX];
+  o.dy = v[RBFX_DY];
+  o.lap = v[RBFX_LAP];
+  o.dxx = v[RBFX_DXX];
+  o.dyy = v[RBFX_DYY];
+  o.dxy = v[RBFX_DXY];
+  o.hess = (o.dxx != 0.0) || (o.dyy != 0.0) || (o.dxy != 0.0);
+  return o;
+}
+
+__device__ inline RowCtx row_ctx(const ProbDev& P, int64_t r) {
+  RowCtx c;
+  c.mono = 0;
+  c.beta = 0.0;
+  c.nx = c.ny = c.mnx = c.mny = c.n2x = c.n2y = 0.0;
+  c.x2 = c.y2 = 0.0;
+  if (r >= P.N) {
+    c.cls = RC_PT;
+    c.mono = (int)(r - P.N);
+    c.i = -1;
+    c.xi = c.yi = 0.0;
+    return c;
+  }
+  c.i = r;
+  c.xi = P.x[r];
+  c.yi = P.y[r];
+  if (P.matrix == RBFX_MAT_A) {
+    c.cls = RC_VAL;
+    return c;
+  }
+  int64_t t = r;
+  if (t < P.Ni) {
+    c.cls = RC_INT;
+    c.rc = make_coef(P.rc, P.rrow, r);
+    c.mc = make_coef(P.mc, P.mrow, r);
+    return c;
+  }
+  t -= P.Ni;
+  if (t < P.Nd) {
+    c.cls = RC_D;
+    return c;
+  }
+  t -= P.Nd;
+  const int64_t nrm0 = r - P.Ni - P.Nd;  // sorted_outward_normals index of node r
+  if (t < P.Nn) {
+    c.cls = RC_N;
+    c.nx = c.mnx = P.nx[nrm0];
+    c.ny = c.mny = P.ny[nrm0];
+    return c;
+  }
+  t -= P.Nn;
+  if (t < P.Nr) {
+    c.cls = RC_R;
+    c.beta = P.beta[t];
+    c.nx = P.rnx[t];
+    c.ny = P.rny[t];
+    c.mnx = P.nx[nrm0];
+    c.mny = P.ny[nrm0];
+    return c;
+  }
+  t -= P.Nr;
+  const bool flux = t >= P.Npd;
+  if (flux) t -= P.Npd;
+  const int64_t i1 = P.pd1[t], i2 = P.pd2[t];
+  c.cls = flux ? RC_PN : RC_PD;
+  c.i = i1;
+  c.xi = P.x[i1];
+  c.yi = P.y[i1];
+  c.x2 = P.x[i2];
+  c.y2 = P.y[i2];
+  if (flux) {
+    c.nx = P.nx[i1 - P.Ni - P.Nd];
+    c.ny = P.ny[i1 - P.Ni - P.Nd];
+    c.n2x = P.nx[i2 - P.Ni - P.Nd];
+    c.n2y = P.ny[i2 - P.Ni - P.Nd];
+  }
+  return c;
+}
+
+// Entry (row context, centre column j < N).
+__device__ inline double elem_rbf(const ProbDev& P, const RowCtx& c, int64_t j, double xj,
+                                  double yj) {
+  switch (c.cls) {
+    case RC_INT:
+      return j == c.i ? 0.0 : apply_op(P.k, c.rc, c.xi - xj, c.yi - yj);
+    case RC_D:
+    case RC_VAL: {
+      if (j == c.i) return 0.0;
+      const double dx = c.xi - xj, dy = c.yi - yj;
+      return radial_value(P.k, sqrt(dx * dx + dy * dy));
+    }
+    case RC_N: {
+      if (j == c.i) return 0.0;
+      const double dx = c.xi - xj, dy = c.yi - yj;
+      return radial_f1(P.k, sqrt(dx * dx + dy * dy)) * (dx * c.nx + dy * c.ny);
+    }
+    case RC_R: {
+      if (j == c.i) return 0.0;
+      const double dx = c.xi - xj, dy = c.yi - yj;
+      const double r = sqrt(dx * dx + dy * dy);
+      if (r == 0.0) return c.beta * P.k.phi0;
+      double phi, f1, g2;
+      radial(P.k, r, phi, f1, g2);
+      return c.beta * phi + f1 * (dx * c.nx + dy * c.ny);
+    }
+    case RC_PD: {
+      const double dx1 = c.xi - xj, dy1 = c.yi - yj;
+      const double dx2 = c.x2 - xj, dy2 = c.y2 - yj;
+      return radial_value(P.k, sqrt(dx1 * dx1 + dy1 * dy1)) -
+             radial_value(P.k, sqrt(dx2 * dx2 + dy2 * dy2));
+    }
+    case RC_PN: {
+      const double dx1 = c.xi - xj, dy1 = c.yi - yj;
+      const double dx2 = c.x2 - xj, dy2 = c.y2 - yj;
+      return radial_f1(P.k, sqrt(dx1 * dx1 + dy1 * dy1)) * (dx1 * c.nx + dy1 * c.ny) +
+             radial_f1(P.k, sqrt(dx2 * dx2 + dy2 * dy2)) * (dx2 * c.n2x + dy2 * c.n2y);
+    }
+    default:  // RC_PT: row N+k holds p_k(x_j)
+    {
+      OpCoef v{1.0, 0, 0, 0, 0, 0, 0, false};
+      return mono_apply(c.mono, xj, yj, v);
+    }
+  }
+}
+
+// Entry (row context, monomial column N+k).
+__device__ inline double elem_mono(const RowCtx& c, int kid) {
+  double o[RBFX_NOPS];
+  switch (c.cls) {
+    case RC_INT:
+      return mono_apply(kid, c.xi, c.yi, c.mc);
+    case RC_D:
+    case RC_VAL:
+      mono_ops(kid, c.xi, c.yi, o);
+      return o[RBFX_VAL];
+    case RC_N:
+      mono_ops(kid, c.xi, c.yi, o);
+      return o[RBFX_DX] * c.mnx + o[RBFX_DY] * c.mny;
+    case RC_R:
+      mono_ops(kid, c.xi, c.yi, o);
+      return c.beta * o[RBFX_VAL] + o[RBFX_DX] * c.mnx + o[RBFX_DY] * c.mny;
+    case RC_PD: {
+      double o2[RBFX_NOPS];
+      mono_ops(kid, c.xi, c.yi, o);
+      mono_ops(kid, c.x2, c.y2, o2);
+      return o[RBFX_VAL] - o2[RBFX_VAL];
+    }
+    case RC_PN: {
+      double o2[RBFX_NOPS];
+      mono_ops(kid, c.xi, c.yi, o);
+      mono_ops(kid, c.x2, c.y2, o2);
+      return (o[RBFX_DX] * c.nx + o[RBFX_DY] * c.ny) + (o2[RBFX_DX] * c.n2x + o2[RBFX_DY] * c.n2y);
+    }
+    default:
+      return 0.0;  // the M x M block of [P^T 0]
+  }
+}
+
+__device__ inline double elem(const ProbDev& P, const RowCtx& c, int64_t j) {
+  if (j < P.N) return elem_rbf(P, c, j, P.x[j], P.y[j]);
+  return elem_mono(c, (int)(j - P.N));
+}
+
+typedef double vd2 __attribute__((ext_vector_type(2)));
+typedef float vf2 __attribute__((ext_vector_type(2)));
+
+constexpr int kAsmThreads = 256;
+constexpr int kAsmCols = 2 * kAsmThreads;  // columns per block
+constexpr int kAsmRows = 16;               // rows per block
+
+template <bool W64, bool W32>
+__global__ __launch_bounds__(kAsmThreads) void assemble_kernel(ProbDev P, int64_t row0,
+                                                               int64_t nrows, double* out64,
+                                                               int64_t ld64, float* out32,
+                                                               int64_t ld32) {
+  const int64_t j0 = (int64_t)blockIdx.x * kAsmCols + 2 * threadIdx.x;
+  const int64_t rb = (int64_t)blockIdx.y * kAsmRows;
+  if (j0 >= P.n) return;
+  const bool two = (j0 + 1) < P.n;
+  const bool c0rbf = j0 < P.N, c1rbf = (j0 + 1) < P.N;
+  const double xj0 = c0rbf ? P.x[j0] : 0.0, yj0 = c0rbf ? P.y[j0] : 0.0;
+  const double xj1 = c1rbf ? P.x[j0 + 1] : 0.0, yj1 = c1rbf ? P.y[j0 + 1] : 0.0;
+  const int64_t rend = rb + kAsmRows < nrows ? rb + kAsmRows : nrows;
+  for (int64_t rr = rb; rr < rend; ++rr) {
+    const int64_t r = row0 + rr;
+    const RowCtx c = row_ctx(P, r);
+    const double v0 = c0rbf ? elem_rbf(P, c, j0, xj0, yj0) : elem_mono(c, (int)(j0 - P.N));
+    double v1 = 0.0;
+    if (two) v1 = c1rbf ? elem_rbf(P, c, j0 + 1, xj1, yj1) : elem_mono(c, (int)(j0 + 1 - P.N));
+    if (W64) {
+      double* dst = out64 + rr * ld64 + j0;
+      if (two && (((uintptr_t)dst & 15) == 0)) {
+        vd2 v = {v0, v1};
+        __builtin_nontemporal_store(v, reinterpret_cast<vd2*>(dst));
+      } else {
+        __builtin_nontemporal_store(v0, dst);
+        if (two) __builtin_nontemporal_store(v1, dst + 1);
+      }
+    }
+    if (W32) {
+      float* dst = out32 + rr * ld32 + j0;
+      if (two && (((uintptr_t)dst & 7) == 0)) {
+        vf2 v = {(float)v0, (float)v1};
+        __builtin_nontemporal_store(v, reinterpret_cast<vf2*>(dst));
+      } else {
+        __builtin_nontemporal_store((float)v0, dst);
+        if (two) __builtin_nontemporal_store((float)v1, dst + 1);
+      }
+    }
+  }
+}
+
+// y = K x: one block per kMvRows rows; each thread strides over columns.
+constexpr int kMvThreads = 256;
+constexpr int kMvRows = 4;
+
+__global__ __launch_bounds__(kMvThreads) void matvec_kernel(ProbDev P, const double* __restrict__ xv,
+                                                            double* __restrict__ yv) {
+  const int64_t r0 = (int64_t)blockIdx.x * kMvRows;
+  RowCtx c[kMvRows];
+  double acc[kMvRows];
+#pragma unroll
+  for (int q = 0; q < kMvRows; ++q) {
+    const int64_t r = r0 + q < P.n ? r0 + q : P.n - 1;
+    c[q] = row_ctx(P, r);
+    acc[q] = 0.0;
+  }
+  for (int64_t j = threadIdx.x; j < P.n; j += kMvThreads) {
+    const double w = xv[j];
+    if (j < P.N) {
+      const double xj = P.x[j], yj = P.y[j];
+#pragma unroll
+      for (int q = 0; q < kMvRows; ++q) acc[q] += elem_rbf(P, c[q], j, xj, yj) * w;
+    } else {
+#pragma unroll
+      for (int q = 0; q < kMvRows; ++q) acc[q] += elem_mono(c[q], (int)(j - P.N)) * w;
+    }
+  }
+  __shared__ double red[kMvRows][kMvThreads / 64];
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+#pragma unroll
+  for (int q = 0; q < kMvRows; ++q) {
+    double v = acc[q];
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+    if (lane == 0) red[q][wid] = v;
+  }
+  __syncthreads();
+  if (threadIdx.x < kMvRows) {
+    const int q = threadIdx.x;
+    double s = 0.0;
+    for (int w = 0; w < kMvThreads / 64; ++w) s += red[q][w];
+    if (r0 + q < P.n) yv[r0 + q] = s;
+  }
+}
+
+// Partial y^T = x^T K over a chunk of rows: part[chunk][j].
+constexpr int kMtThreads = 256;
+
+__global__ __launch_bounds__(kMtThreads) void matvec_t_kernel(ProbDev P, const double* __restrict__ xv,
+                                                              int64_t rows_per_chunk,
+                                                              double* __restrict__ part) {
+  const int64_t j = (int64_t)blockIdx.x * kMtThreads + threadIdx.x;
+  const int64_t rs = (int64_t)blockIdx.y * rows_per_chunk;
+  const int64_t re = rs + rows_per_chunk < P.n ? rs + rows_per_chunk : P.n;
+  const bool valid = j < P.n;
+  const int64_t jj = valid ? j : 0;
+  const bool isrbf = jj < P.N;
+  const double xj = isrbf ? P.x[jj] : 0.0, yj = isrbf ? P.y[jj] : 0.0;
+  double acc = 0.0;
+  for (int64_t r = rs; r < re; ++r) {
+    const RowCtx c = row_ctx(P, r);
+    const double e = isrbf ? elem_rbf(P, c, jj, xj, yj) : elem_mono(c, (int)(jj - P.N));
+    acc += e * xv[r];
+  }
+  if (valid) part[blockIdx.y * P.n + j] = acc;
+}
+
+__global__ void sum_chunks_kernel(const double* __restrict__ part, int64_t chunks, int64_t n,
+                                  double* __restrict__ y) {
+  const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= n) return;
+  double s = 0.0;
+  for (int64_t c = 0; c < chunks; ++c) s += part[c * n + j];
+  y[j] = s;
+}
+
+static int make_prob(const rbfx_problem* p, int matrix, ProbDev& P) {
+  RBFX_REQUIRE(p != nullptr, "null problem");
+  RBFX_REQUIRE(p->N > 0 && p->M >= 0 && p->M <= 15, "bad N=%lld / M=%lld", (long long)p->N,
+               (long long)p->M);
+  RBFX_REQUIRE(p->kernel >= 0 && p->kernel <= 4, "unknown kernel %d", p->kernel);
+  RBFX_REQUIRE(matrix == RBFX_MAT_K || matrix == RBFX_MAT_A, "unknown matrix %d", matrix);
+  RBFX_REQUIRE(p->d_x && p->d_y, "node coordinates missing");
+  P.N = p->N;
+  P.M = p->M;
+  P.n = p->N + p->M;
+  P.Ni = p->Ni;
+  P.Nd = p->Nd;
+  P.Nn = p->Nn;
+  P.Nr = p->Nr;
+  P.Npd = p->Npd;
+  P.k = make_kern(p->kernel, p->param);
+  P.x = p->d_x;
+  P.y = p->d_y;
+  P.nx = p->d_nx;
+  P.ny = p->d_ny;
+  P.rnx = p->d_robin_normal_x;
+  P.rny = p->d_robin_normal_y;
+  P.beta = p->d_beta;
+  P.pd1 = p->d_pd_i1;
+  P.pd2 = p->d_pd_i2;
+  for (int t = 0; t < RBFX_NOPS; ++t) {
+    P.rc[t] = p->op_rbf_const[t];
+    P.mc[t] = p->op_mon_const[t];
+    P.rrow[t] = p->d_op_rbf_row[t];
+    P.mrow[t] = p->d_op_mon_row[t];
+  }
+  P.matrix = matrix;
+  if (matrix == RBFX_MAT_K) {
+    RBFX_REQUIRE(p->Ni + p->Nd + p->Nn + p->Nr + 2 * p->Npd == p->N,
+                 "row classes do not add up: Ni+Nd+Nn+Nr+2Npd=%lld != N=%lld",
+                 (long long)(p->Ni + p->Nd + p->Nn + p->Nr + 2 * p->Npd), (long long)p->N);
+    RBFX_REQUIRE(p->Ni >= 0 && p->Nd >= 0 && p->Nn >= 0 && p->Nr >= 0 && p->Npd >= 0,
+                 "negative row-class size");
+    RBFX_REQUIRE((p->Nn + p->Nr + p->Npd == 0) || (p->d_nx && p->d_ny), "normals missing");
+    RBFX_REQUIRE(p->Nr == 0 || (p->d_beta && p->d_robin_normal_x && p->d_robin_normal_y),
+                 "Robin data missing");
+    RBFX_REQUIRE(p->Npd == 0 || (p->d_pd_i1 && p->d_pd_i2), "periodic pairs missing");
+  }
+  return RBFX_OK;
+}
+
+}  // namespace rbfx
+
+using namespace rbfx;
+
+extern "C" int rbfx_assemble(const rbfx_problem* p, int matrix, int64_t row0, int64_t nrows,
+                             double* d_out64, int64_t ld64, float* d_out32, int64_t ld32,
+                             void* stream) {
+  ProbDev P;
+  int st = make_prob(p, matrix, P);
+  if (st) return st;
+  RBFX_REQUIRE(row0 >= 0 && nrows >= 0 && row0 + nrows <= P.n, "row range out of bounds");
+  RBFX_REQUIRE(d_out64 || d_out32, "no output");
+  RBFX_REQUIRE(!d_out64 || ld64 >= P.n, "ld64 < n");
+  RBFX_REQUIRE(!d_out32 || ld32 >= P.n, "ld32 < n");
+  if (nrows == 0) return RBFX_OK;
+  dim3 grid((unsigned)ceil_div(P.n, kAsmCols), (unsigned)ceil_div(nrows, kAsmRows));
+  RBFX_REQUIRE(grid.y <= 65535u * 1024u, "too many rows");
+  hipStream_t s = as_stream(stream);
+  if (d_out64 && d_out32)
+    hipLaunchKernelGGL((assemble_kernel<true, true>), grid, dim3(kAsmThreads), 0, s, P, row0,
+                       nrows, d_out64, ld64, d_out32, ld32);
+  else if (d_out64)
+    hipLaunchKernelGGL((assemble_kernel<true, false>), grid, dim3(kAsmThreads), 0, s, P, row0,
+                       nrows, d_out64, ld64, d_out32, ld32);
+  else
+    hipLaunchKernelGGL((assemble_kernel<false, true>), grid, dim3(kAsmThreads), 0, s, P, row0,
+                       nrows, d_out64, ld64, d_out32, ld32);
+  RBFX_CHECK_LAUNCH();
+  return RBFX_OK;
+}
+
+static int64_t mt_chunks(int64_t n) {
+  // enough blocks to fill the chip: ceil(n/256) column blocks x chunks >= 4 * CUs
+  const int64_t colb = ceil_div(n, kMtThreads);
+  int64_t want = ceil_div(4 * (int64_t)device_info().num_cus, colb);
+  if (want < 1) want = 1;
+  if (want > n) want = n;
+  return want;
+}
+
+extern "C" size_t rbfx_matvec_workspace(const rbfx_problem* p, int trans) {
+  if (!p || !trans) return 0;
+  const int64_t n = p->N + p->M;
+  return ws_round(sizeof(double) * (size_t)mt_chunks(n) * (size_t)n);
+}
+
+extern "C" int rbfx_matvec(const rbfx_problem* p, int matrix, int trans, const double* d_x,
+                           double* d_y, void* d_work, size_t work_bytes, void* stream) {
+  ProbDev P;
+  int st = make_prob(p, matrix, P);
+  if (st) return st;
+  RBFX_REQUIRE(d_x && d_y && d_x != d_y, "matvec needs distinct x and y");
+  hipStream_t s = as_stream(stream);
+  if (!trans) {
+    hipLaunchKernelGGL(matvec_kernel, dim3((unsigned)ceil_div(P.n, kMvRows)), dim3(kMvThreads), 0,
+                       s, P, d_x, d_y);
+    RBFX_CHECK_LAUNCH();
+    return RBFX_OK;
+  }
+  const int64_t chunks = mt_chunks(P.n);
+  RBFX_REQUIRE(d_work && work_bytes >= sizeof(double) * (size_t)chunks * (size_t)P.n,
+               "matvec workspace too small (%zu < %zu)", work_bytes,
+               sizeof(double) * (size_t)chunks * (size_t)P.n);
+  const int64_t rpc = ceil_div(P.n, chunks);
+  const int64_t used_chunks = ceil_div(P.n, rpc);
+  double* part = static_cast<double*>(d_work);
+  hipLaunchKernelGGL(matvec_t_kernel, dim3((unsigned)ceil_div(P.n, kMtThreads), (unsigned)used_chunks),
+                     dim3(kMtThreads), 0, s, P, d_x, rpc, part);
+  RBFX_CHECK_LAUNCH();
+  hipLaunchKernelGGL(sum_chunks_kernel, dim3((unsigned)ceil_div(P.n, 256)), dim3(256), 0, s, part,
+                     used_chunks, P.n, d_y);
+  RBFX_CHECK_LAUNCH();
+  return RBFX_OK;
+}

@@ -1,0 +1,77 @@
+// Internal helpers shared by the rbfx translation units (gfx950 only).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <stdio.h>
+#include <string>
+
+#include "../../include/rbfx.h"
+
+namespace rbfx {
+
+// Thread-local last error message (rbfx_last_error).
+void set_error(const char* fmt, ...);
+const char* last_error();
+
+#define RBFX_HIP(call)                                                        \
+  do {                                                                        \
+    hipError_t e_ = (call);                                                   \
+    if (e_ != hipSuccess) {                                                   \
+      ::rbfx::set_error("%s failed: %s (%s:%d)", #call, hipGetErrorString(e_), \
+                        __FILE__, __LINE__);                                  \
+      return RBFX_ERR_HIP;                                                    \
+    }                                                                         \
+  } while (0)
+
+#define RBFX_CHECK_LAUNCH()                                                   \
+  do {                                                                        \
+    hipError_t e_ = hipGetLastError();                                        \
+    if (e_ != hipSuccess) {                                                   \
+      ::rbfx::set_error("kernel launch failed: %s (%s:%d)",                   \
+                        hipGetErrorString(e_), __FILE__, __LINE__);           \
+      return RBFX_ERR_HIP;                                                    \
+    }                                                                         \
+  } while (0)
+
+#define RBFX_REQUIRE(cond, ...)                                               \
+  do {                                                                        \
+    if (!(cond)) {                                                            \
+      ::rbfx::set_error(__VA_ARGS__);                                         \
+      return RBFX_ERR_ARG;                                                    \
+    }                                                                         \
+  } while (0)
+
+struct DeviceInfo {
+  int device = -1;
+  int num_cus = 0;
+  size_t lds_per_block = 0;
+};
+// Cached per current device.
+const DeviceInfo& device_info();
+
+inline hipStream_t as_stream(void* s) { return reinterpret_cast<hipStream_t>(s); }
+
+__host__ __device__ inline int64_t ceil_div(int64_t a, int64_t b) { return (a + b - 1) / b; }
+
+// Bump allocator over a caller-provided workspace (256-byte aligned slices).
+struct Workspace {
+  char* base;
+  size_t size;
+  size_t used = 0;
+  Workspace(void* p, size_t n) : base(static_cast<char*>(p)), size(n) {}
+  template <typename T>
+  T* take(size_t count) {
+    size_t off = (used + 255) & ~size_t(255);
+    size_t bytes = count * sizeof(T);
+    if (off + bytes > size) return nullptr;
+    used = off + bytes;
+    return reinterpret_cast<T*>(base + off);
+  }
+};
+inline size_t ws_round(size_t bytes) { return (bytes + 255) & ~size_t(255); }
+
+// Spin limit of every bounded device-side wait (each iteration is one
+// device-scope load plus a short sleep, ~1 us): about 4 s in total.
+constexpr uint32_t kSpinLimit = 1u << 22;
+
+}  // namespace rbfx

@@ -1,0 +1,384 @@
+// K5 FP64 GEMV and the GMRES-based iterative refinement driver.
+//
+// The reference solves B u = q with lineax QR in FP64 only
+// (updes/operators.py:612-613).  Here the collocation system K c = [q; 0] can
+// be factored in FP32 (MFMA) and the FP64 solution recovered by GMRES-IR
+// (Carson & Higham, SIAM J. Sci. Comput. 40(2), 2018): the FP32 factors
+// right-precondition restarted GMRES on the correction equation, residuals
+// b - K x are formed in FP64 either from the stored FP64 matrix (GEMV) or
+// matrix-free from the node coordinates (rbfx_matvec), and refinement stops
+// with RBFX_ERR_STALL when it fails to reach the tolerance so that the caller
+// can fall back to an FP64 factorisation.
+#include <math.h>
+
+#include <algorithm>
+#include <vector>
+
+#include "common.h"
+
+namespace rbfx {
+
+constexpr int kGvThreads = 256;
+
+// y[i] = alpha * sum_j A[i][j] x[j] + beta * y[i]: one block per row.
+__global__ __launch_bounds__(kGvThreads) void gemv_n_block_kernel(int64_t m, int64_t n, double alpha,
+                                                                  const double* __restrict__ A, int64_t lda,
+                                                                  const double* __restrict__ x, double beta,
+                                                                  double* __restrict__ y) {
+  const int64_t i = blockIdx.x;
+  const double* row = A + i * lda;
+  double s = 0.0;
+  for (int64_t j = threadIdx.x; j < n; j += kGvThreads) s += row[j] * x[j];
+  __shared__ double red[kGvThreads / 64];
+  for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off, 64);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double t = red[0] + red[1] + red[2] + red[3];
+    y[i] = alpha * t + (beta != 0.0 ? beta * y[i] : 0.0);
+  }
+}
+
+// Same, one wave per row (many rows): 2 doubles per lane per step.
+__global__ __launch_bounds__(kGvThreads) void gemv_n_wave_kernel(int64_t m, int64_t n, double alpha,
+                                                                 const double* __restrict__ A, int64_t lda,
+                                                                 const double* __restrict__ x, double beta,
+                                                                 double* __restrict__ y) {
+  const int64_t i = (int64_t)blockIdx.x * (kGvThreads / 64) + (threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  if (i >= m) return;
+  const double* row = A + i * lda;
+  double s = 0.0;
+  const bool vec = ((((uintptr_t)row) | ((uintptr_t)x)) & 15) == 0;
+  if (vec) {
+    const int64_t n2 = n / 2;
+    typedef double vd2 __attribute__((ext_vector_type(2)));
+    const vd2* r2 = reinterpret_cast<const vd2*>(row);
+    const vd2* x2 = reinterpret_cast<const vd2*>(x);
+    for (int64_t j = lane; j < n2; j += 64) {
+      const vd2 a = __builtin_nontemporal_load(r2 + j);
+      const vd2 b = x2[j];
+      s += a.x * b.x + a.y * b.y;
+    }
+    if ((n & 1) && lane == 0) s += row[n - 1] * x[n - 1];
+  } else {
+    for (int64_t j = lane; j < n; j += 64) s += row[j] * x[j];
+  }
+  for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off, 64);
+  if (lane == 0) y[i] = alpha * s + (beta != 0.0 ? beta * y[i] : 0.0);
+}
+
+// y[j] = alpha * sum_i A[i][j] x[i] + beta * y[j]: thread per column over a
+// chunk of rows; partial sums reduced by a second kernel when chunked.
+__global__ __launch_bounds__(kGvThreads) void gemv_t_kernel(int64_t m, int64_t n, const double* __restrict__ A,
+                                                            int64_t lda, const double* __restrict__ x,
+                                                            int64_t rows_per_chunk, double* __restrict__ part) {
+  const int64_t j = (int64_t)blockIdx.x * kGvThreads + threadIdx.x;
+  if (j >= n) return;
+  const int64_t rs = (int64_t)blockIdx.y * rows_per_chunk;
+  const int64_t re = rs + rows_per_chunk < m ? rs + rows_per_chunk : m;
+  double s = 0.0;
+  for (int64_t i = rs; i < re; ++i) s += A[i * lda + j] * x[i];
+  part[(int64_t)blockIdx.y * n + j] = s;
+}
+
+__global__ void gemv_t_finish_kernel(int64_t n, int64_t chunks, const double* __restrict__ part,
+                                     double alpha, double beta, double* __restrict__ y) {
+  const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= n) return;
+  double s = 0.0;
+  for (int64_t c = 0; c < chunks; ++c) s += part[c * n + j];
+  y[j] = alpha * s + (beta != 0.0 ? beta * y[j] : 0.0);
+}
+
+static int64_t gemv_t_chunks(int64_t m, int64_t n) {
+  const int64_t colb = ceil_div(n, kGvThreads);
+  int64_t want = ceil_div(4 * (int64_t)device_info().num_cus, colb);
+  want = std::max<int64_t>(1, std::min<int64_t>(want, ceil_div(m, 64)));
+  return want;
+}
+
+size_t gemv_workspace(int trans, int64_t m, int64_t n) {
+  if (!trans) return 0;
+  return ws_round(sizeof(double) * (size_t)gemv_t_chunks(m, n) * (size_t)n);
+}
+
+int gemv(int trans, int64_t m, int64_t n, double alpha, const double* A, int64_t lda, const double* x,
+         double beta, double* y, double* part, hipStream_t s) {
+  if (m <= 0 || n <= 0) return RBFX_OK;
+  if (!trans) {
+    if (m < 2048) {
+      hipLaunchKernelGGL(gemv_n_block_kernel, dim3((unsigned)m), dim3(kGvThreads), 0, s, m, n, alpha, A,
+                         lda, x, beta, y);
+    } else {
+      hipLaunchKernelGGL(gemv_n_wave_kernel, dim3((unsigned)ceil_div(m, kGvThreads / 64)), dim3(kGvThreads),
+                         0, s, m, n, alpha, A, lda, x, beta, y);
+    }
+    RBFX_CHECK_LAUNCH();
+    return RBFX_OK;
+  }
+  const int64_t chunks = gemv_t_chunks(m, n);
+  const int64_t rpc = ceil_div(m, chunks);
+  const int64_t used = ceil_div(m, rpc);
+  RBFX_REQUIRE(part != nullptr, "gemv^T needs a workspace");
+  hipLaunchKernelGGL(gemv_t_kernel, dim3((unsigned)ceil_div(n, kGvThreads), (unsigned)used), dim3(kGvThreads),
+                     0, s, m, n, A, lda, x, rpc, part);
+  RBFX_CHECK_LAUNCH();
+  hipLaunchKernelGGL(gemv_t_finish_kernel, dim3((unsigned)ceil_div(n, 256)), dim3(256), 0, s, n, used, part,
+                     alpha, beta, y);
+  RBFX_CHECK_LAUNCH();
+  return RBFX_OK;
+}
+
+// ---- small vector kernels ------------------------------------------------
+// Deterministic dot product: fixed block partials then one block.
+constexpr int kDotBlocks = 256;
+__global__ __launch_bounds__(256) void dot_partial_kernel(int64_t n, const double* __restrict__ a,
+                                                          const double* __restrict__ b,
+                                                          double* __restrict__ part) {
+  double s = 0.0;
+  for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256)
+    s += a[i] * b[i];
+  __shared__ double red[4];
+  for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off, 64);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) part[blockIdx.x] = red[0] + red[1] + red[2] + red[3];
+}
+__global__ __launch_bounds__(256) void dot_final_kernel(int nparts, const double* __restrict__ part,
+                                                        double* __restrict__ out) {
+  double s = 0.0;
+  for (int i = threadIdx.x; i < nparts; i += 256) s += part[i];
+  __shared__ double red[4];
+  for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off, 64);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) *out = red[0] + red[1] + red[2] + red[3];
+}
+
+// y = a*x + b*y
+__global__ void axpby_kernel(int64_t n, double a, const double* __restrict__ x, double b,
+                             double* __restrict__ y) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) y[i] = a * x[i] + (b != 0.0 ? b * y[i] : 0.0);
+}
+// y = a*x (scaled copy)
+__global__ void scale_copy_kernel(int64_t n, double a, const double* __restrict__ x, double* __restrict__ y) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) y[i] = a * x[i];
+}
+
+static int dot(int64_t n, const double* a, const double* b, double* part, double* out, hipStream_t s) {
+  hipLaunchKernelGGL(dot_partial_kernel, dim3(kDotBlocks), dim3(256), 0, s, n, a, b, part);
+  RBFX_CHECK_LAUNCH();
+  hipLaunchKernelGGL(dot_final_kernel, dim3(1), dim3(256), 0, s, kDotBlocks, part, out);
+  RBFX_CHECK_LAUNCH();
+  return RBFX_OK;
+}
+
+}  // namespace rbfx
+
+using namespace rbfx;
+
+extern "C" size_t rbfx_gemv_workspace(int trans, int64_t m, int64_t n) {
+  if (m <= 0 || n <= 0) return 0;
+  return gemv_workspace(trans, m, n);
+}
+
+extern "C" int rbfx_gemv_f64(int trans, int64_t m, int64_t n, double alpha, const double* d_A, int64_t lda,
+                             const double* d_x, double beta, double* d_y, void* d_work, size_t work_bytes,
+                             void* stream) {
+  RBFX_REQUIRE(m >= 0 && n >= 0 && lda >= n, "bad gemv shape");
+  RBFX_REQUIRE(d_A && d_x && d_y && d_x != d_y, "bad gemv pointers");
+  RBFX_REQUIRE(work_bytes >= gemv_workspace(trans, m, n) && (!trans || d_work), "gemv workspace too small");
+  return gemv(trans, m, n, alpha, d_A, lda, d_x, beta, d_y, static_cast<double*>(d_work), as_stream(stream));
+}
+
+extern "C" size_t rbfx_gmres_ir_workspace(int64_t n, int32_t restart) {
+  if (n <= 0 || restart <= 0) return 0;
+  size_t b = 0;
+  b += ws_round(sizeof(double) * (size_t)(restart + 1) * (size_t)n);  // V
+  b += 4 * ws_round(sizeof(double) * (size_t)n);                      // r, w, z, u
+  b += ws_round(sizeof(double) * (size_t)(2 * restart + 8));          // h, h2, norms
+  b += ws_round(sizeof(double) * kDotBlocks);                         // dot partials
+  b += ws_round(rbfx_getrs_workspace(n));
+  const size_t gw = std::max(gemv_workspace(1, restart + 1, n), gemv_workspace(1, n, n));
+  b += ws_round(gw);
+  return b + 4096;
+}
+
+namespace {
+struct IrOps {
+  int trans;
+  int64_t n;
+  const rbfx_problem* p;
+  const double* K64;
+  int64_t ld64;
+  const float* LU32;
+  int64_t ld32;
+  const int64_t* perm;
+  const double* dinv;
+  void* ts_work;
+  size_t ts_bytes;
+  double* part;       // gemv^T / matvec^T partials
+  size_t part_bytes;
+  hipStream_t s;
+  int applyK(const double* v, double* w) const {
+    if (K64) return gemv(trans, n, n, 1.0, K64, ld64, v, 0.0, w, part, s);
+    return rbfx_matvec(p, RBFX_MAT_K, trans, v, w, part, part_bytes, s);
+  }
+  int applyMinv(double* v) const {
+    return rbfx_getrs_f32(trans, n, LU32, ld32, perm, dinv, v, ts_work, ts_bytes, s);
+  }
+};
+}  // namespace
+
+extern "C" int rbfx_gmres_ir(int trans, int64_t n, const rbfx_problem* p, const double* d_K64, int64_t ld64,
+                             const float* d_LU32, int64_t ld32, const int64_t* d_perm32,
+                             const double* d_dinv32, const double* d_b, double* d_x, double tol,
+                             int32_t restart, int32_t max_inner, void* d_work, size_t work_bytes,
+                             rbfx_ir_stats* stats, void* stream) {
+  RBFX_REQUIRE(n > 0 && restart > 0 && max_inner > 0 && tol > 0.0, "bad gmres_ir parameters");
+  RBFX_REQUIRE(d_LU32 && d_perm32 && d_dinv32 && d_b && d_x && d_b != d_x, "bad gmres_ir pointers");
+  RBFX_REQUIRE(d_K64 || p, "need a stored K or a problem for the residual");
+  RBFX_REQUIRE(!d_K64 || ld64 >= n, "ld64 < n");
+  RBFX_REQUIRE(!p || p->N + p->M == n, "problem size != n");
+  RBFX_REQUIRE(work_bytes >= rbfx_gmres_ir_workspace(n, restart), "gmres_ir workspace too small");
+  hipStream_t s = as_stream(stream);
+  const int m = restart;
+  Workspace ws(d_work, work_bytes);
+  double* V = ws.take<double>((size_t)(m + 1) * n);
+  double* r = ws.take<double>(n);
+  double* w = ws.take<double>(n);
+  double* z = ws.take<double>(n);
+  double* u = ws.take<double>(n);
+  double* hd = ws.take<double>(2 * m + 8);
+  double* dpart = ws.take<double>(kDotBlocks);
+  const size_t tsb = rbfx_getrs_workspace(n);
+  void* tsw = ws.take<char>(tsb);
+  const size_t gw = std::max(gemv_workspace(1, m + 1, n), gemv_workspace(1, n, n));
+  double* part = ws.take<double>(gw / sizeof(double) + 1);
+  RBFX_REQUIRE(part != nullptr, "gmres_ir workspace carve failed");
+  IrOps ops{trans, n, p, d_K64, ld64, d_LU32, ld32, d_perm32, d_dinv32, tsw, tsb, part, gw, s};
+  if (!d_K64) {
+    RBFX_REQUIRE(gw >= rbfx_matvec_workspace(p, trans), "matvec workspace too small");
+  }
+  const unsigned g = (unsigned)ceil_div(n, 256);
+  int st;
+  rbfx_ir_stats loc{0, 0, 0.0, 0, 0};
+
+  auto host_dot = [&](const double* a, const double* b, double& out) -> int {
+    int e = dot(n, a, b, dpart, hd + 2 * m + 6, s);
+    if (e) return e;
+    RBFX_HIP(hipMemcpyAsync(&out, hd + 2 * m + 6, sizeof(double), hipMemcpyDeviceToHost, s));
+    RBFX_HIP(hipStreamSynchronize(s));
+    return RBFX_OK;
+  };
+
+  double bn2 = 0.0;
+  if ((st = host_dot(d_b, d_b, bn2))) return st;
+  const double bn = sqrt(bn2);
+  if (bn == 0.0) {
+    RBFX_HIP(hipMemsetAsync(d_x, 0, n * sizeof(double), s));
+    RBFX_HIP(hipStreamSynchronize(s));
+    if (stats) *stats = loc;
+    return RBFX_OK;
+  }
+  // x0 = M^-1 b
+  RBFX_HIP(hipMemcpyAsync(d_x, d_b, n * sizeof(double), hipMemcpyDeviceToDevice, s));
+  if ((st = ops.applyMinv(d_x))) return st;
+
+  std::vector<double> H((size_t)(m + 1) * m), cs(m), sn(m), gv(m + 1), hh(2 * m + 4), y(m);
+  double prev = INFINITY;
+  int stalls = 0;
+  int result = RBFX_ERR_STALL;
+  for (int outer = 0;; ++outer) {
+    // r = b - K x
+    if ((st = ops.applyK(d_x, w))) return st;
+    hipLaunchKernelGGL(scale_copy_kernel, dim3(g), dim3(256), 0, s, n, 1.0, d_b, r);
+    hipLaunchKernelGGL(axpby_kernel, dim3(g), dim3(256), 0, s, n, -1.0, w, 1.0, r);
+    RBFX_CHECK_LAUNCH();
+    double rn2 = 0.0;
+    if ((st = host_dot(r, r, rn2))) return st;
+    const double rn = sqrt(rn2);
+    const double rel = rn / bn;
+    loc.outer_iters = outer;
+    loc.final_relres = rel;
+    if (!(rel == rel)) break;  // NaN
+    if (rel <= tol) {
+      result = RBFX_OK;
+      break;
+    }
+    if (loc.inner_iters >= max_inner) break;
+    if (rel > 0.5 * prev) {
+      if (++stalls >= 3) break;
+    }
+    prev = std::min(prev, rel);
+
+    // GMRES(m) on K M^-1 y = r
+    hipLaunchKernelGGL(scale_copy_kernel, dim3(g), dim3(256), 0, s, n, 1.0 / rn, r, V);
+    RBFX_CHECK_LAUNCH();
+    std::fill(gv.begin(), gv.end(), 0.0);
+    gv[0] = rn;
+    const double inner_tol = std::max(0.1 * tol * bn, 1e-10 * rn);
+    int kdone = 0;
+    for (int k = 0; k < m; ++k) {
+      double* vk = V + (size_t)k * n;
+      RBFX_HIP(hipMemcpyAsync(z, vk, n * sizeof(double), hipMemcpyDeviceToDevice, s));
+      if ((st = ops.applyMinv(z))) return st;
+      if ((st = ops.applyK(z, w))) return st;
+      // classical Gram-Schmidt, two passes
+      double* h1 = hd;
+      double* h2 = hd + (m + 1);
+      if ((st = gemv(0, k + 1, n, 1.0, V, n, w, 0.0, h1, nullptr, s))) return st;
+      if ((st = gemv(1, k + 1, n, -1.0, V, n, h1, 1.0, w, part, s))) return st;
+      if ((st = gemv(0, k + 1, n, 1.0, V, n, w, 0.0, h2, nullptr, s))) return st;
+      if ((st = gemv(1, k + 1, n, -1.0, V, n, h2, 1.0, w, part, s))) return st;
+      if ((st = dot(n, w, w, dpart, hd + 2 * m + 4, s))) return st;
+      RBFX_HIP(hipMemcpyAsync(hh.data(), hd, (2 * m + 6) * sizeof(double), hipMemcpyDeviceToHost, s));
+      RBFX_HIP(hipStreamSynchronize(s));
+      const double hn = sqrt(hh[2 * m + 4]);
+      double* col = &H[(size_t)k * (m + 1)];
+      for (int i = 0; i <= k; ++i) col[i] = hh[i] + hh[m + 1 + i];
+      col[k + 1] = hn;
+      ++loc.inner_iters;
+      kdone = k + 1;
+      if (hn > 0.0) {
+        hipLaunchKernelGGL(scale_copy_kernel, dim3(g), dim3(256), 0, s, n, 1.0 / hn, w, V + (size_t)(k + 1) * n);
+        RBFX_CHECK_LAUNCH();
+      }
+      // Givens rotations
+      for (int i = 0; i < k; ++i) {
+        const double t = cs[i] * col[i] + sn[i] * col[i + 1];
+        col[i + 1] = -sn[i] * col[i] + cs[i] * col[i + 1];
+        col[i] = t;
+      }
+      const double den = hypot(col[k], col[k + 1]);
+      cs[k] = den > 0.0 ? col[k] / den : 1.0;
+      sn[k] = den > 0.0 ? col[k + 1] / den : 0.0;
+      col[k] = den;
+      col[k + 1] = 0.0;
+      gv[k + 1] = -sn[k] * gv[k];
+      gv[k] = cs[k] * gv[k];
+      if (fabs(gv[k + 1]) <= inner_tol || hn == 0.0 || loc.inner_iters >= max_inner) break;
+    }
+    // y = R^-1 g
+    for (int i = kdone - 1; i >= 0; --i) {
+      double t = gv[i];
+      for (int j = i + 1; j < kdone; ++j) t -= H[(size_t)j * (m + 1) + i] * y[j];
+      y[i] = H[(size_t)i * (m + 1) + i] != 0.0 ? t / H[(size_t)i * (m + 1) + i] : 0.0;
+    }
+    // u = V^T y ; x += M^-1 u
+    RBFX_HIP(hipMemcpyAsync(hd, y.data(), kdone * sizeof(double), hipMemcpyHostToDevice, s));
+    if ((st = gemv(1, kdone, n, 1.0, V, n, hd, 0.0, u, part, s))) return st;
+    if ((st = ops.applyMinv(u))) return st;
+    hipLaunchKernelGGL(axpby_kernel, dim3(g), dim3(256), 0, s, n, 1.0, u, 1.0, d_x);
+    RBFX_CHECK_LAUNCH();
+    RBFX_HIP(hipStreamSynchronize(s));  // y (host) must outlive the copy
+  }
+  loc.stalled = result == RBFX_OK ? 0 : 1;
+  if (stats) *stats = loc;
+  if (result != RBFX_OK) set_error("GMRES-IR stalled: relres %.3e after %d inner iterations", loc.final_relres,
+                                   loc.inner_iters);
+  return result;
+}

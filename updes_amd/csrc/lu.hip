@@ -1,0 +1,774 @@
+// K2/K3: right-looking blocked LU with partial pivoting (FP64 and FP32) for
+// row-major matrices on gfx950.
+//
+// Replaces the factorisations that the reference runs through XLA/LAPACK:
+// jnp.linalg.inv (updes/assembly.py:90, getrf + getri) and lineax QR
+// (updes/operators.py:612-613).  The augmented collocation matrix is factored
+// once: P K = L U.
+//
+// Structure per outer panel of NB columns (k0 .. k0+NB):
+//   1. recursive panel factorisation: halves down to WMAX-wide leaves; each
+//      leaf runs as ONE cooperative launch whose workgroups keep their rows of
+//      the leaf in LDS and agree on every pivot through a grid barrier (one
+//      barrier per column: every workgroup publishes its best row, all of them
+//      pick the same winner);
+//   2. row interchanges of the other columns through a gather/scatter plan
+//      (row-major rows are contiguous, so a swap is a coalesced copy);
+//   3. U12 = L11^-1 A12 with a strip-parallel forward substitution;
+//   4. trailing update A22 -= L21 U12 with an MFMA GEMM
+//      (v_mfma_f64_16x16x4f64 / v_mfma_f32_16x16x4f32, 128x128 tiles,
+//      XCD-aware tile order).
+#include <algorithm>
+#include <vector>
+
+#include "common.h"
+
+namespace rbfx {
+
+typedef double v4d __attribute__((ext_vector_type(4)));
+typedef float v4f __attribute__((ext_vector_type(4)));
+
+template <typename T>
+struct Mma;
+template <>
+struct Mma<double> {
+  typedef v4d acc_t;
+  static constexpr int BK = 16;
+  __device__ static inline acc_t mma(double a, double b, acc_t c) {
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+  }
+};
+template <>
+struct Mma<float> {
+  typedef v4f acc_t;
+  static constexpr int BK = 32;
+  __device__ static inline acc_t mma(float a, float b, acc_t c) {
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+  }
+};
+
+// ---------------------------------------------------------------------------
+// GEMM: C = beta*C + alpha*A*B, row-major, beta in {0,1}.
+// Block tile 128x128, 4 waves (2x2), wave tile 64x64 = 4x4 MFMA 16x16 tiles.
+// MFMA 16x16x4 operand layout: lane l holds A[l%16][l/16] and B[l/16][l%16];
+// the accumulator holds D[4*(l/16)+v][l%16], v = 0..3.
+constexpr int GBM = 128, GBN = 128, GTHREADS = 256;
+
+template <typename T>
+__global__ __launch_bounds__(GTHREADS) void gemm_kernel(int64_t m, int64_t n, int64_t k, T alpha,
+                                                        const T* __restrict__ A, int64_t lda,
+                                                        const T* __restrict__ B, int64_t ldb,
+                                                        T beta, T* __restrict__ C, int64_t ldc,
+                                                        int64_t tiles_m, int64_t tiles_n) {
+  constexpr int BK = Mma<T>::BK;
+  constexpr int AV = GBM * BK / GTHREADS;  // A elements per thread per k-tile
+  constexpr int BV = BK * GBN / GTHREADS;  // B elements per thread per k-tile
+  __shared__ T As[BK][GBM];
+  __shared__ T Bs[BK][GBN];
+
+  // XCD-aware order: consecutive workgroups land on different XCDs (round
+  // robin), so give each XCD a contiguous band of tile rows.
+  const int64_t T_ = tiles_m * tiles_n;
+  const int64_t L = blockIdx.x;
+  const int64_t per = (T_ + 7) / 8;
+  const int64_t tile = (L % 8) * per + L / 8;
+  if (tile >= T_) return;
+  const int64_t tm = tile / tiles_n, tn = tile % tiles_n;
+  const int64_t row0 = tm * GBM, col0 = tn * GBN;
+
+  const int tid = threadIdx.x;
+  const int lane = tid & 63, wid = tid >> 6;
+  const int wm = wid >> 1, wn = wid & 1;
+
+  typename Mma<T>::acc_t acc[4][4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) acc[i][j][v] = 0;
+
+  // global -> register staging maps
+  // A tile (GBM x BK): thread -> row ar = tid % GBM, k chunk ak = (tid / GBM) * AV
+  const int ar = tid % GBM, ak = (tid / GBM) * AV;
+  // B tile (BK x GBN): thread -> k row bk = tid / (GBN / BV), col chunk bc
+  const int bk = tid / (GBN / BV), bc = (tid % (GBN / BV)) * BV;
+  T ra[AV], rb[BV];
+
+  auto load = [&](int64_t kt) {
+    const int64_t gr = row0 + ar;
+    const int64_t gk = kt + ak;
+    const T* pa = A + gr * lda + gk;
+    if (gr < m && gk + AV <= k) {
+#pragma unroll
+      for (int q = 0; q < AV; ++q) ra[q] = pa[q];
+    } else {
+#pragma unroll
+      for (int q = 0; q < AV; ++q) ra[q] = (gr < m && gk + q < k) ? pa[q] : T(0);
+    }
+    const int64_t gkb = kt + bk;
+    const int64_t gc = col0 + bc;
+    const T* pb = B + gkb * ldb + gc;
+    if (gkb < k && gc + BV <= n) {
+#pragma unroll
+      for (int q = 0; q < BV; ++q) rb[q] = pb[q];
+    } else {
+#pragma unroll
+      for (int q = 0; q < BV; ++q) rb[q] = (gkb < k && gc + q < n) ? pb[q] : T(0);
+    }
+  };
+  auto store = [&]() {
+#pragma unroll
+    for (int q = 0; q < AV; ++q) As[ak + q][ar] = ra[q];
+#pragma unroll
+    for (int q = 0; q < BV; ++q) Bs[bk][bc + q] = rb[q];
+  };
+
+  load(0);
+  for (int64_t kt = 0; kt < k; kt += BK) {
+    __syncthreads();
+    store();
+    __syncthreads();
+    if (kt + BK < k) load(kt + BK);
+#pragma unroll
+    for (int kk = 0; kk < BK; kk += 4) {
+      T af[4], bf[4];
+#pragma unroll
+      for (int i = 0; i < 4; ++i) af[i] = As[kk + (lane >> 4)][wm * 64 + i * 16 + (lane & 15)];
+#pragma unroll
+      for (int j = 0; j < 4; ++j) bf[j] = Bs[kk + (lane >> 4)][wn * 64 + j * 16 + (lane & 15)];
+#pragma unroll
+      for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) acc[i][j] = Mma<T>::mma(af[i], bf[j], acc[i][j]);
+    }
+  }
+
+  // epilogue
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const int64_t c = col0 + wn * 64 + j * 16 + (lane & 15);
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int64_t r = row0 + wm * 64 + i * 16 + 4 * (lane >> 4) + v;
+        if (r < m && c < n) {
+          T* pc = C + r * ldc + c;
+          const T prod = alpha * acc[i][j][v];
+          *pc = (beta != T(0)) ? (*pc + prod) : prod;
+        }
+      }
+    }
+  }
+}
+
+template <typename T>
+static int gemm(int64_t m, int64_t n, int64_t k, T alpha, const T* A, int64_t lda, const T* B,
+                int64_t ldb, T beta, T* C, int64_t ldc, hipStream_t s) {
+  if (m <= 0 || n <= 0) return RBFX_OK;
+  const int64_t tm = ceil_div(m, GBM), tn = ceil_div(n, GBN);
+  const int64_t tiles = tm * tn;
+  const int64_t grid = ((tiles + 7) / 8) * 8;
+  hipLaunchKernelGGL((gemm_kernel<T>), dim3((unsigned)grid), dim3(GTHREADS), 0, s, m, n, k, alpha, A,
+                     lda, B, ldb, beta, C, ldc, tm, tn);
+  RBFX_CHECK_LAUNCH();
+  return RBFX_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Grid barrier for the cooperative panel kernel (all workgroups co-resident).
+// The counter only grows; `target` is the cumulative arrival count.
+__device__ inline bool grid_barrier(unsigned* bar, unsigned target, int* status) {
+  __threadfence();
+  __syncthreads();
+  __shared__ int ok;
+  if (threadIdx.x == 0) {
+    __hip_atomic_fetch_add(bar, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    uint32_t spins = 0;
+    int good = 1;
+    while (__hip_atomic_load(bar, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < target) {
+      __builtin_amdgcn_s_sleep(1);
+      ++spins;
+      if ((spins & 255u) == 0 &&
+          __hip_atomic_load(status, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) {
+        good = 0;
+        break;
+      }
+      if (spins > kSpinLimit) {
+        __hip_atomic_store(status, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        good = 0;
+        break;
+      }
+    }
+    if (good && __hip_atomic_load(status, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) good = 0;
+    ok = good;
+  }
+  __syncthreads();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+  return ok != 0;
+}
+
+template <typename T>
+struct PanelArgs {
+  T* A;           // panel top-left (absolute row r0, column c0)
+  int64_t lda;
+  int64_t m;      // rows in the panel (r0 .. r0+m)
+  int w;          // panel width <= kPanelWMax
+  int64_t R;      // rows per workgroup
+  int64_t RP;     // padded LDS column stride
+  int64_t r0;     // absolute row of the panel's first row
+  int64_t* ipiv;  // (w) absolute pivot rows
+  T* cand_val;    // [2][G]
+  int64_t* cand_idx;  // [2][G]
+  T* cand_row;    // [2][G][w]
+  int* pos2phys;  // [m]
+  int* phys2pos;  // [m]
+  unsigned* bar;
+  unsigned bar_base;
+  int* status;
+  int32_t* info;
+};
+
+constexpr int kPanelThreads = 256;
+constexpr int kPanelWMax = 64;
+
+template <typename T>
+__device__ inline void block_argmax(T v, int64_t idx, T& bv, int64_t& bi) {
+  __shared__ T sv[kPanelThreads / 64];
+  __shared__ int64_t si[kPanelThreads / 64];
+  for (int off = 32; off > 0; off >>= 1) {
+    T ov = __shfl_down(v, off, 64);
+    int64_t oi = __shfl_down(idx, off, 64);
+    if (ov > v || (ov == v && oi < idx)) {
+      v = ov;
+      idx = oi;
+    }
+  }
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  if (lane == 0) {
+    sv[wid] = v;
+    si[wid] = idx;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    T b = sv[0];
+    int64_t i = si[0];
+    for (int q = 1; q < kPanelThreads / 64; ++q)
+      if (sv[q] > b || (sv[q] == b && si[q] < i)) {
+        b = sv[q];
+        i = si[q];
+      }
+    sv[0] = b;
+    si[0] = i;
+  }
+  __syncthreads();
+  bv = sv[0];
+  bi = si[0];
+  __syncthreads();
+}
+
+template <typename T>
+__global__ __launch_bounds__(kPanelThreads) void panel_kernel(PanelArgs<T> a) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  T* P = reinterpret_cast<T*>(smem_raw);          // [w][RP] column-major
+  T* urow = P + (int64_t)a.w * a.RP;               // [w]
+  int* retired = reinterpret_cast<int*>(urow + kPanelWMax);  // [RP]
+  __shared__ int64_t s_win;
+  __shared__ int s_wg;
+
+  const int G = gridDim.x, g = blockIdx.x, tid = threadIdx.x;
+  const int w = a.w;
+  const int64_t rb = (int64_t)g * a.R;
+  const int64_t re = rb + a.R < a.m ? rb + a.R : a.m;
+  const int nr = re > rb ? (int)(re - rb) : 0;
+  unsigned nbar = 0;
+
+  for (int64_t idx = tid; idx < (int64_t)nr * w; idx += kPanelThreads) {
+    const int64_t r = idx / w, c = idx % w;
+    P[c * a.RP + r] = a.A[(rb + r) * a.lda + c];
+  }
+  for (int r = tid; r < nr; r += kPanelThreads) retired[r] = 0;
+  if (g == 0) {
+    for (int64_t i = tid; i < a.m; i += kPanelThreads) {
+      a.pos2phys[i] = (int)i;
+      a.phys2pos[i] = (int)i;
+    }
+  }
+  __syncthreads();
+
+  for (int j = 0; j < w; ++j) {
+    const int par = j & 1;
+    // local candidate for column j
+    T best = T(-1);
+    int64_t bidx = INT64_MAX;
+    for (int r = tid; r < nr; r += kPanelThreads) {
+      if (!retired[r]) {
+        const T v = fabs(P[(int64_t)j * a.RP + r]);
+        if (v > best || (v == best && rb + r < bidx)) {
+          best = v;
+          bidx = rb + r;
+        }
+      }
+    }
+    T bv;
+    int64_t bi;
+    block_argmax<T>(best, bidx, bv, bi);
+    if (tid == 0) {
+      a.cand_val[par * G + g] = bv;
+      a.cand_idx[par * G + g] = bi;
+    }
+    if (bv >= T(0) && tid < w) a.cand_row[((int64_t)par * G + g) * w + tid] = P[(int64_t)tid * a.RP + (bi - rb)];
+    ++nbar;
+    if (!grid_barrier(a.bar, a.bar_base + nbar * (unsigned)G, a.status)) return;
+
+    // global winner
+    T v = T(-1);
+    int64_t vi = INT64_MAX;
+    for (int q = tid; q < G; q += kPanelThreads) {
+      const T cv = a.cand_val[par * G + q];
+      const int64_t ci = a.cand_idx[par * G + q];
+      if (cv > v || (cv == v && ci < vi)) {
+        v = cv;
+        vi = ci;
+      }
+    }
+    T wv;
+    int64_t wi;
+    block_argmax<T>(v, vi, wv, wi);
+    if (!(wv >= T(0))) {
+      // no finite candidate anywhere (NaN column): every workgroup sees the
+      // same winner value, so all of them leave here together
+      if (g == 0 && tid == 0) {
+        *a.info = -1;
+        __hip_atomic_store(a.status, 2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      }
+      return;
+    }
+    if (tid == 0) {
+      s_win = wi;
+      s_wg = (int)(wi / a.R);
+    }
+    __syncthreads();
+    const int64_t prow = s_win;
+    const int wg = s_wg;
+    if (tid < w) urow[tid] = a.cand_row[((int64_t)par * G + wg) * w + tid];
+    if (prow >= rb && prow < re && tid == 0) retired[prow - rb] = 1;
+    if (g == 0 && tid == 0) {
+      const int pp = a.phys2pos[prow];
+      const int q = a.pos2phys[j];
+      a.pos2phys[j] = (int)prow;
+      a.pos2phys[pp] = q;
+      a.phys2pos[prow] = j;
+      a.phys2pos[q] = pp;
+      a.ipiv[j] = a.r0 + pp;
+      if (wv == T(0) && *a.info == 0) *a.info = (int32_t)(a.r0 + j + 1);
+    }
+    __syncthreads();
+    const T piv = urow[j];
+    const T rpiv = piv != T(0) ? T(1) / piv : T(0);
+    for (int r = tid; r < nr; r += kPanelThreads) {
+      if (retired[r]) continue;
+      const T l = P[(int64_t)j * a.RP + r] * rpiv;
+      P[(int64_t)j * a.RP + r] = l;
+      for (int c = j + 1; c < w; ++c) P[(int64_t)c * a.RP + r] -= l * urow[c];
+    }
+    __syncthreads();
+  }
+  ++nbar;
+  if (!grid_barrier(a.bar, a.bar_base + nbar * (unsigned)G, a.status)) return;
+  for (int64_t idx = tid; idx < (int64_t)nr * w; idx += kPanelThreads) {
+    const int64_t r = idx / w, c = idx % w;
+    const int64_t dest = a.phys2pos[rb + r];
+    a.A[dest * a.lda + c] = P[c * a.RP + r];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Row interchange plan: the net permutation of the sequential swaps
+// ipiv[s..e) (absolute rows, ipiv[i] >= i) as (dst, src) pairs.
+constexpr int kSwapMax = 256;
+
+__global__ __launch_bounds__(64) void swap_plan_kernel(const int64_t* __restrict__ ipiv, int64_t s,
+                                                       int64_t e, int64_t* __restrict__ dst,
+                                                       int64_t* __restrict__ src,
+                                                       int* __restrict__ count) {
+  __shared__ int64_t in_src[kSwapMax];
+  __shared__ int64_t out_pos[kSwapMax];
+  __shared__ int64_t out_src[kSwapMax];
+  __shared__ int nout;
+  const int lane = threadIdx.x;
+  const int k = (int)(e - s);
+  for (int t = lane; t < k; t += 64) in_src[t] = s + t;
+  if (lane == 0) nout = 0;
+  __syncthreads();
+  for (int i = 0; i < k; ++i) {
+    const int64_t p = ipiv[s + i];
+    if (p < e) {
+      if (lane == 0 && p != s + i) {
+        const int64_t tmp = in_src[i];
+        in_src[i] = in_src[p - s];
+        in_src[p - s] = tmp;
+      }
+    } else {
+      // find p among out_pos (wave-parallel)
+      int found = -1;
+      for (int base = 0; base < nout; base += 64) {
+        const int q = base + lane;
+        const bool hit = q < nout && out_pos[q] == p;
+        const unsigned long long bal = __ballot(hit);
+        if (bal) {
+          found = base + __ffsll((long long)bal) - 1;
+          break;
+        }
+      }
+      if (lane == 0) {
+        int q = found;
+        if (q < 0) {
+          q = nout++;
+          out_pos[q] = p;
+          out_src[q] = p;
+        }
+        const int64_t tmp = in_src[i];
+        in_src[i] = out_src[q];
+        out_src[q] = tmp;
+      }
+    }
+    __syncthreads();
+  }
+  if (lane == 0) {
+    int c = 0;
+    for (int t = 0; t < k; ++t)
+      if (in_src[t] != s + t) {
+        dst[c] = s + t;
+        src[c] = in_src[t];
+        ++c;
+      }
+    for (int q = 0; q < nout; ++q)
+      if (out_src[q] != out_pos[q]) {
+        dst[c] = out_pos[q];
+        src[c] = out_src[q];
+        ++c;
+      }
+    *count = c;
+  }
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void swap_gather_kernel(const T* __restrict__ A, int64_t lda,
+                                                          int64_t c0, int64_t nc,
+                                                          const int64_t* __restrict__ src,
+                                                          const int* __restrict__ count,
+                                                          T* __restrict__ tmp) {
+  const int q = blockIdx.y;
+  if (q >= *count) return;
+  const int64_t row = src[q];
+  for (int64_t c = (int64_t)blockIdx.x * 256 + threadIdx.x; c < nc; c += (int64_t)gridDim.x * 256)
+    tmp[(int64_t)q * nc + c] = A[row * lda + c0 + c];
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void swap_scatter_kernel(T* __restrict__ A, int64_t lda, int64_t c0,
+                                                           int64_t nc,
+                                                           const int64_t* __restrict__ dst,
+                                                           const int* __restrict__ count,
+                                                           const T* __restrict__ tmp) {
+  const int q = blockIdx.y;
+  if (q >= *count) return;
+  const int64_t row = dst[q];
+  for (int64_t c = (int64_t)blockIdx.x * 256 + threadIdx.x; c < nc; c += (int64_t)gridDim.x * 256)
+    A[row * lda + c0 + c] = tmp[(int64_t)q * nc + c];
+}
+
+// ---------------------------------------------------------------------------
+// U12 = L11^-1 A12 (L11 unit lower W x W), strips of 32 columns staged in LDS.
+constexpr int kTrsmCols = 32;
+constexpr int kTrsmThreads = 256;
+
+template <typename T>
+__global__ __launch_bounds__(kTrsmThreads) void trsm_llnu_kernel(const T* __restrict__ L, int64_t ldl,
+                                                                 T* __restrict__ B, int64_t ldb, int W,
+                                                                 int64_t ncols) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  T* X = reinterpret_cast<T*>(smem_raw);  // [W][kTrsmCols]
+  const int64_t c0 = (int64_t)blockIdx.x * kTrsmCols;
+  const int col = threadIdx.x % kTrsmCols, grp = threadIdx.x / kTrsmCols;
+  constexpr int NG = kTrsmThreads / kTrsmCols;
+  const bool valid = c0 + col < ncols;
+  for (int i = grp; i < W; i += NG) X[i * kTrsmCols + col] = valid ? B[i * ldb + c0 + col] : T(0);
+  __syncthreads();
+  for (int kk = 0; kk < W - 1; ++kk) {
+    const T xk = X[kk * kTrsmCols + col];
+    for (int i = kk + 1 + grp; i < W; i += NG) X[i * kTrsmCols + col] -= L[(int64_t)i * ldl + kk] * xk;
+    __syncthreads();
+  }
+  if (valid)
+    for (int i = grp; i < W; i += NG) B[i * ldb + c0 + col] = X[i * kTrsmCols + col];
+}
+
+__global__ void zero_u32_kernel(unsigned* p, int n) {
+  if ((int)threadIdx.x < n) p[threadIdx.x] = 0u;
+}
+
+// A failed or timed-out cooperative panel leaves status != 0: report it as
+// info = -1 (the factors are then invalid).
+__global__ void status_to_info_kernel(const int* status, int32_t* info) {
+  if (threadIdx.x == 0 && *status != 0) *info = -1;
+}
+
+// ---------------------------------------------------------------------------
+// Driver.
+template <typename T>
+struct LuCtx {
+  T* A;
+  int64_t n, lda;
+  int64_t* ipiv;
+  int32_t* info;
+  hipStream_t s;
+  // workspace
+  T* cand_val;
+  int64_t* cand_idx;
+  T* cand_row;
+  int* pos2phys;
+  int* phys2pos;
+  unsigned* bar;
+  int* status;
+  unsigned bar_count;  // cumulative arrivals
+  int64_t* plan_dst;
+  int64_t* plan_src;
+  int* plan_count;
+  T* swap_tmp;
+  int64_t swap_tmp_elems;
+  int G_max;
+  int wmax;
+  size_t lds_budget;
+};
+
+static int panel_width_max(int64_t n, int bytes, const DeviceInfo& di, size_t lds_budget) {
+  // widest leaf whose LDS share (rows per workgroup x w) fits with num_cus groups
+  for (int w = kPanelWMax; w >= 8; w /= 2) {
+    const int64_t R = ceil_div(n, di.num_cus);
+    const size_t need = (size_t)w * (size_t)(R + 1) * bytes + kPanelWMax * bytes + (size_t)(R + 1) * 4;
+    if (need <= lds_budget) return w;
+  }
+  return 0;
+}
+
+template <typename T>
+static int leaf(LuCtx<T>& c, int64_t r0, int w) {
+  const DeviceInfo& di = device_info();
+  const int64_t m = c.n - r0;
+  int G = (int)std::min<int64_t>(di.num_cus, std::max<int64_t>(1, ceil_div(m, 128)));
+  int64_t R = ceil_div(m, G);
+  auto need = [&](int64_t Rr) {
+    const int64_t RP = Rr + 1;
+    return (size_t)w * RP * sizeof(T) + kPanelWMax * sizeof(T) + (size_t)RP * 4;
+  };
+  while (need(R) > c.lds_budget && G < di.num_cus) {
+    G = std::min<int>(di.num_cus, G * 2);
+    R = ceil_div(m, G);
+  }
+  RBFX_REQUIRE(need(R) <= c.lds_budget, "panel does not fit in LDS (m=%lld, w=%d)", (long long)m, w);
+  G = (int)ceil_div(m, R);
+  PanelArgs<T> a;
+  a.A = c.A + r0 * c.lda + r0;
+  a.lda = c.lda;
+  a.m = m;
+  a.w = w;
+  a.R = R;
+  a.RP = R + 1;
+  a.r0 = r0;
+  a.ipiv = c.ipiv + r0;
+  a.cand_val = c.cand_val;
+  a.cand_idx = c.cand_idx;
+  a.cand_row = c.cand_row;
+  a.pos2phys = c.pos2phys;
+  a.phys2pos = c.phys2pos;
+  a.bar = c.bar;
+  a.bar_base = c.bar_count;
+  a.status = c.status;
+  a.info = c.info;
+  const size_t smem = need(R);
+  void* args[] = {&a};
+  RBFX_HIP(hipLaunchCooperativeKernel(reinterpret_cast<const void*>(&panel_kernel<T>), dim3(G),
+                                      dim3(kPanelThreads), args, (unsigned)smem, c.s));
+  c.bar_count += (unsigned)G * (unsigned)(w + 1);
+  return RBFX_OK;
+}
+
+// Apply the swaps ipiv[s..e) to columns [c0, c0+nc).
+template <typename T>
+static int apply_swaps(LuCtx<T>& c, int64_t s, int64_t e, int64_t c0, int64_t nc) {
+  if (nc <= 0 || e <= s) return RBFX_OK;
+  RBFX_REQUIRE(e - s <= kSwapMax, "too many swaps in one plan");
+  hipLaunchKernelGGL(swap_plan_kernel, dim3(1), dim3(64), 0, c.s, c.ipiv, s, e, c.plan_dst,
+                     c.plan_src, c.plan_count);
+  RBFX_CHECK_LAUNCH();
+  const int maxcnt = (int)(2 * (e - s));
+  // process the columns in slices that fit the staging buffer
+  const int64_t slice = std::max<int64_t>(256, c.swap_tmp_elems / maxcnt);
+  for (int64_t off = 0; off < nc; off += slice) {
+    const int64_t w = std::min<int64_t>(slice, nc - off);
+    dim3 grid((unsigned)std::min<int64_t>(ceil_div(w, 256), 64), (unsigned)maxcnt);
+    hipLaunchKernelGGL((swap_gather_kernel<T>), grid, dim3(256), 0, c.s, c.A, c.lda, c0 + off, w,
+                       c.plan_src, c.plan_count, c.swap_tmp);
+    RBFX_CHECK_LAUNCH();
+    hipLaunchKernelGGL((swap_scatter_kernel<T>), grid, dim3(256), 0, c.s, c.A, c.lda, c0 + off, w,
+                       c.plan_dst, c.plan_count, c.swap_tmp);
+    RBFX_CHECK_LAUNCH();
+  }
+  return RBFX_OK;
+}
+
+template <typename T>
+static int trsm(LuCtx<T>& c, int64_t r0, int W, int64_t col, int64_t ncols) {
+  if (ncols <= 0) return RBFX_OK;
+  const size_t smem = (size_t)W * kTrsmCols * sizeof(T);
+  RBFX_REQUIRE(smem <= c.lds_budget, "trsm strip does not fit LDS");
+  hipLaunchKernelGGL((trsm_llnu_kernel<T>), dim3((unsigned)ceil_div(ncols, kTrsmCols)),
+                     dim3(kTrsmThreads), smem, c.s, c.A + r0 * c.lda + r0, c.lda,
+                     c.A + r0 * c.lda + col, c.lda, W, ncols);
+  RBFX_CHECK_LAUNCH();
+  return RBFX_OK;
+}
+
+// Recursive factorisation of the panel rows [r0, n) x columns [r0, r0+W).
+template <typename T>
+static int panel(LuCtx<T>& c, int64_t r0, int W) {
+  int st;
+  if (W <= c.wmax) return leaf(c, r0, W);
+  const int W1 = std::max(8, (W / 2 + 7) / 8 * 8);
+  const int W2 = W - W1;
+  if ((st = panel(c, r0, W1))) return st;
+  // swaps of the left half applied to the right half's columns
+  if ((st = apply_swaps(c, r0, r0 + W1, r0 + W1, W2))) return st;
+  if ((st = trsm(c, r0, W1, r0 + W1, W2))) return st;
+  const int64_t m2 = c.n - r0 - W1;
+  if ((st = gemm<T>(m2, W2, W1, T(-1), c.A + (r0 + W1) * c.lda + r0, c.lda, c.A + r0 * c.lda + r0 + W1,
+                    c.lda, T(1), c.A + (r0 + W1) * c.lda + r0 + W1, c.lda, c.s)))
+    return st;
+  if ((st = panel(c, r0 + W1, W2))) return st;
+  // swaps of the right half applied to the left half's columns
+  return apply_swaps(c, r0 + W1, r0 + W, r0, W1);
+}
+
+constexpr int kNB = 256;
+
+static size_t getrf_ws_bytes(int64_t n, int bytes) {
+  const int G = 1024;  // upper bound on workgroups of a leaf
+  size_t b = 0;
+  b += ws_round((size_t)2 * G * bytes);           // cand_val
+  b += ws_round((size_t)2 * G * 8);               // cand_idx
+  b += ws_round((size_t)2 * G * kPanelWMax * bytes);  // cand_row
+  b += ws_round((size_t)n * 4) * 2;               // pos2phys, phys2pos
+  b += ws_round(64);                              // barrier + status
+  b += ws_round((size_t)2 * kSwapMax * 8) * 2;    // plan dst / src
+  b += ws_round(64);                              // plan count
+  // swap staging: 2*NB rows x min(n, 8192) columns
+  b += ws_round((size_t)2 * kNB * std::min<int64_t>(n, 8192) * bytes);
+  return b;
+}
+
+template <typename T>
+static int getrf(int64_t n, T* A, int64_t lda, int64_t* ipiv, void* work, size_t work_bytes,
+                 int32_t* info, hipStream_t s) {
+  RBFX_REQUIRE(n > 0 && lda >= n, "bad n / lda");
+  RBFX_REQUIRE(A && ipiv && info, "null pointer");
+  RBFX_REQUIRE(n < (int64_t)INT32_MAX, "n too large");
+  RBFX_REQUIRE(work && work_bytes >= getrf_ws_bytes(n, sizeof(T)), "getrf workspace too small");
+  const DeviceInfo& di = device_info();
+  LuCtx<T> c;
+  c.A = A;
+  c.n = n;
+  c.lda = lda;
+  c.ipiv = ipiv;
+  c.info = info;
+  c.s = s;
+  Workspace ws(work, work_bytes);
+  const int G = 1024;
+  c.cand_val = ws.take<T>(2 * G);
+  c.cand_idx = ws.take<int64_t>(2 * G);
+  c.cand_row = ws.take<T>((size_t)2 * G * kPanelWMax);
+  c.pos2phys = ws.take<int>(n);
+  c.phys2pos = ws.take<int>(n);
+  unsigned* ctl = ws.take<unsigned>(16);
+  c.bar = ctl;
+  c.status = reinterpret_cast<int*>(ctl + 1);
+  c.plan_dst = ws.take<int64_t>(2 * kSwapMax);
+  c.plan_src = ws.take<int64_t>(2 * kSwapMax);
+  c.plan_count = ws.take<int>(16);
+  c.swap_tmp_elems = (int64_t)2 * kNB * std::min<int64_t>(n, 8192);
+  c.swap_tmp = ws.take<T>(c.swap_tmp_elems);
+  RBFX_REQUIRE(c.swap_tmp != nullptr, "workspace carve failed");
+  c.bar_count = 0;
+  c.G_max = G;
+  c.lds_budget = std::min<size_t>(di.lds_per_block, 65536);
+  c.wmax = panel_width_max(n, sizeof(T), di, c.lds_budget);
+  RBFX_REQUIRE(c.wmax > 0, "matrix too large for the LDS-resident panel (n=%lld)", (long long)n);
+  c.wmax = std::min(c.wmax, sizeof(T) == 8 ? 32 : 64);
+  hipLaunchKernelGGL(zero_u32_kernel, dim3(1), dim3(64), 0, s, ctl, 16);
+  RBFX_CHECK_LAUNCH();
+  RBFX_HIP(hipMemsetAsync(info, 0, sizeof(int32_t), s));
+  int st;
+  for (int64_t k0 = 0; k0 < n; k0 += kNB) {
+    const int W = (int)std::min<int64_t>(kNB, n - k0);
+    if ((st = panel(c, k0, W))) return st;
+    if ((st = apply_swaps(c, k0, k0 + W, 0, k0))) return st;
+    const int64_t rest = n - k0 - W;
+    if (rest > 0) {
+      if ((st = apply_swaps(c, k0, k0 + W, k0 + W, rest))) return st;
+      if ((st = trsm(c, k0, W, k0 + W, rest))) return st;
+      if ((st = gemm<T>(rest, rest, W, T(-1), A + (k0 + W) * lda + k0, lda, A + k0 * lda + k0 + W, lda,
+                        T(1), A + (k0 + W) * lda + k0 + W, lda, s)))
+        return st;
+    }
+  }
+  hipLaunchKernelGGL(status_to_info_kernel, dim3(1), dim3(64), 0, s, c.status, info);
+  RBFX_CHECK_LAUNCH();
+  return RBFX_OK;
+}
+
+// Exposed for the blocked triangular solves and tests.
+int gemm_f64(int64_t m, int64_t n, int64_t k, double alpha, const double* A, int64_t lda,
+             const double* B, int64_t ldb, double beta, double* C, int64_t ldc, hipStream_t s) {
+  return gemm<double>(m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, s);
+}
+int gemm_f32(int64_t m, int64_t n, int64_t k, float alpha, const float* A, int64_t lda,
+             const float* B, int64_t ldb, float beta, float* C, int64_t ldc, hipStream_t s) {
+  return gemm<float>(m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, s);
+}
+
+}  // namespace rbfx
+
+using namespace rbfx;
+
+extern "C" size_t rbfx_getrf_workspace(int64_t n, int dtype_bytes) {
+  if (n <= 0 || (dtype_bytes != 4 && dtype_bytes != 8)) return 0;
+  return getrf_ws_bytes(n, dtype_bytes);
+}
+
+extern "C" int rbfx_getrf_f64(int64_t n, double* d_A, int64_t lda, int64_t* d_ipiv, void* d_work,
+                              size_t work_bytes, int32_t* d_info, void* stream) {
+  return getrf<double>(n, d_A, lda, d_ipiv, d_work, work_bytes, d_info, as_stream(stream));
+}
+
+extern "C" int rbfx_getrf_f32(int64_t n, float* d_A, int64_t lda, int64_t* d_ipiv, void* d_work,
+                              size_t work_bytes, int32_t* d_info, void* stream) {
+  return getrf<float>(n, d_A, lda, d_ipiv, d_work, work_bytes, d_info, as_stream(stream));
+}
+
+// Test hook: C = beta C + alpha A B with the MFMA kernel.
+extern "C" int rbfx_gemm_f64(int64_t m, int64_t n, int64_t k, double alpha, const double* d_A,
+                             int64_t lda, const double* d_B, int64_t ldb, double beta, double* d_C,
+                             int64_t ldc, void* stream) {
+  RBFX_REQUIRE(m >= 0 && n >= 0 && k >= 0 && d_A && d_B && d_C, "bad gemm args");
+  RBFX_REQUIRE(beta == 0.0 || beta == 1.0, "beta must be 0 or 1");
+  return gemm<double>(m, n, k, alpha, d_A, lda, d_B, ldb, beta, d_C, ldc, as_stream(stream));
+}
+extern "C" int rbfx_gemm_f32(int64_t m, int64_t n, int64_t k, float alpha, const float* d_A,
+                             int64_t lda, const float* d_B, int64_t ldb, float beta, float* d_C,
+                             int64_t ldc, void* stream) {
+  RBFX_REQUIRE(m >= 0 && n >= 0 && k >= 0 && d_A && d_B && d_C, "bad gemm args");
+  RBFX_REQUIRE(beta == 0.0f || beta == 1.0f, "beta must be 0 or 1");
+  return gemm<float>(m, n, k, alpha, d_A, lda, d_B, ldb, beta, d_C, ldc, as_stream(stream));
+}

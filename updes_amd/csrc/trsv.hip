@@ -1,0 +1,304 @@
+// K4: triangular solves with the LU factors (FP64 and FP32), A x = b and
+// A^T x = b, for a single right-hand side.
+//
+// Replaces the solve phase of lineax QR (updes/operators.py:612-613) and the
+// inv(A) @ v products (updes/assembly.py:410,430).  Blocked, synchronisation-
+// free sweep: one workgroup per 64-row block row, taken in dependency order
+// from an atomic ticket; each workgroup folds in the off-diagonal tiles of the
+// blocks already solved (a monotone completion counter tells it how far it
+// may go), then applies the precomputed inverse of its diagonal block and
+// publishes its slice.  Row-major storage makes the no-transpose tiles
+// row-contiguous and the transposed tiles column-coalesced.
+#include <algorithm>
+#include <vector>
+
+#include "common.h"
+
+namespace rbfx {
+
+constexpr int TB = 64;  // block size of the sweep and of the diagonal inverses
+constexpr int kTsThreads = 256;
+
+__host__ __device__ inline int64_t imin64(int64_t a, int64_t b) { return a < b ? a : b; }
+
+// ---- diagonal-block inverses ----------------------------------------------
+// dinv[b][0] = inv(unit lower L_bb), dinv[b][1] = inv(U_bb), TB x TB row-major,
+// identity-padded for the last partial block.
+template <typename TF>
+__global__ __launch_bounds__(TB) void diag_inv_kernel(const TF* __restrict__ LU, int64_t lda, int64_t n,
+                                                      double* __restrict__ dinv) {
+  typedef double T;
+  __shared__ T S[TB][TB + 1];
+  __shared__ T X[TB][TB + 1];
+  const int64_t b = blockIdx.x;
+  const int64_t o = b * TB;
+  const int sz = (int)imin64(TB, n - o);
+  const int c = threadIdx.x;
+  for (int i = 0; i < TB; ++i) S[i][c] = (i < sz && c < sz) ? (T)LU[(o + i) * lda + o + c] : (i == c ? T(1) : T(0));
+  __syncthreads();
+  // L^-1 column c: x_i = -sum_{c<=k<i} L_ik x_k
+  for (int i = 0; i < TB; ++i) X[i][c] = (i == c) ? T(1) : T(0);
+  for (int i = c + 1; i < TB; ++i) {
+    T s = 0;
+    for (int k = c; k < i; ++k) s += S[i][k] * X[k][c];
+    X[i][c] = (i < sz) ? -s : T(0);
+  }
+  T* Li = dinv + (b * 2) * TB * TB;
+  for (int i = 0; i < TB; ++i) Li[i * TB + c] = X[i][c];
+  __syncthreads();
+  // U^-1 column c: x_i = (delta_ic - sum_{k>i} U_ik x_k) / U_ii
+  for (int i = TB - 1; i >= 0; --i) {
+    if (i > c) {
+      X[i][c] = T(0);
+      continue;
+    }
+    T s = (i == c) ? T(1) : T(0);
+    for (int k = i + 1; k <= c; ++k) s -= S[i][k] * X[k][c];
+    X[i][c] = s / S[i][i];
+  }
+  T* Ui = dinv + (b * 2 + 1) * TB * TB;
+  for (int i = 0; i < TB; ++i) Ui[i * TB + c] = X[i][c];
+}
+
+// ---- permutation staging ------------------------------------------------------
+__global__ void gather_perm_kernel(const double* __restrict__ b, const int64_t* __restrict__ perm,
+                                   int64_t n, double* __restrict__ y, unsigned* __restrict__ ctl) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i == 0) {
+    ctl[0] = 0u;
+    ctl[1] = 0u;
+    ctl[2] = 0u;
+  }
+  if (i < n) y[i] = b[perm ? perm[i] : i];
+}
+
+__global__ void scatter_perm_kernel(const double* __restrict__ y, const int64_t* __restrict__ perm,
+                                    int64_t n, double* __restrict__ b) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) b[perm ? perm[i] : i] = y[i];
+}
+
+// ---- the sweep ---------------------------------------------------------------
+// FWD: block rows processed 0..nb-1, else nb-1..0.
+// TRANS=false: the tile (I,J) of the factor itself is used (row-contiguous);
+// TRANS=true : the tile (J,I) transposed.
+// WHICH: which diagonal inverse (0 = L, 1 = U) is applied.
+// Factors are read in their storage type TF; the arithmetic is FP64, so FP32
+// factors are applied exactly as stored (the FP32 preconditioner of GMRES-IR).
+template <typename TF, bool FWD, bool TRANS, int WHICH>
+__global__ __launch_bounds__(kTsThreads) void sweep_kernel(const TF* __restrict__ F, int64_t lda, int64_t n,
+                                                           const double* __restrict__ dinv,
+                                                           double* __restrict__ y, unsigned* __restrict__ ctl) {
+  typedef double T;
+  __shared__ T acc[TB];
+  __shared__ T yj[TB];
+  __shared__ T part[4][TB];
+  __shared__ int s_I;
+  __shared__ unsigned s_avail;
+  const int tid = threadIdx.x;
+  const int64_t nb = ceil_div(n, TB);
+  unsigned* ticket = ctl;
+  unsigned* done = ctl + 1;
+  int* status = reinterpret_cast<int*>(ctl + 2);
+  if (tid == 0) {
+    const unsigned t = __hip_atomic_fetch_add(ticket, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    s_I = (int)t;
+    s_avail = 0;
+  }
+  __syncthreads();
+  const int seq = s_I;
+  if (seq >= nb) return;
+  const int64_t I = FWD ? seq : nb - 1 - seq;
+  const int64_t oI = I * TB;
+  const int szI = (int)imin64(TB, n - oI);
+  if (tid < TB) acc[tid] = tid < szI ? y[oI + tid] : T(0);
+
+  // dependencies in completion order: seq' = 0 .. seq-1
+  for (int d = 0; d < seq; ++d) {
+    if ((unsigned)d >= s_avail) {
+      __syncthreads();
+      if (tid == 0) {
+        unsigned v;
+        uint32_t spins = 0;
+        while ((v = __hip_atomic_load(done, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT)) <= (unsigned)d) {
+          __builtin_amdgcn_s_sleep(1);
+          if (++spins > kSpinLimit) {
+            __hip_atomic_store(status, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            v = 0xffffffffu;
+            break;
+          }
+          if ((spins & 255u) == 0 &&
+              __hip_atomic_load(status, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) {
+            v = 0xffffffffu;
+            break;
+          }
+        }
+        s_avail = v;
+      }
+      __syncthreads();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+      if (s_avail == 0xffffffffu) return;
+    }
+    const int64_t J = FWD ? d : nb - 1 - d;
+    const int64_t oJ = J * TB;
+    const int szJ = (int)imin64(TB, n - oJ);
+    if (tid < TB) yj[tid] = tid < szJ ? y[oJ + tid] : T(0);
+    __syncthreads();
+    if (!TRANS) {
+      // acc[r] -= sum_c F[oI+r][oJ+c] yj[c]; thread: r = tid/4, c in quarter
+      const int r = tid >> 2, q = tid & 3;
+      T s = 0;
+      if (r < szI) {
+        const TF* row = F + (oI + r) * lda + oJ + q * 16;
+#pragma unroll
+        for (int c = 0; c < 16; ++c)
+          if (q * 16 + c < szJ) s += (T)row[c] * yj[q * 16 + c];
+      }
+      s += __shfl_xor(s, 1, 64);
+      s += __shfl_xor(s, 2, 64);
+      if (q == 0 && r < szI) acc[r] -= s;
+    } else {
+      // acc[c] -= sum_r F[oJ+r][oI+c] yj[r]; thread: c = tid%64, rows quarter tid/64
+      const int c = tid & 63, q = tid >> 6;
+      T s = 0;
+      if (c < szI) {
+        for (int r = q * 16; r < q * 16 + 16 && r < szJ; ++r) s += (T)F[(oJ + r) * lda + oI + c] * yj[r];
+      }
+      part[q][c] = s;
+      __syncthreads();
+      if (tid < TB) acc[tid] -= part[0][tid] + part[1][tid] + part[2][tid] + part[3][tid];
+    }
+    __syncthreads();
+  }
+
+  __syncthreads();
+  // diagonal block: y_I = D acc (D = inverse of the diagonal block, or its transpose)
+  const T* D = dinv + (I * 2 + WHICH) * TB * TB;
+  {
+    const int r = tid >> 2, q = tid & 3;
+    T s = 0;
+#pragma unroll
+    for (int c = 0; c < 16; ++c) {
+      const int cc = q * 16 + c;
+      const T dv = TRANS ? D[cc * TB + r] : D[r * TB + cc];
+      s += dv * acc[cc];
+    }
+    s += __shfl_xor(s, 1, 64);
+    s += __shfl_xor(s, 2, 64);
+    if (q == 0 && r < szI) y[oI + r] = s;
+  }
+  __threadfence();
+  __syncthreads();
+  if (tid == 0) __hip_atomic_fetch_add(done, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+template <typename T>
+static int launch_sweep(bool fwd, bool trans, int which, const T* F, int64_t lda, int64_t n,
+                        const double* dinv, double* y, unsigned* ctl, hipStream_t s) {
+  const unsigned nb = (unsigned)ceil_div(n, TB);
+  // reset ticket/done before each sweep
+  hipLaunchKernelGGL(gather_perm_kernel, dim3(1), dim3(1), 0, s, (const double*)nullptr,
+                     (const int64_t*)nullptr, (int64_t)0, (double*)nullptr, ctl);
+  RBFX_CHECK_LAUNCH();
+#define RBFX_SWEEP(F_, T_, W_)                                                               \
+  hipLaunchKernelGGL((sweep_kernel<T, F_, T_, W_>), dim3(nb), dim3(kTsThreads), 0, s, F, lda, n, \
+                     dinv, y, ctl)
+  if (fwd && !trans && which == 0) RBFX_SWEEP(true, false, 0);
+  else if (!fwd && !trans && which == 1) RBFX_SWEEP(false, false, 1);
+  else if (fwd && trans && which == 1) RBFX_SWEEP(true, true, 1);
+  else if (!fwd && trans && which == 0) RBFX_SWEEP(false, true, 0);
+  else {
+    set_error("invalid sweep combination");
+    return RBFX_ERR_ARG;
+  }
+#undef RBFX_SWEEP
+  RBFX_CHECK_LAUNCH();
+  return RBFX_OK;
+}
+
+template <typename T>
+static int getrs(int trans, int64_t n, const T* LU, int64_t lda, const int64_t* perm, const double* dinv,
+                 double* b, void* work, size_t work_bytes, hipStream_t s) {
+  RBFX_REQUIRE(n > 0 && lda >= n && LU && perm && dinv && b, "bad getrs args");
+  Workspace ws(work, work_bytes);
+  double* y = ws.take<double>(n);
+  unsigned* ctl = ws.take<unsigned>(64);
+  RBFX_REQUIRE(y && ctl, "getrs workspace too small");
+  const unsigned g = (unsigned)ceil_div(n, 256);
+  int st;
+  if (!trans) {
+    hipLaunchKernelGGL(gather_perm_kernel, dim3(g), dim3(256), 0, s, b, perm, n, y, ctl);
+    RBFX_CHECK_LAUNCH();
+    if ((st = launch_sweep<T>(true, false, 0, LU, lda, n, dinv, y, ctl, s))) return st;
+    if ((st = launch_sweep<T>(false, false, 1, LU, lda, n, dinv, y, ctl, s))) return st;
+    hipLaunchKernelGGL(scatter_perm_kernel, dim3(g), dim3(256), 0, s, y, (const int64_t*)nullptr, n, b);
+    RBFX_CHECK_LAUNCH();
+  } else {
+    hipLaunchKernelGGL(gather_perm_kernel, dim3(g), dim3(256), 0, s, b, (const int64_t*)nullptr, n, y,
+                       ctl);
+    RBFX_CHECK_LAUNCH();
+    if ((st = launch_sweep<T>(true, true, 1, LU, lda, n, dinv, y, ctl, s))) return st;
+    if ((st = launch_sweep<T>(false, true, 0, LU, lda, n, dinv, y, ctl, s))) return st;
+    hipLaunchKernelGGL(scatter_perm_kernel, dim3(g), dim3(256), 0, s, y, perm, n, b);
+    RBFX_CHECK_LAUNCH();
+  }
+  return RBFX_OK;
+}
+
+template <typename T>
+static int prepare(int64_t n, const T* LU, int64_t lda, const int64_t* ipiv, int64_t* perm, double* dinv,
+                   hipStream_t s) {
+  RBFX_REQUIRE(n > 0 && lda >= n && LU && ipiv && perm && dinv, "bad getrs_prepare args");
+  std::vector<int64_t> hp(n), pm(n);
+  RBFX_HIP(hipMemcpyAsync(hp.data(), ipiv, n * sizeof(int64_t), hipMemcpyDeviceToHost, s));
+  RBFX_HIP(hipStreamSynchronize(s));
+  for (int64_t i = 0; i < n; ++i) pm[i] = i;
+  for (int64_t i = 0; i < n; ++i) {
+    const int64_t p = hp[i];
+    RBFX_REQUIRE(p >= i && p < n, "invalid pivot ipiv[%lld] = %lld", (long long)i, (long long)p);
+    std::swap(pm[i], pm[p]);
+  }
+  RBFX_HIP(hipMemcpyAsync(perm, pm.data(), n * sizeof(int64_t), hipMemcpyHostToDevice, s));
+  hipLaunchKernelGGL((diag_inv_kernel<T>), dim3((unsigned)ceil_div(n, TB)), dim3(TB), 0, s, LU, lda, n, dinv);
+  RBFX_CHECK_LAUNCH();
+  RBFX_HIP(hipStreamSynchronize(s));  // pm must outlive the copy
+  return RBFX_OK;
+}
+
+}  // namespace rbfx
+
+using namespace rbfx;
+
+extern "C" int64_t rbfx_getrs_dinv_elems(int64_t n) { return n <= 0 ? 0 : ceil_div(n, TB) * 2 * TB * TB; }
+
+extern "C" size_t rbfx_getrs_workspace(int64_t n) {
+  return ws_round((size_t)n * sizeof(double)) + ws_round(64 * sizeof(unsigned)) + 512;
+}
+
+extern "C" int rbfx_getrs_prepare_f64(int64_t n, const double* d_LU, int64_t lda, const int64_t* d_ipiv,
+                                      int64_t* d_perm, double* d_dinv, void* d_work, size_t work_bytes,
+                                      void* stream) {
+  (void)d_work;
+  (void)work_bytes;
+  return prepare<double>(n, d_LU, lda, d_ipiv, d_perm, d_dinv, as_stream(stream));
+}
+
+extern "C" int rbfx_getrs_prepare_f32(int64_t n, const float* d_LU, int64_t lda, const int64_t* d_ipiv,
+                                      int64_t* d_perm, double* d_dinv, void* d_work, size_t work_bytes,
+                                      void* stream) {
+  (void)d_work;
+  (void)work_bytes;
+  return prepare<float>(n, d_LU, lda, d_ipiv, d_perm, d_dinv, as_stream(stream));
+}
+
+extern "C" int rbfx_getrs_f64(int trans, int64_t n, const double* d_LU, int64_t lda, const int64_t* d_perm,
+                              const double* d_dinv, double* d_b, void* d_work, size_t work_bytes,
+                              void* stream) {
+  return getrs<double>(trans, n, d_LU, lda, d_perm, d_dinv, d_b, d_work, work_bytes, as_stream(stream));
+}
+
+extern "C" int rbfx_getrs_f32(int trans, int64_t n, const float* d_LU, int64_t lda, const int64_t* d_perm,
+                              const double* d_dinv, double* d_b, void* d_work, size_t work_bytes,
+                              void* stream) {
+  return getrs<float>(trans, n, d_LU, lda, d_perm, d_dinv, d_b, d_work, work_bytes, as_stream(stream));
+}

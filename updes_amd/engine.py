@@ -1,0 +1,365 @@
+"""Device-side engine: clouds and operators as ``rbfx_problem``s, assembly,
+LU factorisations, solves and evaluations, all through the HIP library.
+
+Everything here runs on the GPU through ``librbfx.so``; there is no host
+fallback (``_lib.require_gpu`` raises when no device is present).
+"""
+import collections
+import ctypes
+import hashlib
+
+import numpy as np
+import torch
+
+from . import _lib as L
+
+
+def _ld(n):
+    return (n + 63) // 64 * 64
+
+
+# ---- cloud ------------------------------------------------------------------
+class DeviceCloud:
+    """Node coordinates, normals and periodic pairs of a cloud on the device."""
+
+    def __init__(self, cloud, device):
+        self.N, self.Ni, self.Nd, self.Nn, self.Nr = cloud.N, cloud.Ni, cloud.Nd, cloud.Nn, cloud.Nr
+        self.Np = list(cloud.Np)
+        nodes = torch.as_tensor(cloud.sorted_nodes, dtype=torch.float64)
+        self.nodes = nodes.to(device)
+        self.x = self.nodes[:, 0].contiguous()
+        self.y = self.nodes[:, 1].contiguous()
+        nrm = getattr(cloud, "sorted_outward_normals", None)
+        nb = self.Nn + self.Nr + sum(self.Np)
+        if nb > 0:
+            nrm = torch.as_tensor(nrm, dtype=torch.float64).to(device).reshape(-1, 2)
+            if nrm.shape[0] != nb:
+                raise ValueError("sorted_outward_normals has %d rows, expected %d" % (nrm.shape[0], nb))
+            self.nx = nrm[:, 0].contiguous()
+            self.ny = nrm[:, 1].contiguous()
+            # Robin phi rows read the normal at index i-Ni-Nd-Nn (updes/assembly.py:206)
+            if self.Nr:
+                self.rnx = self.nx[0:self.Nr].contiguous()
+                self.rny = self.ny[0:self.Nr].contiguous()
+            else:
+                self.rnx = self.rny = None
+        else:
+            self.nx = self.ny = self.rnx = self.rny = None
+        # periodic pairs: group g occupies [start, start+Np[g]), node i pairs i + Np[g]/2
+        i1, i2 = [], []
+        start = self.Ni + self.Nd + self.Nn + self.Nr
+        for nbp in self.Np:
+            nc = nbp // 2
+            i1.extend(range(start, start + nc))
+            i2.extend(range(start + nc, start + 2 * nc))
+            start += nbp
+        self.Npd = len(i1)
+        if self.Npd:
+            self.pd1 = torch.tensor(i1, dtype=torch.int64, device=device)
+            self.pd2 = torch.tensor(i2, dtype=torch.int64, device=device)
+        else:
+            self.pd1 = self.pd2 = None
+        if self.Ni + self.Nd + self.Nn + self.Nr + 2 * self.Npd != self.N:
+            raise ValueError("periodic facets must pair up node for node (odd periodic group size)")
+
+
+def device_cloud(cloud):
+    dev = torch.device("cuda")
+    dc = cloud.__dict__.get("_rbfx_device_cloud")
+    if dc is None or dc.nodes.device != dev or dc.nodes.data_ptr() == 0:
+        dc = DeviceCloud(cloud, dev)
+        cloud.__dict__["_rbfx_device_cloud"] = dc
+    return dc
+
+
+# ---- operator descriptor ----------------------------------------------------
+class OpDesc:
+    """Coefficients of a lowered nodal operator (constant part + per-row part)."""
+
+    def __init__(self, rbf_const=None, mon_const=None, rbf_rows=None, mon_rows=None):
+        self.rbf_const = list(rbf_const or [0.0] * L.NOPS)
+        self.mon_const = list(mon_const or [0.0] * L.NOPS)
+        self.rbf_rows = list(rbf_rows or [None] * L.NOPS)
+        self.mon_rows = list(mon_rows or [None] * L.NOPS)
+
+    @staticmethod
+    def builtin(op):
+        c = [0.0] * L.NOPS
+        c[op] = 1.0
+        return OpDesc(c, list(c))
+
+    def key(self):
+        h = hashlib.sha1()
+        h.update(np.asarray(self.rbf_const + self.mon_const, dtype=np.float64).tobytes())
+        for r in self.rbf_rows + self.mon_rows:
+            if r is None:
+                h.update(b"-")
+            else:
+                h.update(r.detach().double().cpu().numpy().tobytes())
+        return h.hexdigest()
+
+    def to(self, device):
+        cv = lambda r: None if r is None else r.detach().to(device=device, dtype=torch.float64).contiguous()
+        return OpDesc(self.rbf_const, self.mon_const, [cv(r) for r in self.rbf_rows], [cv(r) for r in self.mon_rows])
+
+
+class Problem:
+    """A ctypes ``rbfx_problem`` plus the tensors it points to (kept alive)."""
+
+    def __init__(self, dc, kind, param, M, desc=None, robin_beta=None):
+        self.dc, self.kind, self.param, self.M = dc, int(kind), float(param), int(M)
+        self.n = dc.N + self.M
+        p = L.Problem()
+        p.N, p.M, p.kernel, p.flags, p.param = dc.N, self.M, self.kind, 0, self.param
+        p.d_x, p.d_y = dc.x.data_ptr(), dc.y.data_ptr()
+        p.Ni, p.Nd, p.Nn, p.Nr, p.Npd = dc.Ni, dc.Nd, dc.Nn, dc.Nr, dc.Npd
+        p.d_nx = dc.nx.data_ptr() if dc.nx is not None else None
+        p.d_ny = dc.ny.data_ptr() if dc.ny is not None else None
+        p.d_robin_normal_x = dc.rnx.data_ptr() if dc.rnx is not None else None
+        p.d_robin_normal_y = dc.rny.data_ptr() if dc.rny is not None else None
+        self.beta = None
+        if dc.Nr:
+            if robin_beta is None:
+                robin_beta = torch.zeros(dc.Nr, dtype=torch.float64)
+            self.beta = torch.as_tensor(robin_beta, dtype=torch.float64).to(dc.x.device).reshape(-1).contiguous()
+            if self.beta.numel() != dc.Nr:
+                raise ValueError("expected %d Robin coefficients, got %d" % (dc.Nr, self.beta.numel()))
+            p.d_beta = self.beta.data_ptr()
+        p.d_pd_i1 = dc.pd1.data_ptr() if dc.pd1 is not None else None
+        p.d_pd_i2 = dc.pd2.data_ptr() if dc.pd2 is not None else None
+        self.desc = desc.to(dc.x.device) if desc is not None else OpDesc()
+        for t in range(L.NOPS):
+            p.op_rbf_const[t] = float(self.desc.rbf_const[t])
+            p.op_mon_const[t] = float(self.desc.mon_const[t])
+            r, m = self.desc.rbf_rows[t], self.desc.mon_rows[t]
+            if r is not None and r.numel() != dc.Ni:
+                raise ValueError("per-row operator coefficients must have Ni=%d entries" % dc.Ni)
+            if m is not None and m.numel() != dc.Ni:
+                raise ValueError("per-row operator coefficients must have Ni=%d entries" % dc.Ni)
+            p.d_op_rbf_row[t] = r.data_ptr() if r is not None else None
+            p.d_op_mon_row[t] = m.data_ptr() if m is not None else None
+        self.p = p
+
+    def ref(self):
+        return ctypes.byref(self.p)
+
+
+# ---- primitives -------------------------------------------------------------
+def assemble(problem, matrix=L.MAT_K, dtype=torch.float64, also32=False):
+    """Dense row-major (n, ld) matrix; with also32 returns (fp64, fp32)."""
+    n = problem.n
+    ld = _ld(n)
+    dev = problem.dc.x.device
+    out64 = torch.empty((n, ld), dtype=torch.float64, device=dev) if (dtype == torch.float64 or also32) else None
+    out32 = torch.empty((n, ld), dtype=torch.float32, device=dev) if (dtype == torch.float32 or also32) else None
+    L.check(L.lib().rbfx_assemble(problem.ref(), matrix, 0, n, L.ptr(out64), ld, L.ptr(out32), ld, L.stream()))
+    if also32:
+        return out64, out32
+    return out64 if dtype == torch.float64 else out32
+
+
+def matvec(problem, x, matrix=L.MAT_K, trans=False):
+    x = x.detach().to(torch.float64).contiguous()
+    y = torch.empty_like(x)
+    wsb = L.lib().rbfx_matvec_workspace(problem.ref(), int(trans))
+    ws = L.workspace(wsb, x.device)
+    L.check(L.lib().rbfx_matvec(problem.ref(), matrix, int(trans), L.ptr(x), L.ptr(y), L.ptr(ws), ws.numel(),
+                                L.stream()))
+    return y
+
+
+def eval_ops(kind, param, centers, coeffs, points, mask):
+    """(7, P) operator values of the field ``coeffs`` at ``points`` (rows of mask set)."""
+    centers = centers.detach().to(torch.float64)
+    cx, cy = centers[:, 0].contiguous(), centers[:, 1].contiguous()
+    pts = points.detach().to(torch.float64).reshape(-1, 2)
+    px, py = pts[:, 0].contiguous(), pts[:, 1].contiguous()
+    coeffs = coeffs.detach().to(torch.float64).contiguous()
+    N = cx.numel()
+    M = coeffs.numel() - N
+    P = px.numel()
+    out = torch.zeros((L.NOPS, P), dtype=torch.float64, device=cx.device)
+    if P:
+        L.check(L.lib().rbfx_eval(int(kind), float(param), N, M, L.ptr(cx), L.ptr(cy), L.ptr(coeffs), P,
+                                  L.ptr(px), L.ptr(py), int(mask), L.ptr(out), L.stream()))
+    return out
+
+
+def eval_t(kind, param, centers, M, points, op, w):
+    """Adjoint of eval_ops for one operator: (N+M,) vector."""
+    centers = centers.detach().to(torch.float64)
+    cx, cy = centers[:, 0].contiguous(), centers[:, 1].contiguous()
+    pts = points.detach().to(torch.float64).reshape(-1, 2)
+    px, py = pts[:, 0].contiguous(), pts[:, 1].contiguous()
+    w = w.detach().to(torch.float64).contiguous()
+    N = cx.numel()
+    out = torch.empty(N + int(M), dtype=torch.float64, device=cx.device)
+    L.check(L.lib().rbfx_eval_t(int(kind), float(param), N, int(M), L.ptr(cx), L.ptr(cy), px.numel(), L.ptr(px),
+                                L.ptr(py), int(op), L.ptr(w), L.ptr(out), L.stream()))
+    return out
+
+
+class LU:
+    """In-place LU factorisation of a row-major (n, ld) matrix, FP64 or FP32."""
+
+    def __init__(self, mat, n):
+        self.n = n
+        self.a = mat
+        self.ld = mat.shape[1]
+        self.dtype = mat.dtype
+        dev = mat.device
+        lib = L.lib()
+        nbytes = 8 if self.dtype == torch.float64 else 4
+        self.ipiv = torch.empty(n, dtype=torch.int64, device=dev)
+        self.info = torch.zeros(1, dtype=torch.int32, device=dev)
+        ws = L.workspace(lib.rbfx_getrf_workspace(n, nbytes), dev)
+        fn = lib.rbfx_getrf_f64 if nbytes == 8 else lib.rbfx_getrf_f32
+        L.check(fn(n, L.ptr(mat), self.ld, L.ptr(self.ipiv), L.ptr(ws), ws.numel(), L.ptr(self.info), L.stream()))
+        info = int(self.info.item())
+        if info < 0:
+            raise L.RbfxError(-4, "LU factorisation failed (non-finite column or device-side wait expired)")
+        self.perm = torch.empty(n, dtype=torch.int64, device=dev)
+        self.dinv = torch.empty(lib.rbfx_getrs_dinv_elems(n), dtype=torch.float64, device=dev)
+        prep = lib.rbfx_getrs_prepare_f64 if nbytes == 8 else lib.rbfx_getrs_prepare_f32
+        L.check(prep(n, L.ptr(mat), self.ld, L.ptr(self.ipiv), L.ptr(self.perm), L.ptr(self.dinv), None, 0,
+                     L.stream()))
+        self.singular = int(self.info.item())
+        self._ws = L.workspace(lib.rbfx_getrs_workspace(n), dev)
+
+    def solve(self, b, trans=False):
+        """x with A x = b (or A^T x = b), b an (n,) FP64 tensor (not modified)."""
+        x = b.detach().to(torch.float64).contiguous().clone()
+        fn = L.lib().rbfx_getrs_f64 if self.dtype == torch.float64 else L.lib().rbfx_getrs_f32
+        L.check(fn(int(trans), self.n, L.ptr(self.a), self.ld, L.ptr(self.perm), L.ptr(self.dinv), L.ptr(x),
+                   L.ptr(self._ws), self._ws.numel(), L.stream()))
+        return x
+
+
+class SolveStats:
+    def __init__(self):
+        self.precision = "fp64"
+        self.ir_outer = 0
+        self.ir_inner = 0
+        self.ir_relres = None
+        self.fell_back = False
+
+    def as_dict(self):
+        return dict(precision=self.precision, ir_outer=self.ir_outer, ir_inner=self.ir_inner,
+                    ir_relres=self.ir_relres, fell_back=self.fell_back)
+
+
+class KFactor:
+    """Factorisation of the collocation matrix K for one (cloud, rbf, operator).
+
+    precision "fp64": FP64 LU.  precision "fp32_ir": FP32 LU (MFMA) used as
+    the preconditioner of GMRES-IR with matrix-free FP64 residuals; when
+    refinement stalls, an FP64 LU is built and used instead (fell_back).
+    """
+
+    def __init__(self, problem, precision="fp64", ir_tol=1e-12, ir_restart=200, ir_max_inner=600):
+        self.problem = problem
+        self.n = problem.n
+        self.precision = precision
+        self.ir_tol, self.ir_restart, self.ir_max_inner = ir_tol, ir_restart, ir_max_inner
+        self.lu64 = None
+        self.lu32 = None
+        self.last_stats = SolveStats()
+        if precision == "fp64":
+            self.lu64 = LU(assemble(problem, L.MAT_K, torch.float64), self.n)
+        elif precision == "fp32_ir":
+            self.lu32 = LU(assemble(problem, L.MAT_K, torch.float32), self.n)
+        else:
+            raise ValueError("precision must be 'fp64' or 'fp32_ir'")
+
+    def _fallback(self):
+        if self.lu64 is None:
+            self.lu64 = LU(assemble(self.problem, L.MAT_K, torch.float64), self.n)
+        return self.lu64
+
+    def solve(self, b, trans=False):
+        st = SolveStats()
+        st.precision = self.precision
+        if self.precision == "fp64" or (self.lu64 is not None and self.lu32 is None):
+            x = self.lu64.solve(b, trans)
+        else:
+            x = self._ir(b, trans, st)
+        self.last_stats = st
+        return x
+
+    def _ir(self, b, trans, st):
+        lib = L.lib()
+        n = self.n
+        b = b.detach().to(torch.float64).contiguous()
+        x = torch.empty_like(b)
+        ws = L.workspace(max(lib.rbfx_gmres_ir_workspace(n, self.ir_restart),
+                             lib.rbfx_matvec_workspace(self.problem.ref(), int(trans))), b.device)
+        stats = L.IrStats()
+        lu = self.lu32
+        status = lib.rbfx_gmres_ir(int(trans), n, self.problem.ref(), None, 0, L.ptr(lu.a), lu.ld, L.ptr(lu.perm),
+                                   L.ptr(lu.dinv), L.ptr(b), L.ptr(x), float(self.ir_tol), int(self.ir_restart),
+                                   int(self.ir_max_inner), L.ptr(ws), ws.numel(), ctypes.byref(stats), L.stream())
+        st.ir_outer, st.ir_inner, st.ir_relres = stats.outer_iters, stats.inner_iters, stats.final_relres
+        if status == L.RBFX_ERR_STALL:
+            st.fell_back = True
+            return self._fallback().solve(b, trans)
+        L.check(status)
+        return x
+
+
+class _LRU:
+    def __init__(self, size):
+        self.size = size
+        self.d = collections.OrderedDict()
+
+    def get(self, key):
+        v = self.d.get(key)
+        if v is not None:
+            self.d.move_to_end(key)
+        return v
+
+    def put(self, key, val):
+        self.d[key] = val
+        self.d.move_to_end(key)
+        while len(self.d) > self.size:
+            self.d.popitem(last=False)
+
+    def clear(self):
+        self.d.clear()
+
+
+_K_CACHE = _LRU(2)
+_A_CACHE = _LRU(2)
+CACHE_ENABLED = [True]
+
+
+def clear_cache():
+    """Drop cached factorisations (they can hold many GB at large N)."""
+    _K_CACHE.clear()
+    _A_CACHE.clear()
+    torch.cuda.empty_cache()
+
+
+def a_factor(cloud, kind, param, M):
+    """FP64 LU of the interpolation matrix A (updes/assembly.py:62-90), cached."""
+    key = (id(cloud), kind, param, M)
+    f = _A_CACHE.get(key) if CACHE_ENABLED[0] else None
+    if f is None:
+        prob = Problem(device_cloud(cloud), kind, param, M)
+        f = LU(assemble(prob, L.MAT_A, torch.float64), prob.n)
+        f.problem = prob
+        if CACHE_ENABLED[0]:
+            _A_CACHE.put(key, f)
+    return f
+
+
+def k_factor(cloud, kind, param, M, desc, robin_beta, precision, **ir):
+    rkey = None if robin_beta is None else hashlib.sha1(
+        torch.as_tensor(robin_beta, dtype=torch.float64).cpu().numpy().tobytes()).hexdigest()
+    key = (id(cloud), kind, param, M, desc.key(), rkey, precision, tuple(sorted(ir.items())))
+    f = _K_CACHE.get(key) if CACHE_ENABLED[0] else None
+    if f is None:
+        prob = Problem(device_cloud(cloud), kind, param, M, desc, robin_beta)
+        f = KFactor(prob, precision, **ir)
+        if CACHE_ENABLED[0]:
+            _K_CACHE.put(key, f)
+    return f
