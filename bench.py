@@ -1,0 +1,224 @@
+"""Benchmark: solves/s + grads/s of the dense RBF-collocation path at N = 40,000.
+
+Workload (large Poisson, 200 x 200 nodes, N = 40,000, n = N + 6 = 40,006):
+  -Lap u = f on the unit square, polyharmonic r^3, max_degree = 2;
+  Neumann on South, Dirichlet on North/West/East; boundary data are sums of
+  8 Fourier modes with N(0,1)/k^2 coefficients (seeded), the source is
+  f = sum_{k,l<=8} b_kl sin(k pi x) sin(l pi y) with b_kl ~ N(0,1)/(k^2+l^2);
+  loss L = 1/2 mean(u^2) over the interior nodes.
+One step = one solve (assembly + factorisation + solve + nodal values, for a
+new operator/boundary pair: the factorisation cache is cleared) and one
+gradient (dL/db for the 64 source coefficients through the adjoint).  Both
+precision paths are measured: FP64 LU, and FP32-MFMA LU + FP64 GMRES-IR
+(falling back to FP64 LU when refinement stalls).  `value` is the
+throughput of the mixed-precision path; the FP64 path and the error of the
+mixed path against it are reported beside it.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--side 200]
+"""
+import argparse
+import ctypes
+import json
+import math
+import os
+import sys
+import time
+
+import numpy as np
+import torch
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, HERE)
+
+PEAK_FP64_MFMA = 78.6e12
+PEAK_FP32_MFMA = 157.3e12
+PEAK_HBM = 8.0e12
+
+
+def fourier_problem(side, seed=0):
+    """Cloud, boundary data and source basis of the large-Poisson workload."""
+    import updes_amd as u
+    ft = {"North": "d", "South": "n", "East": "d", "West": "d"}
+    cloud = u.SquareCloud(Nx=side, Ny=side, facet_types=ft)
+    rng = np.random.default_rng(seed)
+    k = np.arange(1, 9)
+    bcs = {}
+    for f in ["North", "South", "East", "West"]:
+        a = rng.standard_normal(8) / k ** 2
+        ids = cloud.facet_nodes[f]
+        pts = cloud.sorted_nodes[ids].cpu().numpy()
+        s = pts[:, 0] if f in ("North", "South") else pts[:, 1]
+        bcs[f] = torch.as_tensor((a[None, :] * np.sin(np.pi * np.outer(s, k))).sum(1), device="cuda")
+    kk, ll = np.meshgrid(k, k, indexing="ij")
+    b = torch.as_tensor((rng.standard_normal((8, 8)) / (kk ** 2 + ll ** 2)).reshape(-1), device="cuda")
+    xi = cloud.sorted_nodes[:cloud.Ni].to("cuda")
+    kv = torch.arange(1, 9, dtype=torch.float64, device="cuda")
+    sx = torch.sin(math.pi * xi[:, 0:1] * kv[None, :])       # (Ni, 8)
+    sy = torch.sin(math.pi * xi[:, 1:2] * kv[None, :])
+    basis = (sx[:, :, None] * sy[:, None, :]).reshape(-1, 64)  # (Ni, 64): b_kl index k*8+l
+    return cloud, bcs, b, basis
+
+
+def lap_op(x, center=None, rbf=None, monomial=None, fields=None):
+    import updes_amd as u
+    return u.nodal_laplacian(x, center, rbf, monomial)
+
+
+def solve_and_grad(cloud, bcs, b, basis, precision, want_grad=True):
+    import updes_amd as u
+    bb = b.detach().clone().requires_grad_(want_grad)
+    f_int = basis @ bb
+
+    def rhs(x, centers, rbf, fields):
+        return -f_int
+
+    sol = u.pde_solver(lap_op, rhs, cloud, bcs, u.polyharmonic, 2, precision=precision)
+    vals = sol.vals
+    loss = 0.5 * (vals[:cloud.Ni] ** 2).mean()
+    g = None
+    if want_grad:
+        loss.backward()
+        g = bb.grad.detach().clone()
+    return vals.detach(), loss.detach(), g
+
+
+def profile_step(fn):
+    """Run fn() with per-launch HIP-event timing; returns {kernel: stats}."""
+    from updes_amd import _lib as L
+    lib = L.lib()
+    lib.rbfx_profile_begin.restype = ctypes.c_int
+    lib.rbfx_profile_end.restype = ctypes.c_int
+
+    class Entry(ctypes.Structure):
+        _fields_ = [("name", ctypes.c_char * 48), ("launches", ctypes.c_int64), ("total_ms", ctypes.c_double),
+                    ("flops", ctypes.c_double), ("bytes", ctypes.c_double)]
+
+    lib.rbfx_profile_end.argtypes = [ctypes.POINTER(Entry), ctypes.c_int, ctypes.POINTER(ctypes.c_int)]
+    lib.rbfx_profile_begin()
+    fn()
+    torch.cuda.synchronize()
+    arr = (Entry * 64)()
+    n = ctypes.c_int(0)
+    lib.rbfx_profile_end(arr, 64, ctypes.byref(n))
+    out = {}
+    for i in range(min(n.value, 64)):
+        e = arr[i]
+        out[e.name.decode()] = dict(launches=int(e.launches), total_ms=e.total_ms, flops=e.flops, bytes=e.bytes)
+    return out
+
+
+def cpu_baseline(seconds_budget=30.0):
+    """Restated reference algorithm (inv(A), D inv(A), QR) on a 60 x 60 sample of
+    the same workload, timed on this host, scaled by (n_40k / n_sample)^3."""
+    sys.path.insert(0, HERE)
+    from oracle import assembly as OA, cloud as OC, rbf as R, solver as OS
+    side = 60
+    ft = {"North": "d", "South": "n", "East": "d", "West": "d"}
+    oc = OC.SquareCloud(side, side, ft)
+    rng = np.random.default_rng(0)
+    x, y = oc.sorted_nodes.T
+    q = np.where(np.arange(oc.N) < oc.Ni, -np.sin(np.pi * x) * np.sin(np.pi * y), rng.standard_normal(oc.N) * 0.1)
+    desc = {"rbf": [0, 0, 0, 1, 0, 0, 0], "mon": [0, 0, 0, 1, 0, 0, 0]}
+    times = []
+    t_start = time.time()
+    while len(times) < 3 and (time.time() - t_start) < seconds_budget:
+        t0 = time.perf_counter()
+        OS.pde_solver_bpath(desc, oc, R.POLYHARMONIC, 1.0, 6, q)
+        times.append(time.perf_counter() - t0)
+    t = float(np.median(times))
+    n_s, n_full = oc.N + 6, 40006
+    scale = (n_full / n_s) ** 3
+    cores = len(os.sched_getaffinity(0))
+    return {"value": 1.0 / (t * scale), "unit": "solves/s", "cores": cores, "kind": "port",
+            "sample": "restated reference B-path (numpy/OpenBLAS FP64: assembly, inv(A), D@inv(A), QR solve, "
+                      "coefficients) on the same workload at 60x60 (N=3,600), median of %d runs = %.3f s, "
+                      "scaled to N=40,000 by (40006/3606)^3 = %.0f" % (len(times), t, scale)}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=2)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--side", type=int, default=200)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--precisions", default="fp64,fp32_ir")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py measures one GPU: the dense solve of one collocation system runs on a single MI355X; "
+              "multi-GPU runs are not part of this build (see DESIGN.md for how the path would shard).",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import updes_amd as u
+    from updes_amd import engine as E
+    from updes_amd import _lib as L
+    L.require_gpu()
+    torch.cuda.set_device(0)
+    cloud, bcs, b, basis = fourier_problem(args.side)
+    n = cloud.N + 6
+    print("[bench] N=%d n=%d, precisions %s" % (cloud.N, n, args.precisions), file=sys.stderr, flush=True)
+
+    results = {}
+    for prec in args.precisions.split(","):
+        for w in range(args.warmup):
+            u.clear_cache()
+            solve_and_grad(cloud, bcs, b, basis, prec)
+            print("[bench] %s warmup %d done" % (prec, w), file=sys.stderr, flush=True)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        stats = None
+        for s in range(args.steps):
+            u.clear_cache()
+            vals, loss, g = solve_and_grad(cloud, bcs, b, basis, prec)
+            stats = u.last_solve_stats()
+        torch.cuda.synchronize()
+        dt = (time.perf_counter() - t0) / args.steps
+        results[prec] = dict(ms=dt * 1e3, vals=vals, grad=g, loss=float(loss), stats=stats)
+        print("[bench] %s: %.1f ms per solve+grad, stats %s" % (prec, dt * 1e3, stats), file=sys.stderr, flush=True)
+
+    # kernel-level profile of one FP64 and one FP32 factorisation (outside the timed loop)
+    prof = {}
+    for prec in args.precisions.split(","):
+        u.clear_cache()
+        prof[prec] = profile_step(lambda: solve_and_grad(cloud, bcs, b, basis, prec))
+    head = "fp32_ir" if "fp32_ir" in results else list(results)[0]
+    ref = results.get("fp64")
+    out = {"metric": "solves/s + grads/s at N=40k, FP32-LU+FP64 IR vs FP64 LU, rel-L2 <= 1e-8 vs FP64"}
+    r = results[head]
+    out.update(value=1e3 / r["ms"], unit="solves/s", n_gpus=1, steps=args.steps, warmup=args.warmup,
+               ms_per_step=r["ms"], higher_is_better=True, scaling="weak", vs_baseline=None,
+               dtype="f32-lu+f64-ir" if head == "fp32_ir" else "f64", data="synthetic")
+    out["config"] = {"workload": "large Poisson %dx%d (N=%d, n=%d), r^3, max_degree 2, Neumann South, Fourier "
+                                 "boundary data and 64-coefficient source; step = 1 solve + 1 adjoint gradient"
+                                 % (args.side, args.side, cloud.N, n), "N": cloud.N, "n": n,
+                     "grads_per_s": 1e3 / r["ms"]}
+    for prec, rr in results.items():
+        d = {"solves_per_s": 1e3 / rr["ms"], "grads_per_s": 1e3 / rr["ms"], "ms_per_solve_plus_grad": rr["ms"],
+             "ir": rr["stats"]}
+        if ref is not None and prec != "fp64":
+            d["rel_l2_vals_vs_fp64"] = float(torch.linalg.norm(rr["vals"] - ref["vals"]) / torch.linalg.norm(ref["vals"]))
+            d["rel_l2_grad_vs_fp64"] = float(torch.linalg.norm(rr["grad"] - ref["grad"]) / torch.linalg.norm(ref["grad"]))
+        out[prec] = d
+    if ref is not None and head != "fp64":
+        out["speedup_vs_fp64"] = ref["ms"] / r["ms"]
+    # roofline of the dominant kernel (trailing-update GEMM) in the headline path
+    gname = "gemm_f32" if head == "fp32_ir" and "gemm_f32" in prof.get(head, {}) else "gemm_f64"
+    pk = prof[head].get(gname) or prof[head].get("gemm_f64")
+    peak = PEAK_FP32_MFMA if gname == "gemm_f32" else PEAK_FP64_MFMA
+    achieved = pk["flops"] / (pk["total_ms"] * 1e-3)
+    out["roofline"] = {"bound": "mfma", "kernel": gname, "achieved": achieved / 1e12, "peak": peak / 1e12,
+                       "unit": "TFLOP/s", "frac": achieved / peak, "traffic": None,
+                       "launches": pk["launches"], "avg_launch_ms": pk["total_ms"] / pk["launches"]}
+    out["kernels"] = {p: {k: {"launches": v["launches"], "total_ms": round(v["total_ms"], 3),
+                              "tflops": (v["flops"] / (v["total_ms"] * 1e-3) / 1e12) if v["flops"] else None,
+                              "gbps": (v["bytes"] / (v["total_ms"] * 1e-3) / 1e9) if v["bytes"] else None}
+                          for k, v in pr.items()} for p, pr in prof.items()}
+    if not args.no_cpu_baseline:
+        print("[bench] cpu baseline ...", file=sys.stderr, flush=True)
+        out["cpu_baseline"] = cpu_baseline()
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -200,6 +200,32 @@ int rbfx_gmres_ir(int trans, int64_t n, const rbfx_problem* p, const double* d_K
                   int32_t max_inner, void* d_work, size_t work_bytes,
                   rbfx_ir_stats* stats, void* stream);
 
+/* ---- per-launch timing ----------------------------------------------------
+ * Between rbfx_profile_begin() and rbfx_profile_end() every launch of the
+ * main kernels is bracketed by HIP events on its own stream; _end
+ * synchronises on them and returns per-kernel totals with the algorithmic
+ * flops / bytes of each launch.  No reference counterpart (the reference
+ * times with time.time / process_time around whole calls).                 */
+typedef struct rbfx_prof_entry {
+  char name[48];
+  int64_t launches;
+  double total_ms;
+  double flops;   /* algorithmic flops summed over the launches   */
+  double bytes;   /* algorithmic HBM bytes summed over the launches */
+} rbfx_prof_entry;
+int rbfx_profile_begin(void);
+int rbfx_profile_end(rbfx_prof_entry* out, int max_entries, int* n_entries);
+
+/* ---- test hook --------------------------------------------------------------
+ * C = beta C + alpha A B (row-major, beta 0 or 1) with the MFMA kernel that
+ * performs the LU trailing updates.                                         */
+int rbfx_gemm_f64(int64_t m, int64_t n, int64_t k, double alpha, const double* d_A,
+                  int64_t lda, const double* d_B, int64_t ldb, double beta, double* d_C,
+                  int64_t ldc, void* stream);
+int rbfx_gemm_f32(int64_t m, int64_t n, int64_t k, float alpha, const float* d_A,
+                  int64_t lda, const float* d_B, int64_t ldb, float beta, float* d_C,
+                  int64_t ldc, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,66 @@
+"""Writes the golden vectors under tests/golden from the pinned oracle.
+
+The reference's own tests hold no vectors for pde_solver (SURVEY: their
+tightest solve tolerance is 1e-2), and the reference (JAX) cannot run here,
+so these fixtures are produced by the oracle after it has passed its pins
+(tests/test_oracle_pins.py: SymPy/mpmath closed forms, an independent
+mpmath brute force of the reference loops, invariants).  Run:
+    python tests/golden/make_golden.py
+"""
+import os
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.dirname(os.path.dirname(HERE)))
+
+from oracle import assembly as OA, cloud as OC, rbf as R, solver as OS  # noqa: E402
+from tests import _cases as C  # noqa: E402
+from tests.test_oracle_pins import BRUTE_CASES, _oracle_case  # noqa: E402
+
+
+def readme_laplace():
+    ft = C.README_FACETS
+    oc = OC.SquareCloud(30, 20, ft)
+    bc = {f: (np.sin(np.pi * oc.sorted_nodes[oc.facet_nodes[f], 0]) if f == "North"
+              else np.zeros(len(oc.facet_nodes[f]))) for f in ft}
+    q = OA.assemble_q(oc, 0.0, bc)
+    u, c, B = OS.pde_solver_bpath(C.LAPLACIAN, oc, R.POLYHARMONIC, 1.0, 3, q)
+    np.savez_compressed(os.path.join(HERE, "readme_laplace.npz"), q=q, vals=u, coeffs=c,
+                        B_diag=np.diag(B), B_row0=B[0], nodes=oc.sorted_nodes)
+
+
+def small_cases():
+    out = {}
+    for i in range(len(BRUTE_CASES)):
+        oc, kind, param, M, desc, rb, q = _oracle_case(i)
+        u, c, _ = OS.pde_solver_bpath(desc, oc, kind, param, M, q, rb)
+        K = OA.assemble_K(desc, oc, kind, param, M, rb)
+        A = OA.assemble_A(oc, kind, param, M)
+        out["case%d_K" % i] = K
+        out["case%d_A" % i] = A
+        out["case%d_q" % i] = q
+        out["case%d_vals" % i] = u
+        out["case%d_coeffs" % i] = c
+        out["case%d_nodes" % i] = oc.sorted_nodes
+        out["case%d_beta" % i] = rb if rb is not None else np.zeros(0)
+    np.savez_compressed(os.path.join(HERE, "small_cases.npz"), **out)
+
+
+def eval_fixture():
+    oc = OC.SquareCloud(10, 9, C.README_FACETS, jitter=C.jitter(10, 9, 0.1, 3))
+    rng = np.random.default_rng(11)
+    coeffs = rng.standard_normal(oc.N + 6)
+    pts = np.concatenate([rng.uniform(0, 1, (17, 2)), oc.sorted_nodes[:5]])
+    out = {"centers": oc.sorted_nodes, "coeffs": coeffs, "points": pts}
+    for kind, param in C.KINDS:
+        out["ops_%d_%g" % (kind, param)] = OS.eval_ops(pts, coeffs, oc.sorted_nodes, kind, param)
+    np.savez_compressed(os.path.join(HERE, "eval_ops.npz"), **out)
+
+
+if __name__ == "__main__":
+    readme_laplace()
+    small_cases()
+    eval_fixture()
+    print("written to", HERE)

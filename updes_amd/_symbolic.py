@@ -399,21 +399,26 @@ _SYM = (OpTerm, OpVec, EvalExpr, EvalVec)
 
 def _dispatch(func, args, kwargs):
     """numpy / torch functions applied to symbolic values."""
-    name = getattr(func, "__name__", str(func))
-    if name in ("dot", "matmul", "inner", "vdot"):
+    name = getattr(func, "__name__", str(func)).strip("_")
+    if name in ("dot", "matmul", "inner", "vdot", "rmatmul"):
         a, b = args[0], args[1]
         if isinstance(b, (OpVec, EvalVec)):
             return b.dot(a)
         if isinstance(a, (OpVec, EvalVec)):
             return a.dot(b)
-    if name in ("multiply", "mul"):
-        return args[0] * args[1]
-    if name in ("add",):
-        return args[0] + args[1]
-    if name in ("subtract", "sub"):
-        return args[0] - args[1]
-    if name in ("divide", "true_divide", "div"):
-        return args[0] / args[1]
+    binops = {"multiply": "mul", "mul": "mul", "rmul": "mul", "add": "add", "radd": "add", "subtract": "sub",
+              "sub": "sub", "rsub": "rsub", "divide": "truediv", "true_divide": "truediv", "div": "truediv",
+              "truediv": "truediv", "rtruediv": "rtruediv", "rdiv": "rtruediv"}
+    if name in binops:
+        op = binops[name]
+        a, b = args[0], args[1]
+        if op == "rsub":                # torch: rsub(a, b) = b - a
+            a, b, op = b, a, "sub"
+        if op == "rtruediv":
+            a, b, op = b, a, "truediv"
+        if isinstance(a, _SYM):
+            return getattr(a, "__%s__" % op)(b)
+        return getattr(b, "__r%s__" % op)(a)
     if name in ("negative", "neg"):
         return -args[0]
     if name == "sum" and isinstance(args[0], (OpVec, EvalVec)):

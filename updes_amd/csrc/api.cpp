@@ -50,3 +50,96 @@ extern "C" int rbfx_init(void) {
   (void)rbfx::device_info();
   return RBFX_OK;
 }
+
+// ---- per-launch timing ------------------------------------------------------
+#include <mutex>
+#include <string>
+#include <vector>
+
+namespace rbfx {
+namespace {
+struct ProfRec {
+  std::string name;
+  hipEvent_t a = nullptr, b = nullptr;
+  double flops = 0, bytes = 0;
+};
+std::mutex g_prof_mu;
+bool g_prof_on = false;
+std::vector<ProfRec> g_prof;
+std::vector<hipEvent_t> g_pool;
+
+hipEvent_t take_event() {
+  if (!g_pool.empty()) {
+    hipEvent_t e = g_pool.back();
+    g_pool.pop_back();
+    return e;
+  }
+  hipEvent_t e = nullptr;
+  if (hipEventCreate(&e) != hipSuccess) return nullptr;
+  return e;
+}
+}  // namespace
+
+int prof_start(const char* name, hipStream_t s) {
+  std::lock_guard<std::mutex> lk(g_prof_mu);
+  if (!g_prof_on) return -1;
+  ProfRec r;
+  r.name = name;
+  r.a = take_event();
+  r.b = take_event();
+  if (!r.a || !r.b) return -1;
+  if (hipEventRecord(r.a, s) != hipSuccess) return -1;
+  g_prof.push_back(r);
+  return (int)g_prof.size() - 1;
+}
+
+void prof_stop(int id, hipStream_t s, double flops, double bytes) {
+  if (id < 0) return;
+  std::lock_guard<std::mutex> lk(g_prof_mu);
+  if (id >= (int)g_prof.size()) return;
+  g_prof[id].flops = flops;
+  g_prof[id].bytes = bytes;
+  (void)hipEventRecord(g_prof[id].b, s);
+}
+}  // namespace rbfx
+
+extern "C" int rbfx_profile_begin(void) {
+  std::lock_guard<std::mutex> lk(rbfx::g_prof_mu);
+  for (auto& r : rbfx::g_prof) {
+    rbfx::g_pool.push_back(r.a);
+    rbfx::g_pool.push_back(r.b);
+  }
+  rbfx::g_prof.clear();
+  rbfx::g_prof_on = true;
+  return RBFX_OK;
+}
+
+extern "C" int rbfx_profile_end(rbfx_prof_entry* out, int max_entries, int* n_entries) {
+  std::vector<rbfx_prof_entry> agg;
+  {
+    std::lock_guard<std::mutex> lk(rbfx::g_prof_mu);
+    rbfx::g_prof_on = false;
+    for (auto& r : rbfx::g_prof) {
+      RBFX_HIP(hipEventSynchronize(r.b));
+      float ms = 0.f;
+      RBFX_HIP(hipEventElapsedTime(&ms, r.a, r.b));
+      rbfx_prof_entry* e = nullptr;
+      for (auto& x : agg)
+        if (r.name == x.name) e = &x;
+      if (!e) {
+        rbfx_prof_entry z{};
+        snprintf(z.name, sizeof(z.name), "%s", r.name.c_str());
+        agg.push_back(z);
+        e = &agg.back();
+      }
+      e->launches += 1;
+      e->total_ms += ms;
+      e->flops += r.flops;
+      e->bytes += r.bytes;
+    }
+  }
+  const int n = (int)agg.size();
+  for (int i = 0; i < n && i < max_entries; ++i) out[i] = agg[i];
+  if (n_entries) *n_entries = n;
+  return RBFX_OK;
+}

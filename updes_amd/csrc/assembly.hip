@@ -408,6 +408,7 @@ extern "C" int rbfx_assemble(const rbfx_problem* p, int matrix, int64_t row0, in
   dim3 grid((unsigned)ceil_div(P.n, kAsmCols), (unsigned)ceil_div(nrows, kAsmRows));
   RBFX_REQUIRE(grid.y <= 65535u * 1024u, "too many rows");
   hipStream_t s = as_stream(stream);
+  const int pid = prof_start(d_out64 ? (d_out32 ? "assemble_f64_f32" : "assemble_f64") : "assemble_f32", s);
   if (d_out64 && d_out32)
     hipLaunchKernelGGL((assemble_kernel<true, true>), grid, dim3(kAsmThreads), 0, s, P, row0,
                        nrows, d_out64, ld64, d_out32, ld32);
@@ -417,6 +418,7 @@ extern "C" int rbfx_assemble(const rbfx_problem* p, int matrix, int64_t row0, in
   else
     hipLaunchKernelGGL((assemble_kernel<false, true>), grid, dim3(kAsmThreads), 0, s, P, row0,
                        nrows, d_out64, ld64, d_out32, ld32);
+  prof_stop(pid, s, 0.0, (double)nrows * P.n * ((d_out64 ? 8.0 : 0.0) + (d_out32 ? 4.0 : 0.0)));
   RBFX_CHECK_LAUNCH();
   return RBFX_OK;
 }
@@ -444,8 +446,10 @@ extern "C" int rbfx_matvec(const rbfx_problem* p, int matrix, int trans, const d
   RBFX_REQUIRE(d_x && d_y && d_x != d_y, "matvec needs distinct x and y");
   hipStream_t s = as_stream(stream);
   if (!trans) {
+    const int pid = prof_start("matvec", s);
     hipLaunchKernelGGL(matvec_kernel, dim3((unsigned)ceil_div(P.n, kMvRows)), dim3(kMvThreads), 0,
                        s, P, d_x, d_y);
+    prof_stop(pid, s, 0.0, 16.0 * P.n);
     RBFX_CHECK_LAUNCH();
     return RBFX_OK;
   }

@@ -75,3 +75,11 @@ inline size_t ws_round(size_t bytes) { return (bytes + 255) & ~size_t(255); }
 constexpr uint32_t kSpinLimit = 1u << 22;
 
 }  // namespace rbfx
+
+namespace rbfx {
+// Optional per-launch timing with HIP events on the launch stream
+// (rbfx_profile_begin / rbfx_profile_end).  When disabled, prof_start()
+// returns -1 and prof_stop() does nothing.
+int prof_start(const char* name, hipStream_t s);
+void prof_stop(int id, hipStream_t s, double flops, double bytes);
+}  // namespace rbfx

@@ -170,8 +170,10 @@ static int gemm(int64_t m, int64_t n, int64_t k, T alpha, const T* A, int64_t ld
   const int64_t tm = ceil_div(m, GBM), tn = ceil_div(n, GBN);
   const int64_t tiles = tm * tn;
   const int64_t grid = ((tiles + 7) / 8) * 8;
+  const int pid = prof_start(sizeof(T) == 8 ? "gemm_f64" : "gemm_f32", s);
   hipLaunchKernelGGL((gemm_kernel<T>), dim3((unsigned)grid), dim3(GTHREADS), 0, s, m, n, k, alpha, A,
                      lda, B, ldb, beta, C, ldc, tm, tn);
+  prof_stop(pid, s, 2.0 * m * n * k, (double)sizeof(T) * (2.0 * m * n + (double)m * k + (double)k * n));
   RBFX_CHECK_LAUNCH();
   return RBFX_OK;
 }
@@ -590,8 +592,10 @@ static int leaf(LuCtx<T>& c, int64_t r0, int w) {
   a.info = c.info;
   const size_t smem = need(R);
   void* args[] = {&a};
+  const int pid = prof_start(sizeof(T) == 8 ? "panel_f64" : "panel_f32", c.s);
   RBFX_HIP(hipLaunchCooperativeKernel(reinterpret_cast<const void*>(&panel_kernel<T>), dim3(G),
                                       dim3(kPanelThreads), args, (unsigned)smem, c.s));
+  prof_stop(pid, c.s, (double)m * w * w, 2.0 * sizeof(T) * m * w);
   c.bar_count += (unsigned)G * (unsigned)(w + 1);
   return RBFX_OK;
 }
@@ -605,6 +609,7 @@ static int apply_swaps(LuCtx<T>& c, int64_t s, int64_t e, int64_t c0, int64_t nc
                      c.plan_src, c.plan_count);
   RBFX_CHECK_LAUNCH();
   const int maxcnt = (int)(2 * (e - s));
+  const int pid = prof_start("row_swaps", c.s);
   // process the columns in slices that fit the staging buffer
   const int64_t slice = std::max<int64_t>(256, c.swap_tmp_elems / maxcnt);
   for (int64_t off = 0; off < nc; off += slice) {
@@ -617,6 +622,7 @@ static int apply_swaps(LuCtx<T>& c, int64_t s, int64_t e, int64_t c0, int64_t nc
                        c.plan_dst, c.plan_count, c.swap_tmp);
     RBFX_CHECK_LAUNCH();
   }
+  prof_stop(pid, c.s, 0.0, 4.0 * sizeof(T) * (e - s) * nc);
   return RBFX_OK;
 }
 
@@ -625,9 +631,11 @@ static int trsm(LuCtx<T>& c, int64_t r0, int W, int64_t col, int64_t ncols) {
   if (ncols <= 0) return RBFX_OK;
   const size_t smem = (size_t)W * kTrsmCols * sizeof(T);
   RBFX_REQUIRE(smem <= c.lds_budget, "trsm strip does not fit LDS");
+  const int pid = prof_start(sizeof(T) == 8 ? "trsm_f64" : "trsm_f32", c.s);
   hipLaunchKernelGGL((trsm_llnu_kernel<T>), dim3((unsigned)ceil_div(ncols, kTrsmCols)),
                      dim3(kTrsmThreads), smem, c.s, c.A + r0 * c.lda + r0, c.lda,
                      c.A + r0 * c.lda + col, c.lda, W, ncols);
+  prof_stop(pid, c.s, (double)W * W * ncols, 2.0 * sizeof(T) * W * ncols);
   RBFX_CHECK_LAUNCH();
   return RBFX_OK;
 }

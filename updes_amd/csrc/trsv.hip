@@ -200,6 +200,7 @@ static int launch_sweep(bool fwd, bool trans, int which, const T* F, int64_t lda
   hipLaunchKernelGGL(gather_perm_kernel, dim3(1), dim3(1), 0, s, (const double*)nullptr,
                      (const int64_t*)nullptr, (int64_t)0, (double*)nullptr, ctl);
   RBFX_CHECK_LAUNCH();
+  const int pid = prof_start(sizeof(T) == 8 ? "sweep_f64" : "sweep_f32", s);
 #define RBFX_SWEEP(F_, T_, W_)                                                               \
   hipLaunchKernelGGL((sweep_kernel<T, F_, T_, W_>), dim3(nb), dim3(kTsThreads), 0, s, F, lda, n, \
                      dinv, y, ctl)
@@ -212,6 +213,7 @@ static int launch_sweep(bool fwd, bool trans, int which, const T* F, int64_t lda
     return RBFX_ERR_ARG;
   }
 #undef RBFX_SWEEP
+  prof_stop(pid, s, (double)n * n, (double)sizeof(T) * n * (n + 1) / 2.0);
   RBFX_CHECK_LAUNCH();
   return RBFX_OK;
 }

@@ -321,10 +321,21 @@ def _coef_tensor(v, rows, dev):
 
 def lower_diff_operator(diff_operator, cloud, rbf, diff_args=None):
     """Trace ``diff_operator`` into an engine.OpDesc (per-row coefficients over Ni rows)."""
-    dev = _dev()
+    _dev()
     dc = E.device_cloud(cloud)
-    Ni = dc.Ni
-    x = dc.nodes[:Ni].T
+    return trace_diff_operator(diff_operator, dc.nodes[:dc.Ni].T, rbf, diff_args)
+
+
+def trace_diff_operator(diff_operator, x, rbf, diff_args=None):
+    """Descriptor of ``diff_operator`` at the collocation points x (2, Ni) on x's device.
+
+    The reference evaluates the operator per (node, centre) pair with
+    ``fields[i]`` = the diff_args values at node i, or ones when there are
+    none (``updes/assembly.py:114-135``); here ``fields[k]`` is the (Ni,)
+    vector of the k-th argument.
+    """
+    dev = x.device
+    Ni = x.shape[1]
     if diff_args:
         fields = torch.stack([torch.as_tensor(a, dtype=torch.float64).to(dev)[:Ni] for a in diff_args], dim=0)
     else:
