@@ -86,18 +86,10 @@ def profile_step(fn):
     """Run fn() with per-launch HIP-event timing; returns {kernel: stats}."""
     from updes_amd import _lib as L
     lib = L.lib()
-    lib.rbfx_profile_begin.restype = ctypes.c_int
-    lib.rbfx_profile_end.restype = ctypes.c_int
-
-    class Entry(ctypes.Structure):
-        _fields_ = [("name", ctypes.c_char * 48), ("launches", ctypes.c_int64), ("total_ms", ctypes.c_double),
-                    ("flops", ctypes.c_double), ("bytes", ctypes.c_double)]
-
-    lib.rbfx_profile_end.argtypes = [ctypes.POINTER(Entry), ctypes.c_int, ctypes.POINTER(ctypes.c_int)]
-    lib.rbfx_profile_begin()
+    L.check(lib.rbfx_profile_begin())
     fn()
     torch.cuda.synchronize()
-    arr = (Entry * 64)()
+    arr = (L.ProfEntry * 64)()
     n = ctypes.c_int(0)
     lib.rbfx_profile_end(arr, 64, ctypes.byref(n))
     out = {}

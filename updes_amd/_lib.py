@@ -52,6 +52,11 @@ class IrStats(ctypes.Structure):
                 ("stalled", _i32), ("reserved", _i32)]
 
 
+class ProfEntry(ctypes.Structure):
+    """Mirror of ``rbfx_prof_entry``."""
+    _fields_ = [("name", ctypes.c_char * 48), ("launches", _i64), ("total_ms", _d), ("flops", _d), ("bytes", _d)]
+
+
 # name -> (restype, argtypes)
 _SIGS = {
     "rbfx_abi_version": (ctypes.c_int, []),
@@ -76,6 +81,8 @@ _SIGS = {
     "rbfx_gmres_ir_workspace": (_sz, [_i64, _i32]),
     "rbfx_gmres_ir": (ctypes.c_int, [ctypes.c_int, _i64, ctypes.POINTER(Problem), _p, _i64, _p, _i64, _p, _p,
                                      _p, _p, _d, _i32, _i32, _p, _sz, ctypes.POINTER(IrStats), _p]),
+    "rbfx_profile_begin": (ctypes.c_int, []),
+    "rbfx_profile_end": (ctypes.c_int, [ctypes.POINTER(ProfEntry), ctypes.c_int, ctypes.POINTER(ctypes.c_int)]),
     "rbfx_gemm_f64": (ctypes.c_int, [_i64, _i64, _i64, _d, _p, _i64, _p, _i64, _d, _p, _i64, _p]),
     "rbfx_gemm_f32": (ctypes.c_int, [_i64, _i64, _i64, _f, _p, _i64, _p, _i64, _f, _p, _i64, _p]),
 }
