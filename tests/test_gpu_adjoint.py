@@ -1,0 +1,159 @@
+"""GPU tests of the adjoint: gradients w.r.t. Dirichlet values, source
+coefficients and the initial state, against the closed form B^-T dL/du from the
+oracle, central finite differences (h = 1e-6), and checkpointed vs. plain BPTT."""
+import numpy as np
+import pytest
+import torch
+
+from oracle import assembly as OA, cloud as OC, rbf as R, solver as OS
+from tests import _cases as C
+
+pytestmark = pytest.mark.gpu
+
+
+def lap(x, center=None, rbf=None, monomial=None, fields=None):
+    import updes_amd as u
+    return u.nodal_laplacian(x, center, rbf, monomial)
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32_ir"])
+def test_laplace_control_gradient(precision):
+    """Laplace boundary control (small version of the 60x60 workload)."""
+    import updes_amd as u
+    nx = 20
+    ft = {"North": "d", "South": "d", "East": "d", "West": "d"}
+    cloud = u.SquareCloud(Nx=nx, Ny=nx, facet_types=ft)
+    rng = np.random.default_rng(0)
+    nN = len(cloud.facet_nodes["North"])
+    c0 = torch.as_tensor(rng.normal(0, 0.1, nN), device="cuda")
+    xi = cloud.sorted_nodes[:cloud.Ni].cuda()
+    target = torch.sin(torch.pi * xi[:, 0]) * xi[:, 1]
+
+    def loss_of(c):
+        bcs = {"North": c, "South": 0.0, "East": 0.0, "West": 0.0}
+        sol = u.pde_solver(lap, lambda x, ce, r, f: 0.0, cloud, bcs, u.polyharmonic, 1, precision=precision)
+        return ((sol.vals[:cloud.Ni] - target) ** 2).mean()
+
+    c = c0.clone().requires_grad_(True)
+    L0 = loss_of(c)
+    L0.backward()
+    g = c.grad.detach().cpu().numpy()
+    # closed form from the oracle: dL/dq = B^-T dL/du
+    oc = OC.SquareCloud(nx, nx, ft)
+    q = OA.assemble_q(oc, 0.0, {"North": c0.cpu().numpy(), "South": np.zeros(nx), "East": np.zeros(nx - 2),
+                                "West": np.zeros(nx - 2)})
+    u_ref, _, B = OS.pde_solver_bpath(C.LAPLACIAN, oc, R.POLYHARMONIC, 1.0, 3, q)
+    tg = target.cpu().numpy()
+    gu = np.zeros(oc.N)
+    gu[:oc.Ni] = 2 * (u_ref[:oc.Ni] - tg) / oc.Ni
+    g_ref = np.linalg.solve(B.T, gu)[oc.facet_nodes["North"]]
+    assert np.linalg.norm(g - g_ref) <= 1e-7 * np.linalg.norm(g_ref)
+    # central finite differences on a few controls (factorisation reused)
+    h = 1e-6
+    for k in [0, nN // 2, nN - 1]:
+        e = torch.zeros_like(c0)
+        e[k] = h
+        with torch.no_grad():
+            fd = (loss_of(c0 + e) - loss_of(c0 - e)).item() / (2 * h)
+        assert abs(fd - g[k]) <= 1e-6 * max(1.0, np.abs(g).max()), (k, fd, g[k])
+
+
+def test_source_coefficient_gradient_matches_fd():
+    """Gradient of an interior L2 loss w.r.t. 64 Fourier source coefficients."""
+    import updes_amd as u
+    import bench
+    cloud, bcs, b, basis = bench.fourier_problem(24, seed=3)
+    _, _, g = bench.solve_and_grad(cloud, bcs, b, basis, "fp64")
+    g = g.cpu().numpy()
+    h = 1e-6
+
+    def L(bv):
+        vals, loss, _ = bench.solve_and_grad(cloud, bcs, bv, basis, "fp64", want_grad=False)
+        return loss.item()
+
+    for k in [0, 9, 27, 63]:
+        e = torch.zeros_like(b)
+        e[k] = h
+        fd = (L(b + e) - L(b - e)) / (2 * h)
+        assert abs(fd - g[k]) <= 1e-6 * max(1e-3, np.abs(g).max()), (k, fd, g[k])
+    _, _, g32 = bench.solve_and_grad(cloud, bcs, b, basis, "fp32_ir")
+    assert np.linalg.norm(g32.cpu().numpy() - g) <= 1e-8 * np.linalg.norm(g)
+
+
+def _advdiff(nx, dt=1e-3, nu=0.01, vel=(1.0, 0.5), eps=5.0):
+    import functools
+    import updes_amd as u
+    ft = {"North": "d", "South": "d", "West": "p1", "East": "p1"}
+    cloud = u.SquareCloud(Nx=nx, Ny=nx, facet_types=ft)
+    rbf = functools.partial(u.gaussian, eps=eps)
+    V = torch.tensor(vel, dtype=torch.float64)
+
+    def diff(x, center=None, rbf=None, monomial=None, fields=None):
+        val = u.nodal_value(x, center, rbf, monomial)
+        grad = u.nodal_gradient(x, center, rbf, monomial)
+        lapl = u.nodal_laplacian(x, center, rbf, monomial)
+        return val + dt * torch.dot(V, grad) - dt * nu * lapl
+
+    def rhs(x, centers=None, rbf=None, fields=None):
+        return u.value(x, fields[:, 0], centers, rbf)
+
+    bcs = {"North": 0.0, "South": 0.0, "West": 0.0, "East": 0.0}
+
+    def step(un):
+        return u.pde_solver(diff, rhs, cloud, bcs, rbf, 1, rhs_args=[un]).vals
+
+    return cloud, step
+
+
+def test_time_step_matches_oracle_semantics():
+    """One implicit-Euler step with rhs value(x, coeffs(u^n)) = u^n + phi(0) lambda."""
+    import updes_amd as u
+    nx = 14
+    cloud, step = _advdiff(nx)
+    xy = cloud.sorted_nodes.cuda()
+    u0 = torch.exp(-((xy[:, 0] - 0.3) ** 2 + (xy[:, 1] - 0.5) ** 2) / (2 * 0.1 ** 2))
+    u1 = step(u0).detach().cpu().numpy()
+    ft = {"North": "d", "South": "d", "West": "p1", "East": "p1"}
+    oc = OC.SquareCloud(nx, nx, ft)
+    un = u0.cpu().numpy()
+    cn = OS.field_coefficients(un, oc, R.GAUSSIAN, 5.0, 3)
+    qi = OS.value(oc.sorted_nodes[:oc.Ni], cn, oc.sorted_nodes, R.GAUSSIAN, 5.0)
+    bc = {f: np.zeros(len(oc.facet_nodes[f])) for f in ft}
+    q = OA.assemble_q(oc, qi, bc)
+    u_ref, _, _ = OS.pde_solver_bpath(C.adv_diff_desc(), oc, R.GAUSSIAN, 5.0, 3, q)
+    assert np.linalg.norm(u1 - u_ref) <= 1e-8 * np.linalg.norm(u_ref)
+
+
+def test_checkpointed_bptt_matches_full_graph():
+    import updes_amd as u
+    cloud, step = _advdiff(12)
+    xy = cloud.sorted_nodes.cuda()
+    u0 = torch.exp(-((xy[:, 0] - 0.3) ** 2 + (xy[:, 1] - 0.5) ** 2) / (2 * 0.1 ** 2))
+    target = torch.exp(-((xy[:, 0] - 0.5) ** 2 + (xy[:, 1] - 0.4) ** 2) / (2 * 0.1 ** 2)) * 0.5
+    nsteps = 7
+
+    a = u0.clone().requires_grad_(True)
+    v = a
+    for _ in range(nsteps):
+        v = step(v)
+    La = ((v - target) ** 2).sum()
+    La.backward()
+
+    b = u0.clone().requires_grad_(True)
+    w = u.checkpointed_rollout(step, b, nsteps, every=3)
+    Lb = ((w - target) ** 2).sum()
+    Lb.backward()
+    assert abs(La.item() - Lb.item()) <= 1e-12 * abs(La.item())
+    assert torch.linalg.norm(a.grad - b.grad) <= 1e-10 * torch.linalg.norm(a.grad)
+    # finite differences on one component
+    h = 1e-6
+    k = int(cloud.Ni // 2)
+    e = torch.zeros_like(u0)
+    e[k] = h
+    with torch.no_grad():
+        def L(x):
+            for _ in range(nsteps):
+                x = step(x)
+            return ((x - target) ** 2).sum().item()
+        fd = (L(u0 + e) - L(u0 - e)) / (2 * h)
+    assert abs(fd - b.grad[k].item()) <= 1e-6 * max(1.0, b.grad.abs().max().item())

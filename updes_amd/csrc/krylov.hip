@@ -19,6 +19,8 @@
 namespace rbfx {
 
 constexpr int kGvThreads = 256;
+// Inner iterations observed before the convergence rate is extrapolated.
+constexpr int kRateWindow = 24;
 
 // y[i] = alpha * sum_j A[i][j] x[j] + beta * y[i]: one block per row.
 __global__ __launch_bounds__(kGvThreads) void gemv_n_block_kernel(int64_t m, int64_t n, double alpha,
@@ -288,7 +290,7 @@ extern "C" int rbfx_gmres_ir(int trans, int64_t n, const rbfx_problem* p, const 
   RBFX_HIP(hipMemcpyAsync(d_x, d_b, n * sizeof(double), hipMemcpyDeviceToDevice, s));
   if ((st = ops.applyMinv(d_x))) return st;
 
-  std::vector<double> H((size_t)(m + 1) * m), cs(m), sn(m), gv(m + 1), hh(2 * m + 4), y(m);
+  std::vector<double> H((size_t)(m + 1) * m), cs(m), sn(m), gv(m + 1), hh(2 * m + 8), y(m);
   double prev = INFINITY;
   int stalls = 0;
   int result = RBFX_ERR_STALL;
@@ -322,6 +324,7 @@ extern "C" int rbfx_gmres_ir(int trans, int64_t n, const rbfx_problem* p, const 
     gv[0] = rn;
     const double inner_tol = std::max(0.1 * tol * bn, 1e-10 * rn);
     int kdone = 0;
+    bool early_stall = false;
     for (int k = 0; k < m; ++k) {
       double* vk = V + (size_t)k * n;
       RBFX_HIP(hipMemcpyAsync(z, vk, n * sizeof(double), hipMemcpyDeviceToDevice, s));
@@ -361,6 +364,17 @@ extern "C" int rbfx_gmres_ir(int trans, int64_t n, const rbfx_problem* p, const 
       gv[k + 1] = -sn[k] * gv[k];
       gv[k] = cs[k] * gv[k];
       if (fabs(gv[k + 1]) <= inner_tol || hn == 0.0 || loc.inner_iters >= max_inner) break;
+      // Early stall: extrapolate the observed geometric rate of the residual
+      // estimate; give up when it cannot reach the tolerance in the budget.
+      if (k + 1 >= kRateWindow) {
+        const double est = fabs(gv[k + 1]);
+        const double rate = pow(est / rn, 1.0 / (k + 1));
+        const double need = (rate < 1.0 && rate > 0.0) ? log(inner_tol / est) / log(rate) : 1e30;
+        if (loc.inner_iters + need > (double)max_inner) {
+          early_stall = true;
+          break;
+        }
+      }
     }
     // y = R^-1 g
     for (int i = kdone - 1; i >= 0; --i) {
@@ -375,6 +389,18 @@ extern "C" int rbfx_gmres_ir(int trans, int64_t n, const rbfx_problem* p, const 
     hipLaunchKernelGGL(axpby_kernel, dim3(g), dim3(256), 0, s, n, 1.0, u, 1.0, d_x);
     RBFX_CHECK_LAUNCH();
     RBFX_HIP(hipStreamSynchronize(s));  // y (host) must outlive the copy
+    if (early_stall) {
+      // report the residual of the improved iterate, then stop
+      if ((st = ops.applyK(d_x, w))) return st;
+      hipLaunchKernelGGL(scale_copy_kernel, dim3(g), dim3(256), 0, s, n, 1.0, d_b, r);
+      hipLaunchKernelGGL(axpby_kernel, dim3(g), dim3(256), 0, s, n, -1.0, w, 1.0, r);
+      RBFX_CHECK_LAUNCH();
+      double e2 = 0.0;
+      if ((st = host_dot(r, r, e2))) return st;
+      loc.final_relres = sqrt(e2) / bn;
+      if (loc.final_relres <= tol) result = RBFX_OK;
+      break;
+    }
   }
   loc.stalled = result == RBFX_OK ? 0 : 1;
   if (stats) *stats = loc;

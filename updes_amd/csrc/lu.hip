@@ -30,6 +30,10 @@ typedef float v4f __attribute__((ext_vector_type(4)));
 
 template <typename T>
 struct Mma;
+// MFMA 16x16x4 operand layout (both types): lane l holds A[l%16][l/16] and
+// B[l/16][l%16].  Accumulator element v of lane l is D[row(l,v)][l%16] with
+//   f32: row = 4*(l/16) + v        f64: row = 4*v + l/16
+// (measured on gfx950 with one-hot operands).
 template <>
 struct Mma<double> {
   typedef v4d acc_t;
@@ -37,6 +41,7 @@ struct Mma<double> {
   __device__ static inline acc_t mma(double a, double b, acc_t c) {
     return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
   }
+  __device__ static inline int row(int lane, int v) { return 4 * v + (lane >> 4); }
 };
 template <>
 struct Mma<float> {
@@ -45,13 +50,12 @@ struct Mma<float> {
   __device__ static inline acc_t mma(float a, float b, acc_t c) {
     return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
   }
+  __device__ static inline int row(int lane, int v) { return 4 * (lane >> 4) + v; }
 };
 
 // ---------------------------------------------------------------------------
 // GEMM: C = beta*C + alpha*A*B, row-major, beta in {0,1}.
 // Block tile 128x128, 4 waves (2x2), wave tile 64x64 = 4x4 MFMA 16x16 tiles.
-// MFMA 16x16x4 operand layout: lane l holds A[l%16][l/16] and B[l/16][l%16];
-// the accumulator holds D[4*(l/16)+v][l%16], v = 0..3.
 constexpr int GBM = 128, GBN = 128, GTHREADS = 256;
 
 template <typename T>
@@ -152,7 +156,7 @@ __global__ __launch_bounds__(GTHREADS) void gemm_kernel(int64_t m, int64_t n, in
       const int64_t c = col0 + wn * 64 + j * 16 + (lane & 15);
 #pragma unroll
       for (int v = 0; v < 4; ++v) {
-        const int64_t r = row0 + wm * 64 + i * 16 + 4 * (lane >> 4) + v;
+        const int64_t r = row0 + wm * 64 + i * 16 + Mma<T>::row(lane, v);
         if (r < m && c < n) {
           T* pc = C + r * ldc + c;
           const T prod = alpha * acc[i][j][v];
