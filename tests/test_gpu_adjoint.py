@@ -65,7 +65,10 @@ def test_source_coefficient_gradient_matches_fd():
     cloud, bcs, b, basis = bench.fourier_problem(24, seed=3)
     _, _, g = bench.solve_and_grad(cloud, bcs, b, basis, "fp64")
     g = g.cpu().numpy()
-    h = 1e-6
+    # the loss is quadratic in b (linear PDE), so central differences have no
+    # truncation error; h = 1e-5 keeps the solve's rounding noise (~1e-12 in L)
+    # well below the 1e-6 comparison
+    h = 1e-5
 
     def L(bv):
         vals, loss, _ = bench.solve_and_grad(cloud, bcs, bv, basis, "fp64", want_grad=False)
@@ -120,8 +123,14 @@ def test_time_step_matches_oracle_semantics():
     qi = OS.value(oc.sorted_nodes[:oc.Ni], cn, oc.sorted_nodes, R.GAUSSIAN, 5.0)
     bc = {f: np.zeros(len(oc.facet_nodes[f])) for f in ft}
     q = OA.assemble_q(oc, qi, bc)
-    u_ref, _, _ = OS.pde_solver_bpath(C.adv_diff_desc(), oc, R.GAUSSIAN, 5.0, 3, q)
-    assert np.linalg.norm(u1 - u_ref) <= 1e-8 * np.linalg.norm(u_ref)
+    u_ref, _, B = OS.pde_solver_bpath(C.adv_diff_desc(), oc, R.GAUSSIAN, 5.0, 3, q)
+    # the reference algorithm itself is only accurate to ~kappa(B) eps here
+    # (kappa(B) ~ 1e10 for the periodic Gaussian system; its FP64 augmented-system
+    # solve differs from it by 2e-7), so the bound follows the conditioning
+    tol = max(1e-8, 10 * np.linalg.cond(B) * np.finfo(float).eps)
+    assert np.linalg.norm(u1 - u_ref) <= tol * np.linalg.norm(u_ref)
+    u_k, _, _ = OS.pde_solver_kpath(C.adv_diff_desc(), oc, R.GAUSSIAN, 5.0, 3, q)
+    assert np.linalg.norm(u1 - u_k) <= tol * np.linalg.norm(u_k)
 
 
 def test_checkpointed_bptt_matches_full_graph():

@@ -432,10 +432,13 @@ static int64_t mt_chunks(int64_t n) {
   return want;
 }
 
+namespace rbfx {
+size_t matvec_t_ws_bytes(int64_t n) { return ws_round(sizeof(double) * (size_t)mt_chunks(n) * (size_t)n); }
+}  // namespace rbfx
+
 extern "C" size_t rbfx_matvec_workspace(const rbfx_problem* p, int trans) {
   if (!p || !trans) return 0;
-  const int64_t n = p->N + p->M;
-  return ws_round(sizeof(double) * (size_t)mt_chunks(n) * (size_t)n);
+  return matvec_t_ws_bytes(p->N + p->M);
 }
 
 extern "C" int rbfx_matvec(const rbfx_problem* p, int matrix, int trans, const double* d_x,

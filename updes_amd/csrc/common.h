@@ -82,4 +82,6 @@ namespace rbfx {
 // returns -1 and prof_stop() does nothing.
 int prof_start(const char* name, hipStream_t s);
 void prof_stop(int id, hipStream_t s, double flops, double bytes);
+// Workspace bytes of the transposed matrix-free product over n rows.
+size_t matvec_t_ws_bytes(int64_t n);
 }  // namespace rbfx

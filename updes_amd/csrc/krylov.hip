@@ -204,7 +204,7 @@ extern "C" size_t rbfx_gmres_ir_workspace(int64_t n, int32_t restart) {
   b += ws_round(sizeof(double) * (size_t)(2 * restart + 8));          // h, h2, norms
   b += ws_round(sizeof(double) * kDotBlocks);                         // dot partials
   b += ws_round(rbfx_getrs_workspace(n));
-  const size_t gw = std::max(gemv_workspace(1, restart + 1, n), gemv_workspace(1, n, n));
+  const size_t gw = std::max({gemv_workspace(1, restart + 1, n), gemv_workspace(1, n, n), matvec_t_ws_bytes(n)});
   b += ws_round(gw);
   return b + 4096;
 }
@@ -258,7 +258,7 @@ extern "C" int rbfx_gmres_ir(int trans, int64_t n, const rbfx_problem* p, const 
   double* dpart = ws.take<double>(kDotBlocks);
   const size_t tsb = rbfx_getrs_workspace(n);
   void* tsw = ws.take<char>(tsb);
-  const size_t gw = std::max(gemv_workspace(1, m + 1, n), gemv_workspace(1, n, n));
+  const size_t gw = std::max({gemv_workspace(1, m + 1, n), gemv_workspace(1, n, n), matvec_t_ws_bytes(n)});
   double* part = ws.take<double>(gw / sizeof(double) + 1);
   RBFX_REQUIRE(part != nullptr, "gmres_ir workspace carve failed");
   IrOps ops{trans, n, p, d_K64, ld64, d_LU32, ld32, d_perm32, d_dinv32, tsw, tsb, part, gw, s};

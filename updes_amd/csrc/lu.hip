@@ -58,8 +58,11 @@ struct Mma<float> {
 // Block tile 128x128, 4 waves (2x2), wave tile 64x64 = 4x4 MFMA 16x16 tiles.
 constexpr int GBM = 128, GBN = 128, GTHREADS = 256;
 
+// Two waves per SIMD (<= 256 registers each) so one wave's barrier or load
+// wait overlaps the other's MFMAs.
 template <typename T>
-__global__ __launch_bounds__(GTHREADS) void gemm_kernel(int64_t m, int64_t n, int64_t k, T alpha,
+__global__ __launch_bounds__(GTHREADS) __attribute__((amdgpu_waves_per_eu(2, 2))) void gemm_kernel(
+                                                        int64_t m, int64_t n, int64_t k, T alpha,
                                                         const T* __restrict__ A, int64_t lda,
                                                         const T* __restrict__ B, int64_t ldb,
                                                         T beta, T* __restrict__ C, int64_t ldc,
@@ -67,8 +70,9 @@ __global__ __launch_bounds__(GTHREADS) void gemm_kernel(int64_t m, int64_t n, in
   constexpr int BK = Mma<T>::BK;
   constexpr int AV = GBM * BK / GTHREADS;  // A elements per thread per k-tile
   constexpr int BV = BK * GBN / GTHREADS;  // B elements per thread per k-tile
-  __shared__ T As[BK][GBM];
-  __shared__ T Bs[BK][GBN];
+  // double-buffered k-tiles: the next tile is written while this one is read
+  __shared__ T As[2][BK][GBM];
+  __shared__ T Bs[2][BK][GBN];
 
   // XCD-aware order: consecutive workgroups land on different XCDs (round
   // robin), so give each XCD a contiguous band of tile rows.
@@ -84,13 +88,21 @@ __global__ __launch_bounds__(GTHREADS) void gemm_kernel(int64_t m, int64_t n, in
   const int lane = tid & 63, wid = tid >> 6;
   const int wm = wid >> 1, wn = wid & 1;
 
+  // The accumulators start from beta*C (beta is 0 or 1) and alpha is folded
+  // into the A tile, so the epilogue only stores (no C reads held live).
   typename Mma<T>::acc_t acc[4][4];
+  const bool add = beta != T(0);
 #pragma unroll
   for (int i = 0; i < 4; ++i)
 #pragma unroll
-    for (int j = 0; j < 4; ++j)
+    for (int j = 0; j < 4; ++j) {
+      const int64_t c = col0 + wn * 64 + j * 16 + (lane & 15);
 #pragma unroll
-      for (int v = 0; v < 4; ++v) acc[i][j][v] = 0;
+      for (int v = 0; v < 4; ++v) {
+        const int64_t r = row0 + wm * 64 + i * 16 + Mma<T>::row(lane, v);
+        acc[i][j][v] = (add && r < m && c < n) ? C[r * ldc + c] : T(0);
+      }
+    }
 
   // global -> register staging maps
   // A tile (GBM x BK): thread -> row ar = tid % GBM, k chunk ak = (tid / GBM) * AV
@@ -121,34 +133,40 @@ __global__ __launch_bounds__(GTHREADS) void gemm_kernel(int64_t m, int64_t n, in
       for (int q = 0; q < BV; ++q) rb[q] = (gkb < k && gc + q < n) ? pb[q] : T(0);
     }
   };
-  auto store = [&]() {
+  auto store = [&](int buf) {
 #pragma unroll
-    for (int q = 0; q < AV; ++q) As[ak + q][ar] = ra[q];
+    for (int q = 0; q < AV; ++q) As[buf][ak + q][ar] = alpha * ra[q];
 #pragma unroll
-    for (int q = 0; q < BV; ++q) Bs[bk][bc + q] = rb[q];
+    for (int q = 0; q < BV; ++q) Bs[buf][bk][bc + q] = rb[q];
   };
 
   load(0);
+  store(0);
+  __syncthreads();
+  int buf = 0;
   for (int64_t kt = 0; kt < k; kt += BK) {
-    __syncthreads();
-    store();
-    __syncthreads();
-    if (kt + BK < k) load(kt + BK);
+    const bool more = kt + BK < k;
+    if (more) load(kt + BK);
 #pragma unroll
     for (int kk = 0; kk < BK; kk += 4) {
       T af[4], bf[4];
 #pragma unroll
-      for (int i = 0; i < 4; ++i) af[i] = As[kk + (lane >> 4)][wm * 64 + i * 16 + (lane & 15)];
+      for (int i = 0; i < 4; ++i) af[i] = As[buf][kk + (lane >> 4)][wm * 64 + i * 16 + (lane & 15)];
 #pragma unroll
-      for (int j = 0; j < 4; ++j) bf[j] = Bs[kk + (lane >> 4)][wn * 64 + j * 16 + (lane & 15)];
+      for (int j = 0; j < 4; ++j) bf[j] = Bs[buf][kk + (lane >> 4)][wn * 64 + j * 16 + (lane & 15)];
 #pragma unroll
       for (int i = 0; i < 4; ++i)
 #pragma unroll
         for (int j = 0; j < 4; ++j) acc[i][j] = Mma<T>::mma(af[i], bf[j], acc[i][j]);
     }
+    // the other buffer was last read in the previous iteration, which every
+    // wave finished before the barrier that ended it
+    if (more) store(buf ^ 1);
+    __syncthreads();
+    buf ^= 1;
   }
 
-  // epilogue
+  // epilogue: plain stores
 #pragma unroll
   for (int i = 0; i < 4; ++i) {
 #pragma unroll
@@ -157,11 +175,7 @@ __global__ __launch_bounds__(GTHREADS) void gemm_kernel(int64_t m, int64_t n, in
 #pragma unroll
       for (int v = 0; v < 4; ++v) {
         const int64_t r = row0 + wm * 64 + i * 16 + Mma<T>::row(lane, v);
-        if (r < m && c < n) {
-          T* pc = C + r * ldc + c;
-          const T prod = alpha * acc[i][j][v];
-          *pc = (beta != T(0)) ? (*pc + prod) : prod;
-        }
+        if (r < m && c < n) C[r * ldc + c] = acc[i][j][v];
       }
     }
   }
