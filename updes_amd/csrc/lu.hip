@@ -197,36 +197,40 @@ static int gemm(int64_t m, int64_t n, int64_t k, T alpha, const T* A, int64_t ld
 }
 
 // ---------------------------------------------------------------------------
-// Grid barrier for the cooperative panel kernel (all workgroups co-resident).
-// The counter only grows; `target` is the cumulative arrival count.
-__device__ inline bool grid_barrier(unsigned* bar, unsigned target, int* status) {
+// Grid-wide step barrier for the cooperative panel kernel (all workgroups
+// co-resident).  Workgroup g announces step `target` in its own flag and then
+// waits until every workgroup's flag has reached it: one release store per
+// workgroup and parallel polling, instead of all arrivals serialising on a
+// single counter.  Flags only grow (targets are cumulative per factorisation).
+__device__ inline bool grid_step(unsigned* flags, unsigned target, int* status) {
   __threadfence();
   __syncthreads();
-  __shared__ int ok;
+  __shared__ int bad;
   if (threadIdx.x == 0) {
-    __hip_atomic_fetch_add(bar, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    bad = 0;
+    __hip_atomic_store(flags + blockIdx.x, target, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+  }
+  __syncthreads();
+  for (unsigned q = threadIdx.x; q < gridDim.x; q += blockDim.x) {
     uint32_t spins = 0;
-    int good = 1;
-    while (__hip_atomic_load(bar, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < target) {
+    while (__hip_atomic_load(flags + q, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < target) {
       __builtin_amdgcn_s_sleep(1);
       ++spins;
       if ((spins & 255u) == 0 &&
           __hip_atomic_load(status, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) {
-        good = 0;
+        bad = 1;
         break;
       }
       if (spins > kSpinLimit) {
         __hip_atomic_store(status, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        good = 0;
+        bad = 1;
         break;
       }
     }
-    if (good && __hip_atomic_load(status, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) good = 0;
-    ok = good;
   }
   __syncthreads();
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-  return ok != 0;
+  return bad == 0;
 }
 
 template <typename T>
@@ -244,8 +248,8 @@ struct PanelArgs {
   T* cand_row;    // [2][G][w]
   int* pos2phys;  // [m]
   int* phys2pos;  // [m]
-  unsigned* bar;
-  unsigned bar_base;
+  unsigned* flags;    // [G] step flags (grid_step)
+  unsigned seq_base;  // step count before this launch
   int* status;
   int32_t* info;
 };
@@ -340,7 +344,7 @@ __global__ __launch_bounds__(kPanelThreads) void panel_kernel(PanelArgs<T> a) {
     }
     if (bv >= T(0) && tid < w) a.cand_row[((int64_t)par * G + g) * w + tid] = P[(int64_t)tid * a.RP + (bi - rb)];
     ++nbar;
-    if (!grid_barrier(a.bar, a.bar_base + nbar * (unsigned)G, a.status)) return;
+    if (!grid_step(a.flags, a.seq_base + nbar, a.status)) return;
 
     // global winner
     T v = T(-1);
@@ -396,7 +400,7 @@ __global__ __launch_bounds__(kPanelThreads) void panel_kernel(PanelArgs<T> a) {
     __syncthreads();
   }
   ++nbar;
-  if (!grid_barrier(a.bar, a.bar_base + nbar * (unsigned)G, a.status)) return;
+  if (!grid_step(a.flags, a.seq_base + nbar, a.status)) return;
   for (int64_t idx = tid; idx < (int64_t)nr * w; idx += kPanelThreads) {
     const int64_t r = idx / w, c = idx % w;
     const int64_t dest = a.phys2pos[rb + r];
@@ -527,7 +531,8 @@ __global__ __launch_bounds__(kTrsmThreads) void trsm_llnu_kernel(const T* __rest
 }
 
 __global__ void zero_u32_kernel(unsigned* p, int n) {
-  if ((int)threadIdx.x < n) p[threadIdx.x] = 0u;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0u;
 }
 
 // A failed or timed-out cooperative panel leaves status != 0: report it as
@@ -551,9 +556,9 @@ struct LuCtx {
   T* cand_row;
   int* pos2phys;
   int* phys2pos;
-  unsigned* bar;
+  unsigned* flags;  // [kMaxPanelGroups] step flags of the panel launches
   int* status;
-  unsigned bar_count;  // cumulative arrivals
+  unsigned seq;     // steps taken by the panel launches so far
   int64_t* plan_dst;
   int64_t* plan_src;
   int* plan_count;
@@ -604,8 +609,8 @@ static int leaf(LuCtx<T>& c, int64_t r0, int w) {
   a.cand_row = c.cand_row;
   a.pos2phys = c.pos2phys;
   a.phys2pos = c.phys2pos;
-  a.bar = c.bar;
-  a.bar_base = c.bar_count;
+  a.flags = c.flags;
+  a.seq_base = c.seq;
   a.status = c.status;
   a.info = c.info;
   const size_t smem = need(R);
@@ -614,7 +619,7 @@ static int leaf(LuCtx<T>& c, int64_t r0, int w) {
   RBFX_HIP(hipLaunchCooperativeKernel(reinterpret_cast<const void*>(&panel_kernel<T>), dim3(G),
                                       dim3(kPanelThreads), args, (unsigned)smem, c.s));
   prof_stop(pid, c.s, (double)m * w * w, 2.0 * sizeof(T) * m * w);
-  c.bar_count += (unsigned)G * (unsigned)(w + 1);
+  c.seq += (unsigned)(w + 1);
   return RBFX_OK;
 }
 
@@ -687,7 +692,8 @@ static size_t getrf_ws_bytes(int64_t n, int bytes) {
   b += ws_round((size_t)2 * G * 8);               // cand_idx
   b += ws_round((size_t)2 * G * kPanelWMax * bytes);  // cand_row
   b += ws_round((size_t)n * 4) * 2;               // pos2phys, phys2pos
-  b += ws_round(64);                              // barrier + status
+  b += ws_round(64);                              // status
+  b += ws_round((size_t)G * 4);                   // panel step flags
   b += ws_round((size_t)2 * kSwapMax * 8) * 2;    // plan dst / src
   b += ws_round(64);                              // plan count
   // swap staging: 2*NB rows x min(n, 8192) columns
@@ -718,21 +724,23 @@ static int getrf(int64_t n, T* A, int64_t lda, int64_t* ipiv, void* work, size_t
   c.pos2phys = ws.take<int>(n);
   c.phys2pos = ws.take<int>(n);
   unsigned* ctl = ws.take<unsigned>(16);
-  c.bar = ctl;
   c.status = reinterpret_cast<int*>(ctl + 1);
+  c.flags = ws.take<unsigned>(G);
   c.plan_dst = ws.take<int64_t>(2 * kSwapMax);
   c.plan_src = ws.take<int64_t>(2 * kSwapMax);
   c.plan_count = ws.take<int>(16);
   c.swap_tmp_elems = (int64_t)2 * kNB * std::min<int64_t>(n, 8192);
   c.swap_tmp = ws.take<T>(c.swap_tmp_elems);
   RBFX_REQUIRE(c.swap_tmp != nullptr, "workspace carve failed");
-  c.bar_count = 0;
+  c.seq = 0;
   c.G_max = G;
   c.lds_budget = std::min<size_t>(di.lds_per_block, 65536);
   c.wmax = panel_width_max(n, sizeof(T), di, c.lds_budget);
   RBFX_REQUIRE(c.wmax > 0, "matrix too large for the LDS-resident panel (n=%lld)", (long long)n);
   c.wmax = std::min(c.wmax, sizeof(T) == 8 ? 32 : 64);
   hipLaunchKernelGGL(zero_u32_kernel, dim3(1), dim3(64), 0, s, ctl, 16);
+  RBFX_CHECK_LAUNCH();
+  hipLaunchKernelGGL(zero_u32_kernel, dim3(G / 64), dim3(64), 0, s, c.flags, G);
   RBFX_CHECK_LAUNCH();
   RBFX_HIP(hipMemsetAsync(info, 0, sizeof(int32_t), s));
   int st;

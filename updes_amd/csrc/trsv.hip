@@ -91,8 +91,8 @@ __global__ __launch_bounds__(kTsThreads) void sweep_kernel(const TF* __restrict_
                                                            double* __restrict__ y, unsigned* __restrict__ ctl) {
   typedef double T;
   __shared__ T acc[TB];
-  __shared__ T yj[TB];
   __shared__ T part[4][TB];
+  __shared__ T Ds[TB][TB + 1];
   __shared__ int s_I;
   __shared__ unsigned s_avail;
   const int tid = threadIdx.x;
@@ -111,10 +111,36 @@ __global__ __launch_bounds__(kTsThreads) void sweep_kernel(const TF* __restrict_
   const int64_t I = FWD ? seq : nb - 1 - seq;
   const int64_t oI = I * TB;
   const int szI = (int)imin64(TB, n - oI);
-  if (tid < TB) acc[tid] = tid < szI ? y[oI + tid] : T(0);
 
-  // dependencies in completion order: seq' = 0 .. seq-1
-  for (int d = 0; d < seq; ++d) {
+  // Thread ownership of a tile, kept for every off-diagonal tile:
+  //   no-transpose: row rr = tid/4 of block I, columns q*16 .. q*16+15 of block J
+  //   transpose:    column rr = tid%64 of block I, rows q*16 .. q*16+15 of block J
+  const int rr = TRANS ? (tid & 63) : (tid >> 2);
+  const int q = TRANS ? (tid >> 6) : (tid & 3);
+
+  // prefetch the diagonal inverse and the tile of the last dependency (the
+  // one on the critical path) before waiting
+  const double* D = dinv + (I * 2 + WHICH) * TB * TB;
+  for (int e = tid; e < TB * TB; e += kTsThreads) Ds[e / TB][e % TB] = D[e];
+  T pre[16];
+  if (seq > 0) {
+    const int64_t Jl = FWD ? I - 1 : I + 1;
+    const int64_t oJ = Jl * TB;
+    const int szJ = (int)imin64(TB, n - oJ);
+#pragma unroll
+    for (int k = 0; k < 16; ++k) {
+      const int kk = q * 16 + k;
+      pre[k] = (rr < szI && kk < szJ)
+                   ? (TRANS ? (T)F[(oJ + kk) * lda + oI + rr] : (T)F[(oI + rr) * lda + oJ + kk])
+                   : T(0);
+    }
+  }
+
+  // fold in the dependencies in completion order d = 0 .. seq-1 as they
+  // become available (one private partial sum per thread, no barrier per tile)
+  T s = 0;
+  int d = 0;
+  while (d < seq) {
     if ((unsigned)d >= s_avail) {
       __syncthreads();
       if (tid == 0) {
@@ -139,53 +165,56 @@ __global__ __launch_bounds__(kTsThreads) void sweep_kernel(const TF* __restrict_
       __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
       if (s_avail == 0xffffffffu) return;
     }
-    const int64_t J = FWD ? d : nb - 1 - d;
-    const int64_t oJ = J * TB;
-    const int szJ = (int)imin64(TB, n - oJ);
-    if (tid < TB) yj[tid] = tid < szJ ? y[oJ + tid] : T(0);
-    __syncthreads();
-    if (!TRANS) {
-      // acc[r] -= sum_c F[oI+r][oJ+c] yj[c]; thread: r = tid/4, c in quarter
-      const int r = tid >> 2, q = tid & 3;
-      T s = 0;
-      if (r < szI) {
-        const TF* row = F + (oI + r) * lda + oJ + q * 16;
+    const int lim = (unsigned)seq < s_avail ? seq : (int)s_avail;
+    for (; d < lim; ++d) {
+      const int64_t J = FWD ? d : nb - 1 - d;
+      const int64_t oJ = J * TB;
+      const int szJ = (int)imin64(TB, n - oJ);
+      if (rr >= szI) continue;
+      if (d == seq - 1) {
 #pragma unroll
-        for (int c = 0; c < 16; ++c)
-          if (q * 16 + c < szJ) s += (T)row[c] * yj[q * 16 + c];
+        for (int k = 0; k < 16; ++k) {
+          const int kk = q * 16 + k;
+          if (kk < szJ) s += pre[k] * y[oJ + kk];
+        }
+      } else if (!TRANS) {
+        const TF* row = F + (oI + rr) * lda + oJ + q * 16;
+#pragma unroll
+        for (int k = 0; k < 16; ++k)
+          if (q * 16 + k < szJ) s += (T)row[k] * y[oJ + q * 16 + k];
+      } else {
+#pragma unroll
+        for (int k = 0; k < 16; ++k) {
+          const int kk = q * 16 + k;
+          if (kk < szJ) s += (T)F[(oJ + kk) * lda + oI + rr] * y[oJ + kk];
+        }
       }
-      s += __shfl_xor(s, 1, 64);
-      s += __shfl_xor(s, 2, 64);
-      if (q == 0 && r < szI) acc[r] -= s;
-    } else {
-      // acc[c] -= sum_r F[oJ+r][oI+c] yj[r]; thread: c = tid%64, rows quarter tid/64
-      const int c = tid & 63, q = tid >> 6;
-      T s = 0;
-      if (c < szI) {
-        for (int r = q * 16; r < q * 16 + 16 && r < szJ; ++r) s += (T)F[(oJ + r) * lda + oI + c] * yj[r];
-      }
-      part[q][c] = s;
-      __syncthreads();
-      if (tid < TB) acc[tid] -= part[0][tid] + part[1][tid] + part[2][tid] + part[3][tid];
     }
-    __syncthreads();
   }
 
-  __syncthreads();
-  // diagonal block: y_I = D acc (D = inverse of the diagonal block, or its transpose)
-  const T* D = dinv + (I * 2 + WHICH) * TB * TB;
-  {
-    const int r = tid >> 2, q = tid & 3;
-    T s = 0;
-#pragma unroll
-    for (int c = 0; c < 16; ++c) {
-      const int cc = q * 16 + c;
-      const T dv = TRANS ? D[cc * TB + r] : D[r * TB + cc];
-      s += dv * acc[cc];
-    }
+  // acc = b_I - (sum of the folded tiles)
+  if (!TRANS) {
     s += __shfl_xor(s, 1, 64);
     s += __shfl_xor(s, 2, 64);
-    if (q == 0 && r < szI) y[oI + r] = s;
+    if (q == 0) acc[rr] = rr < szI ? y[oI + rr] - s : T(0);
+  } else {
+    part[q][rr] = s;
+    __syncthreads();
+    if (tid < TB) acc[tid] = tid < szI ? y[oI + tid] - (part[0][tid] + part[1][tid] + part[2][tid] + part[3][tid]) : T(0);
+  }
+  __syncthreads();
+  // diagonal block: y_I = D acc (D = inverse of the diagonal block, or its transpose)
+  {
+    const int r = tid >> 2, qq = tid & 3;
+    T t = 0;
+#pragma unroll
+    for (int c = 0; c < 16; ++c) {
+      const int cc = qq * 16 + c;
+      t += (TRANS ? Ds[cc][r] : Ds[r][cc]) * acc[cc];
+    }
+    t += __shfl_xor(t, 1, 64);
+    t += __shfl_xor(t, 2, 64);
+    if (qq == 0 && r < szI) y[oI + r] = t;
   }
   __threadfence();
   __syncthreads();
