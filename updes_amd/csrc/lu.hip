@@ -235,7 +235,7 @@ __device__ inline bool grid_step(unsigned* flags, unsigned target, int* status) 
 
 template <typename T>
 struct PanelArgs {
-  T* A;           // panel top-left (absolute row r0, column c0)
+  T* A;           // panel top-left (absolute row r0, column r0)
   int64_t lda;
   int64_t m;      // rows in the panel (r0 .. r0+m)
   int w;          // panel width <= kPanelWMax
@@ -243,11 +243,8 @@ struct PanelArgs {
   int64_t RP;     // padded LDS column stride
   int64_t r0;     // absolute row of the panel's first row
   int64_t* ipiv;  // (w) absolute pivot rows
-  T* cand_val;    // [2][G]
-  int64_t* cand_idx;  // [2][G]
+  double* cand;   // [2][G] (|value|, row) pairs
   T* cand_row;    // [2][G][w]
-  int* pos2phys;  // [m]
-  int* phys2pos;  // [m]
   unsigned* flags;    // [G] step flags (grid_step)
   unsigned seq_base;  // step count before this launch
   int* status;
@@ -257,110 +254,118 @@ struct PanelArgs {
 constexpr int kPanelThreads = 256;
 constexpr int kPanelWMax = 64;
 
-template <typename T>
-__device__ inline void block_argmax(T v, int64_t idx, T& bv, int64_t& bi) {
-  __shared__ T sv[kPanelThreads / 64];
-  __shared__ int64_t si[kPanelThreads / 64];
+// (value, index) maximum with the smaller index winning ties.
+__device__ inline void argmax_merge(double& v, int64_t& i, double ov, int64_t oi) {
+  if (ov > v || (ov == v && oi < i)) {
+    v = ov;
+    i = oi;
+  }
+}
+
+// Block-wide argmax; one barrier.  The shared slots are reused by the next
+// call only after at least one more barrier of the caller.
+__device__ inline void block_argmax(double v, int64_t idx, double* sv, int64_t* si, double& bv, int64_t& bi) {
   for (int off = 32; off > 0; off >>= 1) {
-    T ov = __shfl_down(v, off, 64);
-    int64_t oi = __shfl_down(idx, off, 64);
-    if (ov > v || (ov == v && oi < idx)) {
-      v = ov;
-      idx = oi;
-    }
+    const double ov = __shfl_down(v, off, 64);
+    const int64_t oi = __shfl_down(idx, off, 64);
+    argmax_merge(v, idx, ov, oi);
   }
-  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-  if (lane == 0) {
-    sv[wid] = v;
-    si[wid] = idx;
-  }
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    T b = sv[0];
-    int64_t i = si[0];
-    for (int q = 1; q < kPanelThreads / 64; ++q)
-      if (sv[q] > b || (sv[q] == b && si[q] < i)) {
-        b = sv[q];
-        i = si[q];
-      }
-    sv[0] = b;
-    si[0] = i;
+  if ((threadIdx.x & 63) == 0) {
+    sv[threadIdx.x >> 6] = v;
+    si[threadIdx.x >> 6] = idx;
   }
   __syncthreads();
   bv = sv[0];
   bi = si[0];
-  __syncthreads();
+  for (int q = 1; q < kPanelThreads / 64; ++q) argmax_merge(bv, bi, sv[q], si[q]);
+}
+
+// Position bookkeeping of the panel's row interchanges, replayed from the
+// winners (every workgroup knows them) by one wave.  Only moved rows are
+// listed: pos(p) = p unless p is in the list.
+struct MoveList {
+  int64_t phys[2 * kPanelWMax];
+  int64_t pos[2 * kPanelWMax];
+  int cnt;
+};
+
+// Orders lane 0's LDS writes before the wave's following LDS reads.
+__device__ inline void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+__device__ inline int ml_find(const int64_t* keys, int cnt, int64_t key) {
+  const int lane = threadIdx.x & 63;
+  for (int base = 0; base < cnt; base += 64) {
+    const int q = base + lane;
+    const unsigned long long bal = __ballot(q < cnt && keys[q] == key);
+    if (bal) return base + __ffsll((long long)bal) - 1;
+  }
+  return -1;
 }
 
 template <typename T>
 __global__ __launch_bounds__(kPanelThreads) void panel_kernel(PanelArgs<T> a) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
-  T* P = reinterpret_cast<T*>(smem_raw);          // [w][RP] column-major
-  T* urow = P + (int64_t)a.w * a.RP;               // [w]
+  T* P = reinterpret_cast<T*>(smem_raw);                     // [w][RP] column-major
+  T* urow = P + (int64_t)a.w * a.RP;                          // [kPanelWMax]
   int* retired = reinterpret_cast<int*>(urow + kPanelWMax);  // [RP]
-  __shared__ int64_t s_win;
-  __shared__ int s_wg;
+  __shared__ double s_v[kPanelThreads / 64];
+  __shared__ int64_t s_i[kPanelThreads / 64];
+  __shared__ int64_t s_win[kPanelWMax];
+  __shared__ int s_zero[kPanelWMax];
+  __shared__ MoveList ml;
 
   const int G = gridDim.x, g = blockIdx.x, tid = threadIdx.x;
   const int w = a.w;
   const int64_t rb = (int64_t)g * a.R;
   const int64_t re = rb + a.R < a.m ? rb + a.R : a.m;
   const int nr = re > rb ? (int)(re - rb) : 0;
-  unsigned nbar = 0;
 
   for (int64_t idx = tid; idx < (int64_t)nr * w; idx += kPanelThreads) {
     const int64_t r = idx / w, c = idx % w;
     P[c * a.RP + r] = a.A[(rb + r) * a.lda + c];
   }
   for (int r = tid; r < nr; r += kPanelThreads) retired[r] = 0;
-  if (g == 0) {
-    for (int64_t i = tid; i < a.m; i += kPanelThreads) {
-      a.pos2phys[i] = (int)i;
-      a.phys2pos[i] = (int)i;
-    }
-  }
   __syncthreads();
 
   for (int j = 0; j < w; ++j) {
     const int par = j & 1;
-    // local candidate for column j
-    T best = T(-1);
+    // 1. best active row of this workgroup for column j
+    double best = -1.0;
     int64_t bidx = INT64_MAX;
-    for (int r = tid; r < nr; r += kPanelThreads) {
-      if (!retired[r]) {
-        const T v = fabs(P[(int64_t)j * a.RP + r]);
-        if (v > best || (v == best && rb + r < bidx)) {
-          best = v;
-          bidx = rb + r;
-        }
-      }
-    }
-    T bv;
+    for (int r = tid; r < nr; r += kPanelThreads)
+      if (!retired[r]) argmax_merge(best, bidx, (double)fabs(P[(int64_t)j * a.RP + r]), rb + r);
+    double bv;
     int64_t bi;
-    block_argmax<T>(best, bidx, bv, bi);
+    block_argmax(best, bidx, s_v, s_i, bv, bi);
+    // 2. publish it: (|value|, row) and the row's w entries
+    double* rec = a.cand + 2 * ((int64_t)par * G + g);
     if (tid == 0) {
-      a.cand_val[par * G + g] = bv;
-      a.cand_idx[par * G + g] = bi;
+      rec[0] = bv;
+      rec[1] = (double)bi;
+      __threadfence();
     }
-    if (bv >= T(0) && tid < w) a.cand_row[((int64_t)par * G + g) * w + tid] = P[(int64_t)tid * a.RP + (bi - rb)];
-    ++nbar;
-    if (!grid_step(a.flags, a.seq_base + nbar, a.status)) return;
-
-    // global winner
-    T v = T(-1);
+    if (bv >= 0.0 && tid < w) {
+      a.cand_row[((int64_t)par * G + g) * w + tid] = P[(int64_t)tid * a.RP + (bi - rb)];
+      __threadfence();
+    }
+    // 3. wait until every workgroup has published column j
+    if (!grid_step(a.flags, a.seq_base + (unsigned)(j + 1), a.status)) return;
+    // 4. the winner across workgroups
+    double v = -1.0;
     int64_t vi = INT64_MAX;
     for (int q = tid; q < G; q += kPanelThreads) {
-      const T cv = a.cand_val[par * G + q];
-      const int64_t ci = a.cand_idx[par * G + q];
-      if (cv > v || (cv == v && ci < vi)) {
-        v = cv;
-        vi = ci;
-      }
+      const double* o = a.cand + 2 * ((int64_t)par * G + q);
+      const double ov = o[0];
+      if (ov >= 0.0) argmax_merge(v, vi, ov, (int64_t)o[1]);
     }
-    T wv;
+    double wv;
     int64_t wi;
-    block_argmax<T>(v, vi, wv, wi);
-    if (!(wv >= T(0))) {
+    block_argmax(v, vi, s_v, s_i, wv, wi);
+    if (!(wv >= 0.0)) {
       // no finite candidate anywhere (NaN column): every workgroup sees the
       // same winner value, so all of them leave here together
       if (g == 0 && tid == 0) {
@@ -369,26 +374,15 @@ __global__ __launch_bounds__(kPanelThreads) void panel_kernel(PanelArgs<T> a) {
       }
       return;
     }
-    if (tid == 0) {
-      s_win = wi;
-      s_wg = (int)(wi / a.R);
-    }
-    __syncthreads();
-    const int64_t prow = s_win;
-    const int wg = s_wg;
+    const int wg = (int)(wi / a.R);
     if (tid < w) urow[tid] = a.cand_row[((int64_t)par * G + wg) * w + tid];
-    if (prow >= rb && prow < re && tid == 0) retired[prow - rb] = 1;
-    if (g == 0 && tid == 0) {
-      const int pp = a.phys2pos[prow];
-      const int q = a.pos2phys[j];
-      a.pos2phys[j] = (int)prow;
-      a.pos2phys[pp] = q;
-      a.phys2pos[prow] = j;
-      a.phys2pos[q] = pp;
-      a.ipiv[j] = a.r0 + pp;
-      if (wv == T(0) && *a.info == 0) *a.info = (int32_t)(a.r0 + j + 1);
+    if (tid == 0) {
+      s_win[j] = wi;
+      s_zero[j] = wv == 0.0;
+      if (wi >= rb && wi < re) retired[wi - rb] = 1;
     }
     __syncthreads();
+    // 5. eliminate column j from this workgroup's active rows
     const T piv = urow[j];
     const T rpiv = piv != T(0) ? T(1) / piv : T(0);
     for (int r = tid; r < nr; r += kPanelThreads) {
@@ -397,14 +391,56 @@ __global__ __launch_bounds__(kPanelThreads) void panel_kernel(PanelArgs<T> a) {
       P[(int64_t)j * a.RP + r] = l;
       for (int c = j + 1; c < w; ++c) P[(int64_t)c * a.RP + r] -= l * urow[c];
     }
-    __syncthreads();
   }
-  ++nbar;
-  if (!grid_step(a.flags, a.seq_base + nbar, a.status)) return;
-  for (int64_t idx = tid; idx < (int64_t)nr * w; idx += kPanelThreads) {
-    const int64_t r = idx / w, c = idx % w;
-    const int64_t dest = a.phys2pos[rb + r];
-    a.A[dest * a.lda + c] = P[c * a.RP + r];
+
+  // Replay the interchanges (LAPACK order: at step j rows at positions j and
+  // pos(winner) swap) to find every row's final position; wave 0 only.
+  if (tid < 64) {
+    if (tid == 0) ml.cnt = 0;
+    wave_sync();
+    for (int j = 0; j < w; ++j) {
+      const int64_t p = s_win[j];
+      const int cnt = ml.cnt;
+      const int ip = ml_find(ml.phys, cnt, p);
+      const int64_t pp = ip >= 0 ? ml.pos[ip] : p;         // position of the winner
+      const int iq = ml_find(ml.pos, cnt, (int64_t)j);
+      const int64_t q = iq >= 0 ? ml.phys[iq] : (int64_t)j; // row at position j
+      if (tid == 0) {
+        int k = ip;
+        if (k < 0) {
+          k = ml.cnt++;
+          ml.phys[k] = p;
+        }
+        ml.pos[k] = j;
+        if (q != p) {
+          // no listed row at position j means its occupant is row j itself,
+          // which is then not listed either
+          int kq = iq;
+          if (kq < 0) {
+            kq = ml.cnt++;
+            ml.phys[kq] = q;
+          }
+          ml.pos[kq] = pp;
+        }
+        if (g == 0) {
+          a.ipiv[j] = a.r0 + pp;
+          if (s_zero[j] && *a.info == 0) *a.info = (int32_t)(a.r0 + j + 1);
+        }
+      }
+      wave_sync();
+    }
+  }
+  __syncthreads();
+  // write every row of this workgroup to its final position
+  const int cnt = ml.cnt;
+  for (int r = tid; r < nr; r += kPanelThreads) {
+    int64_t dest = rb + r;
+    for (int k = 0; k < cnt; ++k)
+      if (ml.phys[k] == rb + r) {
+        dest = ml.pos[k];
+        break;
+      }
+    for (int c = 0; c < w; ++c) a.A[dest * a.lda + c] = P[(int64_t)c * a.RP + r];
   }
 }
 
@@ -551,12 +587,9 @@ struct LuCtx {
   int32_t* info;
   hipStream_t s;
   // workspace
-  T* cand_val;
-  int64_t* cand_idx;
-  T* cand_row;
-  int* pos2phys;
-  int* phys2pos;
-  unsigned* flags;  // [kMaxPanelGroups] step flags of the panel launches
+  double* cand;     // [2][G_max] (|value|, row) records of the panel launches
+  T* cand_row;      // [2][G_max][kPanelWMax]
+  unsigned* flags;  // [G_max] step flags of the panel launches
   int* status;
   unsigned seq;     // steps taken by the panel launches so far
   int64_t* plan_dst;
@@ -569,12 +602,16 @@ struct LuCtx {
   size_t lds_budget;
 };
 
+// Static LDS of panel_kernel (argmax slots, winners, move list), kept free
+// of the dynamic panel storage.
+constexpr size_t kPanelStaticLds = 8192;
+
 static int panel_width_max(int64_t n, int bytes, const DeviceInfo& di, size_t lds_budget) {
   // widest leaf whose LDS share (rows per workgroup x w) fits with num_cus groups
   for (int w = kPanelWMax; w >= 8; w /= 2) {
     const int64_t R = ceil_div(n, di.num_cus);
     const size_t need = (size_t)w * (size_t)(R + 1) * bytes + kPanelWMax * bytes + (size_t)(R + 1) * 4;
-    if (need <= lds_budget) return w;
+    if (need + kPanelStaticLds <= lds_budget) return w;
   }
   return 0;
 }
@@ -589,11 +626,12 @@ static int leaf(LuCtx<T>& c, int64_t r0, int w) {
     const int64_t RP = Rr + 1;
     return (size_t)w * RP * sizeof(T) + kPanelWMax * sizeof(T) + (size_t)RP * 4;
   };
-  while (need(R) > c.lds_budget && G < di.num_cus) {
+  const size_t budget = c.lds_budget - kPanelStaticLds;  // room for the kernel's static LDS
+  while (need(R) > budget && G < di.num_cus) {
     G = std::min<int>(di.num_cus, G * 2);
     R = ceil_div(m, G);
   }
-  RBFX_REQUIRE(need(R) <= c.lds_budget, "panel does not fit in LDS (m=%lld, w=%d)", (long long)m, w);
+  RBFX_REQUIRE(need(R) <= budget, "panel does not fit in LDS (m=%lld, w=%d)", (long long)m, w);
   G = (int)ceil_div(m, R);
   PanelArgs<T> a;
   a.A = c.A + r0 * c.lda + r0;
@@ -604,22 +642,20 @@ static int leaf(LuCtx<T>& c, int64_t r0, int w) {
   a.RP = R + 1;
   a.r0 = r0;
   a.ipiv = c.ipiv + r0;
-  a.cand_val = c.cand_val;
-  a.cand_idx = c.cand_idx;
+  a.cand = c.cand;
   a.cand_row = c.cand_row;
-  a.pos2phys = c.pos2phys;
-  a.phys2pos = c.phys2pos;
   a.flags = c.flags;
   a.seq_base = c.seq;
   a.status = c.status;
   a.info = c.info;
+  RBFX_REQUIRE(G <= c.G_max, "too many panel workgroups");
   const size_t smem = need(R);
   void* args[] = {&a};
   const int pid = prof_start(sizeof(T) == 8 ? "panel_f64" : "panel_f32", c.s);
   RBFX_HIP(hipLaunchCooperativeKernel(reinterpret_cast<const void*>(&panel_kernel<T>), dim3(G),
                                       dim3(kPanelThreads), args, (unsigned)smem, c.s));
   prof_stop(pid, c.s, (double)m * w * w, 2.0 * sizeof(T) * m * w);
-  c.seq += (unsigned)(w + 1);
+  c.seq += (unsigned)w;
   return RBFX_OK;
 }
 
@@ -688,10 +724,8 @@ constexpr int kNB = 256;
 static size_t getrf_ws_bytes(int64_t n, int bytes) {
   const int G = 1024;  // upper bound on workgroups of a leaf
   size_t b = 0;
-  b += ws_round((size_t)2 * G * bytes);           // cand_val
-  b += ws_round((size_t)2 * G * 8);               // cand_idx
+  b += ws_round((size_t)4 * G * 8);               // cand records
   b += ws_round((size_t)2 * G * kPanelWMax * bytes);  // cand_row
-  b += ws_round((size_t)n * 4) * 2;               // pos2phys, phys2pos
   b += ws_round(64);                              // status
   b += ws_round((size_t)G * 4);                   // panel step flags
   b += ws_round((size_t)2 * kSwapMax * 8) * 2;    // plan dst / src
@@ -718,11 +752,8 @@ static int getrf(int64_t n, T* A, int64_t lda, int64_t* ipiv, void* work, size_t
   c.s = s;
   Workspace ws(work, work_bytes);
   const int G = 1024;
-  c.cand_val = ws.take<T>(2 * G);
-  c.cand_idx = ws.take<int64_t>(2 * G);
+  c.cand = ws.take<double>((size_t)4 * G);
   c.cand_row = ws.take<T>((size_t)2 * G * kPanelWMax);
-  c.pos2phys = ws.take<int>(n);
-  c.phys2pos = ws.take<int>(n);
   unsigned* ctl = ws.take<unsigned>(16);
   c.status = reinterpret_cast<int*>(ctl + 1);
   c.flags = ws.take<unsigned>(G);
