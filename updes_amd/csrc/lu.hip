@@ -19,6 +19,7 @@
 //      (v_mfma_f64_16x16x4f64 / v_mfma_f32_16x16x4f32, 128x128 tiles,
 //      XCD-aware tile order).
 #include <algorithm>
+#include <type_traits>
 #include <vector>
 
 #include "common.h"
@@ -197,23 +198,46 @@ static int gemm(int64_t m, int64_t n, int64_t k, T alpha, const T* A, int64_t ld
 }
 
 // ---------------------------------------------------------------------------
+// Exchange between the workgroups of the cooperative panel kernel goes
+// through agent-coherent relaxed loads and stores (they bypass the
+// non-coherent cache levels) ordered by waiting for the issuing thread's
+// memory operations to complete; no cache-wide write-back or invalidation is
+// needed, which is what made release/acquire fences cost several us per
+// column (measured).
+template <typename T>
+__device__ inline void st_coh(T* p, T v) {
+  static_assert(sizeof(T) == 4 || sizeof(T) == 8, "32/64-bit only");
+  typedef typename std::conditional<sizeof(T) == 8, unsigned long long, unsigned>::type U;
+  U u;
+  __builtin_memcpy(&u, &v, sizeof(T));
+  __hip_atomic_store(reinterpret_cast<U*>(p), u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+template <typename T>
+__device__ inline T ld_coh(const T* p) {
+  typedef typename std::conditional<sizeof(T) == 8, unsigned long long, unsigned>::type U;
+  U u = __hip_atomic_load(reinterpret_cast<const U*>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  T v;
+  __builtin_memcpy(&v, &u, sizeof(T));
+  return v;
+}
+
 // Grid-wide step barrier for the cooperative panel kernel (all workgroups
-// co-resident).  Workgroup g announces step `target` in its own flag and then
-// waits until every workgroup's flag has reached it: one release store per
-// workgroup and parallel polling, instead of all arrivals serialising on a
-// single counter.  Flags only grow (targets are cumulative per factorisation).
+// co-resident).  Every thread's coherent stores of the step are complete
+// (s_waitcnt 0) before workgroup g announces step `target` in its own flag;
+// then every workgroup polls all flags in parallel.  Flags only grow (targets
+// are cumulative per factorisation).
 __device__ inline bool grid_step(unsigned* flags, unsigned target, int* status) {
-  __threadfence();
+  __builtin_amdgcn_s_waitcnt(0);
   __syncthreads();
   __shared__ int bad;
   if (threadIdx.x == 0) {
     bad = 0;
-    __hip_atomic_store(flags + blockIdx.x, target, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    st_coh(flags + blockIdx.x, target);
   }
   __syncthreads();
   for (unsigned q = threadIdx.x; q < gridDim.x; q += blockDim.x) {
     uint32_t spins = 0;
-    while (__hip_atomic_load(flags + q, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < target) {
+    while (ld_coh(flags + q) < target) {
       __builtin_amdgcn_s_sleep(1);
       ++spins;
       if ((spins & 255u) == 0 &&
@@ -229,7 +253,6 @@ __device__ inline bool grid_step(unsigned* flags, unsigned target, int* status) 
     }
   }
   __syncthreads();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
   return bad == 0;
 }
 
@@ -344,14 +367,10 @@ __global__ __launch_bounds__(kPanelThreads) void panel_kernel(PanelArgs<T> a) {
     // 2. publish it: (|value|, row) and the row's w entries
     double* rec = a.cand + 2 * ((int64_t)par * G + g);
     if (tid == 0) {
-      rec[0] = bv;
-      rec[1] = (double)bi;
-      __threadfence();
+      st_coh(rec, bv);
+      st_coh(rec + 1, (double)bi);
     }
-    if (bv >= 0.0 && tid < w) {
-      a.cand_row[((int64_t)par * G + g) * w + tid] = P[(int64_t)tid * a.RP + (bi - rb)];
-      __threadfence();
-    }
+    if (bv >= 0.0 && tid < w) st_coh(a.cand_row + ((int64_t)par * G + g) * w + tid, P[(int64_t)tid * a.RP + (bi - rb)]);
     // 3. wait until every workgroup has published column j
     if (!grid_step(a.flags, a.seq_base + (unsigned)(j + 1), a.status)) return;
     // 4. the winner across workgroups
@@ -359,8 +378,8 @@ __global__ __launch_bounds__(kPanelThreads) void panel_kernel(PanelArgs<T> a) {
     int64_t vi = INT64_MAX;
     for (int q = tid; q < G; q += kPanelThreads) {
       const double* o = a.cand + 2 * ((int64_t)par * G + q);
-      const double ov = o[0];
-      if (ov >= 0.0) argmax_merge(v, vi, ov, (int64_t)o[1]);
+      const double ov = ld_coh(o);
+      if (ov >= 0.0) argmax_merge(v, vi, ov, (int64_t)ld_coh(o + 1));
     }
     double wv;
     int64_t wi;
@@ -375,7 +394,7 @@ __global__ __launch_bounds__(kPanelThreads) void panel_kernel(PanelArgs<T> a) {
       return;
     }
     const int wg = (int)(wi / a.R);
-    if (tid < w) urow[tid] = a.cand_row[((int64_t)par * G + wg) * w + tid];
+    if (tid < w) urow[tid] = ld_coh(a.cand_row + ((int64_t)par * G + wg) * w + tid);
     if (tid == 0) {
       s_win[j] = wi;
       s_zero[j] = wv == 0.0;
@@ -431,7 +450,8 @@ __global__ __launch_bounds__(kPanelThreads) void panel_kernel(PanelArgs<T> a) {
     }
   }
   __syncthreads();
-  // write every row of this workgroup to its final position
+  // final position of every row of this workgroup (kept in the retired[]
+  // slots, no longer needed), then a row-contiguous write-back
   const int cnt = ml.cnt;
   for (int r = tid; r < nr; r += kPanelThreads) {
     int64_t dest = rb + r;
@@ -440,7 +460,12 @@ __global__ __launch_bounds__(kPanelThreads) void panel_kernel(PanelArgs<T> a) {
         dest = ml.pos[k];
         break;
       }
-    for (int c = 0; c < w; ++c) a.A[dest * a.lda + c] = P[(int64_t)c * a.RP + r];
+    retired[r] = (int)dest;
+  }
+  __syncthreads();
+  for (int64_t idx = tid; idx < (int64_t)nr * w; idx += kPanelThreads) {
+    const int64_t r = idx / w, c = idx % w;
+    a.A[(int64_t)retired[r] * a.lda + c] = P[c * a.RP + r];
   }
 }
 
