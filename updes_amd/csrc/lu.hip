@@ -354,13 +354,15 @@ __global__ __launch_bounds__(kPanelThreads) void panel_kernel(PanelArgs<T> a) {
   for (int r = tid; r < nr; r += kPanelThreads) retired[r] = 0;
   __syncthreads();
 
+  // best active row of this workgroup for column 0 (later columns are scanned
+  // inside the elimination loop)
+  double best = -1.0;
+  int64_t bidx = INT64_MAX;
+  for (int r = tid; r < nr; r += kPanelThreads) argmax_merge(best, bidx, (double)fabs(P[r]), rb + r);
+
   for (int j = 0; j < w; ++j) {
     const int par = j & 1;
     // 1. best active row of this workgroup for column j
-    double best = -1.0;
-    int64_t bidx = INT64_MAX;
-    for (int r = tid; r < nr; r += kPanelThreads)
-      if (!retired[r]) argmax_merge(best, bidx, (double)fabs(P[(int64_t)j * a.RP + r]), rb + r);
     double bv;
     int64_t bi;
     block_argmax(best, bidx, s_v, s_i, bv, bi);
@@ -379,7 +381,8 @@ __global__ __launch_bounds__(kPanelThreads) void panel_kernel(PanelArgs<T> a) {
     for (int q = tid; q < G; q += kPanelThreads) {
       const double* o = a.cand + 2 * ((int64_t)par * G + q);
       const double ov = ld_coh(o);
-      if (ov >= 0.0) argmax_merge(v, vi, ov, (int64_t)ld_coh(o + 1));
+      const double oi = ld_coh(o + 1);
+      if (ov >= 0.0) argmax_merge(v, vi, ov, (int64_t)oi);
     }
     double wv;
     int64_t wi;
@@ -401,14 +404,18 @@ __global__ __launch_bounds__(kPanelThreads) void panel_kernel(PanelArgs<T> a) {
       if (wi >= rb && wi < re) retired[wi - rb] = 1;
     }
     __syncthreads();
-    // 5. eliminate column j from this workgroup's active rows
+    // 5. eliminate column j from this workgroup's active rows, tracking the
+    //    best candidate of column j+1 on the way
     const T piv = urow[j];
     const T rpiv = piv != T(0) ? T(1) / piv : T(0);
+    best = -1.0;
+    bidx = INT64_MAX;
     for (int r = tid; r < nr; r += kPanelThreads) {
       if (retired[r]) continue;
       const T l = P[(int64_t)j * a.RP + r] * rpiv;
       P[(int64_t)j * a.RP + r] = l;
       for (int c = j + 1; c < w; ++c) P[(int64_t)c * a.RP + r] -= l * urow[c];
+      if (j + 1 < w) argmax_merge(best, bidx, (double)fabs(P[(int64_t)(j + 1) * a.RP + r]), rb + r);
     }
   }
 
