@@ -228,11 +228,12 @@ constexpr int kAsmThreads = 256;
 constexpr int kAsmCols = 2 * kAsmThreads;  // columns per block
 constexpr int kAsmRows = 16;               // rows per block
 
-template <bool W64, bool W32>
+template <int KIND, bool FAST, bool W64, bool W32>
 __global__ __launch_bounds__(kAsmThreads) void assemble_kernel(ProbDev P, int64_t row0,
                                                                int64_t nrows, double* out64,
                                                                int64_t ld64, float* out32,
                                                                int64_t ld32) {
+  pin_kind<KIND, FAST>(P.k);
   const int64_t j0 = (int64_t)blockIdx.x * kAsmCols + 2 * threadIdx.x;
   const int64_t rb = (int64_t)blockIdx.y * kAsmRows;
   if (j0 >= P.n) return;
@@ -274,8 +275,10 @@ __global__ __launch_bounds__(kAsmThreads) void assemble_kernel(ProbDev P, int64_
 constexpr int kMvThreads = 256;
 constexpr int kMvRows = 4;
 
+template <int KIND, bool FAST>
 __global__ __launch_bounds__(kMvThreads) void matvec_kernel(ProbDev P, const double* __restrict__ xv,
                                                             double* __restrict__ yv) {
+  pin_kind<KIND, FAST>(P.k);
   const int64_t r0 = (int64_t)blockIdx.x * kMvRows;
   RowCtx c[kMvRows];
   double acc[kMvRows];
@@ -316,9 +319,11 @@ __global__ __launch_bounds__(kMvThreads) void matvec_kernel(ProbDev P, const dou
 // Partial y^T = x^T K over a chunk of rows: part[chunk][j].
 constexpr int kMtThreads = 256;
 
+template <int KIND, bool FAST>
 __global__ __launch_bounds__(kMtThreads) void matvec_t_kernel(ProbDev P, const double* __restrict__ xv,
                                                               int64_t rows_per_chunk,
                                                               double* __restrict__ part) {
+  pin_kind<KIND, FAST>(P.k);
   const int64_t j = (int64_t)blockIdx.x * kMtThreads + threadIdx.x;
   const int64_t rs = (int64_t)blockIdx.y * rows_per_chunk;
   const int64_t re = rs + rows_per_chunk < P.n ? rs + rows_per_chunk : P.n;
@@ -409,15 +414,17 @@ extern "C" int rbfx_assemble(const rbfx_problem* p, int matrix, int64_t row0, in
   RBFX_REQUIRE(grid.y <= 65535u * 1024u, "too many rows");
   hipStream_t s = as_stream(stream);
   const int pid = prof_start(d_out64 ? (d_out32 ? "assemble_f64_f32" : "assemble_f64") : "assemble_f32", s);
-  if (d_out64 && d_out32)
-    hipLaunchKernelGGL((assemble_kernel<true, true>), grid, dim3(kAsmThreads), 0, s, P, row0,
-                       nrows, d_out64, ld64, d_out32, ld32);
-  else if (d_out64)
-    hipLaunchKernelGGL((assemble_kernel<true, false>), grid, dim3(kAsmThreads), 0, s, P, row0,
-                       nrows, d_out64, ld64, d_out32, ld32);
-  else
-    hipLaunchKernelGGL((assemble_kernel<false, true>), grid, dim3(kAsmThreads), 0, s, P, row0,
-                       nrows, d_out64, ld64, d_out32, ld32);
+  RBFX_KIND_SWITCH(P.k, {
+    if (d_out64 && d_out32)
+      hipLaunchKernelGGL((assemble_kernel<KIND_, FAST_, true, true>), grid, dim3(kAsmThreads), 0, s, P, row0,
+                         nrows, d_out64, ld64, d_out32, ld32);
+    else if (d_out64)
+      hipLaunchKernelGGL((assemble_kernel<KIND_, FAST_, true, false>), grid, dim3(kAsmThreads), 0, s, P, row0,
+                         nrows, d_out64, ld64, d_out32, ld32);
+    else
+      hipLaunchKernelGGL((assemble_kernel<KIND_, FAST_, false, true>), grid, dim3(kAsmThreads), 0, s, P, row0,
+                         nrows, d_out64, ld64, d_out32, ld32);
+  });
   prof_stop(pid, s, 0.0, (double)nrows * P.n * ((d_out64 ? 8.0 : 0.0) + (d_out32 ? 4.0 : 0.0)));
   RBFX_CHECK_LAUNCH();
   return RBFX_OK;
@@ -450,8 +457,10 @@ extern "C" int rbfx_matvec(const rbfx_problem* p, int matrix, int trans, const d
   hipStream_t s = as_stream(stream);
   if (!trans) {
     const int pid = prof_start("matvec", s);
-    hipLaunchKernelGGL(matvec_kernel, dim3((unsigned)ceil_div(P.n, kMvRows)), dim3(kMvThreads), 0,
-                       s, P, d_x, d_y);
+    RBFX_KIND_SWITCH(P.k, {
+      hipLaunchKernelGGL((matvec_kernel<KIND_, FAST_>), dim3((unsigned)ceil_div(P.n, kMvRows)), dim3(kMvThreads),
+                         0, s, P, d_x, d_y);
+    });
     prof_stop(pid, s, 0.0, 16.0 * P.n);
     RBFX_CHECK_LAUNCH();
     return RBFX_OK;
@@ -463,8 +472,10 @@ extern "C" int rbfx_matvec(const rbfx_problem* p, int matrix, int trans, const d
   const int64_t rpc = ceil_div(P.n, chunks);
   const int64_t used_chunks = ceil_div(P.n, rpc);
   double* part = static_cast<double*>(d_work);
-  hipLaunchKernelGGL(matvec_t_kernel, dim3((unsigned)ceil_div(P.n, kMtThreads), (unsigned)used_chunks),
-                     dim3(kMtThreads), 0, s, P, d_x, rpc, part);
+  RBFX_KIND_SWITCH(P.k, {
+    hipLaunchKernelGGL((matvec_t_kernel<KIND_, FAST_>), dim3((unsigned)ceil_div(P.n, kMtThreads),
+                       (unsigned)used_chunks), dim3(kMtThreads), 0, s, P, d_x, rpc, part);
+  });
   RBFX_CHECK_LAUNCH();
   hipLaunchKernelGGL(sum_chunks_kernel, dim3((unsigned)ceil_div(P.n, 256)), dim3(256), 0, s, part,
                      used_chunks, P.n, d_y);

@@ -76,6 +76,35 @@ constexpr uint32_t kSpinLimit = 1u << 22;
 
 }  // namespace rbfx
 
+#include <type_traits>
+
+namespace rbfx {
+// Data exchanged between workgroups of one kernel (pivot records, solved
+// slices of a triangular sweep) goes through agent-coherent relaxed loads and
+// stores, which bypass the non-coherent cache levels.  The writer waits for
+// completion of its stores (s_waitcnt 0) before it publishes a flag, and the
+// reader reads the data after it has seen the flag; no cache-wide write-back
+// or invalidation (what agent-scope release/acquire fences cost, several us
+// per exchange on MI355X, measured) is involved.
+template <typename T>
+__device__ inline void st_coh(T* p, T v) {
+  static_assert(sizeof(T) == 4 || sizeof(T) == 8, "32/64-bit only");
+  typedef typename std::conditional<sizeof(T) == 8, unsigned long long, unsigned>::type U;
+  U u;
+  __builtin_memcpy(&u, &v, sizeof(T));
+  __hip_atomic_store(reinterpret_cast<U*>(p), u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+template <typename T>
+__device__ inline T ld_coh(const T* p) {
+  static_assert(sizeof(T) == 4 || sizeof(T) == 8, "32/64-bit only");
+  typedef typename std::conditional<sizeof(T) == 8, unsigned long long, unsigned>::type U;
+  U u = __hip_atomic_load(reinterpret_cast<const U*>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  T v;
+  __builtin_memcpy(&v, &u, sizeof(T));
+  return v;
+}
+}  // namespace rbfx
+
 namespace rbfx {
 // Optional per-launch timing with HIP events on the launch stream
 // (rbfx_profile_begin / rbfx_profile_end).  When disabled, prof_start()

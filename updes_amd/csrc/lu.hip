@@ -198,29 +198,6 @@ static int gemm(int64_t m, int64_t n, int64_t k, T alpha, const T* A, int64_t ld
 }
 
 // ---------------------------------------------------------------------------
-// Exchange between the workgroups of the cooperative panel kernel goes
-// through agent-coherent relaxed loads and stores (they bypass the
-// non-coherent cache levels) ordered by waiting for the issuing thread's
-// memory operations to complete; no cache-wide write-back or invalidation is
-// needed, which is what made release/acquire fences cost several us per
-// column (measured).
-template <typename T>
-__device__ inline void st_coh(T* p, T v) {
-  static_assert(sizeof(T) == 4 || sizeof(T) == 8, "32/64-bit only");
-  typedef typename std::conditional<sizeof(T) == 8, unsigned long long, unsigned>::type U;
-  U u;
-  __builtin_memcpy(&u, &v, sizeof(T));
-  __hip_atomic_store(reinterpret_cast<U*>(p), u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-template <typename T>
-__device__ inline T ld_coh(const T* p) {
-  typedef typename std::conditional<sizeof(T) == 8, unsigned long long, unsigned>::type U;
-  U u = __hip_atomic_load(reinterpret_cast<const U*>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  T v;
-  __builtin_memcpy(&v, &u, sizeof(T));
-  return v;
-}
-
 // Grid-wide step barrier for the cooperative panel kernel (all workgroups
 // co-resident).  Every thread's coherent stores of the step are complete
 // (s_waitcnt 0) before workgroup g announces step `target` in its own flag;

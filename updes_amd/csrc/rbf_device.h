@@ -48,6 +48,57 @@ __host__ __device__ inline Kern make_kern(int kind, double param) {
   return k;
 }
 
+// Kernels are instantiated per (kind, a == 1): pinning the kind in the local
+// copy of Kern turns every switch below into straight-line code.
+template <int KIND, bool FAST>
+__device__ inline void pin_kind(Kern& k) {
+  k.kind = KIND;
+  k.fast = FAST;
+}
+
+#define RBFX_KIND_SWITCH(k, ...)                                        \
+  do {                                                                  \
+    switch ((k).kind) {                                                 \
+      case RBFX_POLYHARMONIC:                                           \
+        if ((k).fast) {                                                 \
+          constexpr int KIND_ = RBFX_POLYHARMONIC;                      \
+          constexpr bool FAST_ = true;                                  \
+          __VA_ARGS__;                                                  \
+        } else {                                                        \
+          constexpr int KIND_ = RBFX_POLYHARMONIC;                      \
+          constexpr bool FAST_ = false;                                 \
+          __VA_ARGS__;                                                  \
+        }                                                               \
+        break;                                                          \
+      case RBFX_GAUSSIAN: {                                             \
+        constexpr int KIND_ = RBFX_GAUSSIAN;                            \
+        constexpr bool FAST_ = false;                                   \
+        __VA_ARGS__;                                                    \
+      } break;                                                          \
+      case RBFX_THIN_PLATE:                                             \
+        if ((k).fast) {                                                 \
+          constexpr int KIND_ = RBFX_THIN_PLATE;                        \
+          constexpr bool FAST_ = true;                                  \
+          __VA_ARGS__;                                                  \
+        } else {                                                        \
+          constexpr int KIND_ = RBFX_THIN_PLATE;                        \
+          constexpr bool FAST_ = false;                                 \
+          __VA_ARGS__;                                                  \
+        }                                                               \
+        break;                                                          \
+      case RBFX_MULTIQUADRIC: {                                         \
+        constexpr int KIND_ = RBFX_MULTIQUADRIC;                        \
+        constexpr bool FAST_ = false;                                   \
+        __VA_ARGS__;                                                    \
+      } break;                                                          \
+      default: {                                                        \
+        constexpr int KIND_ = RBFX_INV_MULTIQUADRIC;                    \
+        constexpr bool FAST_ = false;                                   \
+        __VA_ARGS__;                                                    \
+      } break;                                                          \
+    }                                                                   \
+  } while (0)
+
 // phi, f1 = phi'/r and g2 = phi'' - phi'/r (= r^2 f2) for r > 0.  Using g2
 // keeps the Laplacian division-free: Lap phi = g2 + 2 f1.
 __device__ inline void radial(const Kern& k, double r, double& phi, double& f1, double& g2) {

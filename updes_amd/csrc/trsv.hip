@@ -146,7 +146,7 @@ __global__ __launch_bounds__(kTsThreads) void sweep_kernel(const TF* __restrict_
       if (tid == 0) {
         unsigned v;
         uint32_t spins = 0;
-        while ((v = __hip_atomic_load(done, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT)) <= (unsigned)d) {
+        while ((v = ld_coh(done)) <= (unsigned)d) {
           __builtin_amdgcn_s_sleep(1);
           if (++spins > kSpinLimit) {
             __hip_atomic_store(status, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -162,7 +162,6 @@ __global__ __launch_bounds__(kTsThreads) void sweep_kernel(const TF* __restrict_
         s_avail = v;
       }
       __syncthreads();
-      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
       if (s_avail == 0xffffffffu) return;
     }
     const int lim = (unsigned)seq < s_avail ? seq : (int)s_avail;
@@ -175,18 +174,18 @@ __global__ __launch_bounds__(kTsThreads) void sweep_kernel(const TF* __restrict_
 #pragma unroll
         for (int k = 0; k < 16; ++k) {
           const int kk = q * 16 + k;
-          if (kk < szJ) s += pre[k] * y[oJ + kk];
+          if (kk < szJ) s += pre[k] * ld_coh(y + oJ + kk);
         }
       } else if (!TRANS) {
         const TF* row = F + (oI + rr) * lda + oJ + q * 16;
 #pragma unroll
         for (int k = 0; k < 16; ++k)
-          if (q * 16 + k < szJ) s += (T)row[k] * y[oJ + q * 16 + k];
+          if (q * 16 + k < szJ) s += (T)row[k] * ld_coh(y + oJ + q * 16 + k);
       } else {
 #pragma unroll
         for (int k = 0; k < 16; ++k) {
           const int kk = q * 16 + k;
-          if (kk < szJ) s += (T)F[(oJ + kk) * lda + oI + rr] * y[oJ + kk];
+          if (kk < szJ) s += (T)F[(oJ + kk) * lda + oI + rr] * ld_coh(y + oJ + kk);
         }
       }
     }
@@ -214,11 +213,12 @@ __global__ __launch_bounds__(kTsThreads) void sweep_kernel(const TF* __restrict_
     }
     t += __shfl_xor(t, 1, 64);
     t += __shfl_xor(t, 2, 64);
-    if (qq == 0 && r < szI) y[oI + r] = t;
+    if (qq == 0 && r < szI) st_coh(y + oI + r, t);
   }
-  __threadfence();
+  // publish: the slice's coherent stores are complete before the counter moves
+  __builtin_amdgcn_s_waitcnt(0);
   __syncthreads();
-  if (tid == 0) __hip_atomic_fetch_add(done, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+  if (tid == 0) __hip_atomic_fetch_add(done, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 template <typename T>
