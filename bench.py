@@ -10,11 +10,15 @@ One step = one solve (assembly + factorisation + solve + nodal values, for a
 new operator/boundary pair: the factorisation cache is cleared) and one
 gradient (dL/db for the 64 source coefficients through the adjoint).  Both
 precision paths are measured: FP64 LU, and FP32-MFMA LU + FP64 GMRES-IR
-(falling back to FP64 LU when refinement stalls).  `value` is the
-throughput of the mixed-precision path; the FP64 path and the error of the
-mixed path against it are reported beside it.
+(falling back to FP64 LU when refinement stalls).  `value` is the throughput
+of the faster path whose values meet rel-L2 <= 1e-8 against the FP64 path
+(the FP64 path meets it by definition); both paths, the refinement statistics
+and the error of the mixed path are reported beside it.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--side 200]
+                       [--precisions fp64,fp32_ir] [--copy-probe-mib M]
+--copy-probe-mib runs one plain device copy of M MiB before the workload, so a
+counter run (FETCH_SIZE / WRITE_SIZE) can be scaled by a copy of known size.
 """
 import argparse
 import ctypes
@@ -135,6 +139,8 @@ def main():
     ap.add_argument("--side", type=int, default=200)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--precisions", default="fp64,fp32_ir")
+    ap.add_argument("--copy-probe-mib", type=int, default=0)
+    ap.add_argument("--no-kernel-profile", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: the dense solve of one collocation system runs on a single MI355X; "
@@ -147,6 +153,15 @@ def main():
     from updes_amd import _lib as L
     L.require_gpu()
     torch.cuda.set_device(0)
+    if args.copy_probe_mib > 0:
+        nel = args.copy_probe_mib * (1 << 20) // 8
+        src = torch.ones(nel, dtype=torch.float64, device="cuda")
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        print("[bench] copy probe: %d bytes read, %d bytes written" % (8 * nel, 8 * nel), file=sys.stderr, flush=True)
+        del src, dst
     cloud, bcs, b, basis = fourier_problem(args.side)
     n = cloud.N + 6
     print("[bench] N=%d n=%d, precisions %s" % (cloud.N, n, args.precisions), file=sys.stderr, flush=True)
@@ -169,43 +184,58 @@ def main():
         results[prec] = dict(ms=dt * 1e3, vals=vals, grad=g, loss=float(loss), stats=stats)
         print("[bench] %s: %.1f ms per solve+grad, stats %s" % (prec, dt * 1e3, stats), file=sys.stderr, flush=True)
 
-    # kernel-level profile of one FP64 and one FP32 factorisation (outside the timed loop)
+    # kernel-level profile of one solve+gradient per path (outside the timed loop)
     prof = {}
-    for prec in args.precisions.split(","):
-        u.clear_cache()
-        prof[prec] = profile_step(lambda: solve_and_grad(cloud, bcs, b, basis, prec))
-    head = "fp32_ir" if "fp32_ir" in results else list(results)[0]
+    if not args.no_kernel_profile:
+        for prec in args.precisions.split(","):
+            u.clear_cache()
+            prof[prec] = profile_step(lambda: solve_and_grad(cloud, bcs, b, basis, prec))
     ref = results.get("fp64")
+    verified = {}
+    for prec, rr in results.items():
+        if prec == "fp64" or ref is None:
+            verified[prec] = prec == "fp64"
+            continue
+        err = float(torch.linalg.norm(rr["vals"] - ref["vals"]) / torch.linalg.norm(ref["vals"]))
+        rr["err"] = err
+        rr["gerr"] = float(torch.linalg.norm(rr["grad"] - ref["grad"]) / torch.linalg.norm(ref["grad"]))
+        verified[prec] = err <= 1e-8
+    ok = [p for p in results if verified[p]] or list(results)
+    head = min(ok, key=lambda p: results[p]["ms"])
     out = {"metric": "solves/s + grads/s at N=40k, FP32-LU+FP64 IR vs FP64 LU, rel-L2 <= 1e-8 vs FP64"}
     r = results[head]
+    st = r["stats"] or {}
+    used = "fp64" if (head == "fp64" or st.get("fell_back")) else head
     out.update(value=1e3 / r["ms"], unit="solves/s", n_gpus=1, steps=args.steps, warmup=args.warmup,
                ms_per_step=r["ms"], higher_is_better=True, scaling="weak", vs_baseline=None,
-               dtype="f32-lu+f64-ir" if head == "fp32_ir" else "f64", data="synthetic")
+               dtype="f32-lu+f64-ir" if used == "fp32_ir" else "f64", data="synthetic")
     out["config"] = {"workload": "large Poisson %dx%d (N=%d, n=%d), r^3, max_degree 2, Neumann South, Fourier "
                                  "boundary data and 64-coefficient source; step = 1 solve + 1 adjoint gradient"
                                  % (args.side, args.side, cloud.N, n), "N": cloud.N, "n": n,
-                     "grads_per_s": 1e3 / r["ms"]}
+                     "precision": head, "factorisation_used": used, "grads_per_s": 1e3 / r["ms"]}
     for prec, rr in results.items():
         d = {"solves_per_s": 1e3 / rr["ms"], "grads_per_s": 1e3 / rr["ms"], "ms_per_solve_plus_grad": rr["ms"],
-             "ir": rr["stats"]}
-        if ref is not None and prec != "fp64":
-            d["rel_l2_vals_vs_fp64"] = float(torch.linalg.norm(rr["vals"] - ref["vals"]) / torch.linalg.norm(ref["vals"]))
-            d["rel_l2_grad_vs_fp64"] = float(torch.linalg.norm(rr["grad"] - ref["grad"]) / torch.linalg.norm(ref["grad"]))
+             "ir": rr["stats"], "meets_1e-8": verified[prec]}
+        if "err" in rr:
+            d["rel_l2_vals_vs_fp64"] = rr["err"]
+            d["rel_l2_grad_vs_fp64"] = rr["gerr"]
         out[prec] = d
-    if ref is not None and head != "fp64":
-        out["speedup_vs_fp64"] = ref["ms"] / r["ms"]
-    # roofline of the dominant kernel (trailing-update GEMM) in the headline path
-    gname = "gemm_f32" if head == "fp32_ir" and "gemm_f32" in prof.get(head, {}) else "gemm_f64"
-    pk = prof[head].get(gname) or prof[head].get("gemm_f64")
-    peak = PEAK_FP32_MFMA if gname == "gemm_f32" else PEAK_FP64_MFMA
-    achieved = pk["flops"] / (pk["total_ms"] * 1e-3)
-    out["roofline"] = {"bound": "mfma", "kernel": gname, "achieved": achieved / 1e12, "peak": peak / 1e12,
-                       "unit": "TFLOP/s", "frac": achieved / peak, "traffic": None,
-                       "launches": pk["launches"], "avg_launch_ms": pk["total_ms"] / pk["launches"]}
-    out["kernels"] = {p: {k: {"launches": v["launches"], "total_ms": round(v["total_ms"], 3),
-                              "tflops": (v["flops"] / (v["total_ms"] * 1e-3) / 1e12) if v["flops"] else None,
-                              "gbps": (v["bytes"] / (v["total_ms"] * 1e-3) / 1e9) if v["bytes"] else None}
-                          for k, v in pr.items()} for p, pr in prof.items()}
+    if ref is not None and "fp32_ir" in results:
+        out["fp32_ir_speedup_vs_fp64"] = ref["ms"] / results["fp32_ir"]["ms"]
+    if prof:
+        # roofline of the dominant kernel (trailing-update GEMM) of the factorisation that produced `value`
+        gname = "gemm_f32" if used == "fp32_ir" else "gemm_f64"
+        pk = prof[head].get(gname)
+        peak = PEAK_FP32_MFMA if gname == "gemm_f32" else PEAK_FP64_MFMA
+        achieved = pk["flops"] / (pk["total_ms"] * 1e-3)
+        out["roofline"] = {"bound": "mfma", "kernel": gname, "achieved": achieved / 1e12, "peak": peak / 1e12,
+                           "unit": "TFLOP/s", "frac": achieved / peak, "traffic": None,
+                           "algorithmic_bytes_per_launch": pk["bytes"] / pk["launches"],
+                           "launches": pk["launches"], "avg_launch_ms": pk["total_ms"] / pk["launches"]}
+        out["kernels"] = {p: {k: {"launches": v["launches"], "total_ms": round(v["total_ms"], 3),
+                                  "tflops": (v["flops"] / (v["total_ms"] * 1e-3) / 1e12) if v["flops"] else None,
+                                  "gbps": (v["bytes"] / (v["total_ms"] * 1e-3) / 1e9) if v["bytes"] else None}
+                              for k, v in pr.items()} for p, pr in prof.items()}
     if not args.no_cpu_baseline:
         print("[bench] cpu baseline ...", file=sys.stderr, flush=True)
         out["cpu_baseline"] = cpu_baseline()

@@ -81,3 +81,37 @@ def field_coefficients(field, cloud, kind, param, M):
     if field.shape[0] == cloud.N:
         return AS.core_compute_coefficients(field, cloud, kind, param, M)
     return field
+
+
+def integrate_field_points(Nx, Ny):
+    """Quadrature points and weights of ``integrate_field`` (operators.py:381-450).
+
+    Restated with the reference's quirks: the interior points are
+    ``linspace(h, 1-h, Nx-1)`` x ``linspace(h, 1-h, Ny-1)`` stacked as
+    ``array(meshgrid(x, y))`` (shape (2, Ny-1, Nx-1)) and reshaped straight to
+    (nb_squares, 2), which pairs consecutive entries (operators.py:402-404);
+    the edges use Nx-2 / Ny-2 points on the same span (408-429); the top-left
+    corner term evaluates at (1, 0) because it reuses ``quad_points``
+    (operators.py:439-441).  Weights: 1 inside, 1/2 on the edges, 1/4 at the
+    corners, all times the cell area.
+    """
+    hx, hy = 1.0 / (Nx - 1), 1.0 / (Ny - 1)
+    qx = np.linspace(hx, 1 - hx, Nx - 1)
+    qy = np.linspace(hy, 1 - hy, Ny - 1)
+    inner = np.array(np.meshgrid(qx, qy)).reshape((Nx - 1) * (Ny - 1), 2)
+    ey = np.linspace(hy, 1 - hy, Ny - 2)
+    ex = np.linspace(hx, 1 - hx, Nx - 2)
+    left = np.stack([np.zeros(Ny - 2), ey], -1)
+    right = np.stack([np.ones(Ny - 2), ey], -1)
+    bottom = np.stack([ex, np.zeros(Nx - 2)], -1)
+    top = np.stack([ex, np.ones(Nx - 2)], -1)
+    corners = np.array([[0.0, 0.0], [1.0, 0.0], [1.0, 0.0], [1.0, 1.0]])
+    pts = np.concatenate([inner, left, right, bottom, top, corners])
+    w = np.concatenate([np.ones(len(inner)), np.full(2 * (Ny - 2) + 2 * (Nx - 2), 0.5), np.full(4, 0.25)])
+    return pts, w * hx * hy
+
+
+def integrate_field(coeffs, cloud, kind, param):
+    """Integral of the field with coefficients ``coeffs`` (operators.py:381-450)."""
+    pts, w = integrate_field_points(cloud.Nx, cloud.Ny)
+    return float(w @ value(pts, coeffs, cloud.sorted_nodes, kind, param))

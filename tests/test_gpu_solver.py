@@ -127,3 +127,32 @@ def test_eval_value_gradient_laplacian():
         assert np.abs(g[:, 0] - ref[R.DX]).max() <= 1e-12 * max(sc[R.DX], 1)
         assert np.abs(g[:, 1] - ref[R.DY]).max() <= 1e-12 * max(sc[R.DY], 1)
         assert np.abs(l - ref[R.LAP]).max() <= 1e-12 * max(sc[R.LAP], 1)
+
+
+def test_integrate_field_matches_oracle_and_reference_test():
+    """updes/tests/test_integrals.py on the GPU path: r^11, degree 3, 12 x 12,
+    x^2/(1+y^2): same integral as the oracle, and |I - pi/12| < 0.1."""
+    import functools
+    import updes_amd as u
+    ft = {"North": "d", "South": "d", "East": "d", "West": "d"}
+    cloud = u.SquareCloud(Nx=12, Ny=12, facet_types=ft)
+    rbf = functools.partial(u.polyharmonic, a=5)
+    xy = cloud.sorted_nodes.cuda()
+    s = xy[:, 0] ** 2 / (1 + xy[:, 1] ** 2)
+    c = u.get_field_coefficients(s, cloud, rbf, 3)
+    val = float(u.integrate_field(c, cloud, rbf, 3))
+    assert abs(val - np.pi / 12) < 1e-1
+    # against the oracle: with r^11 and degree 3, kappa(A) ~ 2e12 and two FP64
+    # solves differ by ~kappa eps (1e-6 in this integral), so the entrywise
+    # comparison uses the well-conditioned r^3 / degree-1 interpolant
+    oc = OC.SquareCloud(12, 12, ft)
+    c3 = u.get_field_coefficients(s, cloud, u.polyharmonic, 1)
+    val3 = float(u.integrate_field(c3, cloud, u.polyharmonic, 1))
+    c_ref = OA.core_compute_coefficients(s.cpu().numpy(), oc, R.POLYHARMONIC, 1.0, 3)
+    ref = OS.integrate_field(c_ref, oc, R.POLYHARMONIC, 1.0)
+    kappa = np.linalg.cond(OA.assemble_A(oc, R.POLYHARMONIC, 1.0, 3))
+    assert abs(val3 - ref) <= max(1e-12, 10 * kappa * np.finfo(float).eps) * abs(ref)
+    # constant field: the rule's weights exactly
+    one = float(u.integrate_field(u.get_field_coefficients(torch.ones(cloud.N, dtype=torch.float64, device="cuda"),
+                                                           cloud, rbf, 3), cloud, rbf, 3))
+    assert abs(one - (11 * 11 + 10 + 10 + 1) / 121) < 1e-9

@@ -329,3 +329,52 @@ def test_planted_periodic_pairing_is_caught(monkeypatch):
     monkeypatch.setattr(OA, "assemble_bd_Phi_P", swapped)
     with pytest.raises(AssertionError):
         test_oracle_matches_bruteforce(3)
+
+
+def _quirky_rule_closed_forms(Nx, Ny):
+    """Integral of f = 1 and f = x under integrate_field's rule (operators.py:381-450),
+    summed by hand: weights 1 / 1/2 / 1/4 times the cell area; interior points are
+    consecutive pairs of the flattened meshgrid stack, so their x-coordinates are
+    the even entries of concat(gx.ravel(), gy.ravel())."""
+    hx, hy = 1.0 / (Nx - 1), 1.0 / (Ny - 1)
+    area = hx * hy
+    ones = (Nx - 1) * (Ny - 1) + 0.5 * (2 * (Ny - 2) + 2 * (Nx - 2)) + 0.25 * 4
+    qx = [hx + i * (1 - 2 * hx) / (Nx - 2) for i in range(Nx - 1)]
+    qy = [hy + j * (1 - 2 * hy) / (Ny - 2) for j in range(Ny - 1)]
+    flat = [qx[i] for j in range(Ny - 1) for i in range(Nx - 1)] + [qy[j] for j in range(Ny - 1) for i in range(Nx - 1)]
+    inner_x = sum(flat[0::2])
+    ex = [hx + i * (1 - 2 * hx) / (Nx - 3) for i in range(Nx - 2)]
+    # left x=0, right x=1 (Ny-2 points each), bottom/top at ex; corners (0,0),(1,0),(1,0),(1,1)
+    xs = inner_x + 0.5 * ((Ny - 2) * 1.0 + 2 * sum(ex)) + 0.25 * 3.0
+    return ones * area, xs * area
+
+
+def test_integrate_field_rule_closed_forms():
+    """f = 1 (pins the weights) and f = x (pins the point layout, including the
+    top-left corner evaluated at (1, 0)) are reproduced exactly by deg >= 1."""
+    Nx, Ny = 9, 7
+    oc = OC.SquareCloud(Nx, Ny, {"North": "d", "South": "d", "East": "d", "West": "d"})
+    one, xint = _quirky_rule_closed_forms(Nx, Ny)
+    x = oc.sorted_nodes[:, 0]
+    for kind, param in [(R.POLYHARMONIC, 1.0), (R.THIN_PLATE, 1.0)]:
+        c1 = OA.core_compute_coefficients(np.ones(oc.N), oc, kind, param, 3)
+        cx = OA.core_compute_coefficients(x, oc, kind, param, 3)
+        assert abs(OS.integrate_field(c1, oc, kind, param) - one) < 1e-10
+        assert abs(OS.integrate_field(cx, oc, kind, param) - xint) < 1e-10
+    # the rule is not a plain midpoint rule: a corrected top-left corner differs
+    pts, w = OS.integrate_field_points(Nx, Ny)
+    assert np.allclose(pts[-2], [1.0, 0.0])
+
+
+def test_integral_like_reference_test():
+    """updes/tests/test_integrals.py: r^11 (a=5), max_degree 3, 12 x 12, field
+    x^2/(1+y^2); the reference asserts |I - pi/12| < 0.1 (test_integrals.py:83-84)."""
+    oc = OC.SquareCloud(12, 12, {"North": "d", "South": "d", "East": "d", "West": "d"})
+    xy = oc.sorted_nodes
+    s = xy[:, 0] ** 2 / (1 + xy[:, 1] ** 2)
+    c = OA.core_compute_coefficients(s, oc, R.POLYHARMONIC, 5.0, 10)
+    v = OS.value(xy, c, xy, R.POLYHARMONIC, 5.0)
+    # kappa(A) ~ 2e12 for r^11 with degree-3 monomials: reconstruction to ~1e-6
+    assert np.mean(np.abs(v - s)) < 1e-5
+    val = OS.integrate_field(c, oc, R.POLYHARMONIC, 5.0)
+    assert abs(val - np.pi / 12) < 1e-1

@@ -14,7 +14,8 @@
 //      pick the same winner);
 //   2. row interchanges of the other columns through a gather/scatter plan
 //      (row-major rows are contiguous, so a swap is a coalesced copy);
-//   3. U12 = L11^-1 A12 with a strip-parallel forward substitution;
+//   3. U12 = inv(L11) A12: the unit-lower diagonal block is inverted (64 x 64
+//      diagonal inverses + block recurrence) and applied with the MFMA GEMM;
 //   4. trailing update A22 -= L21 U12 with an MFMA GEMM
 //      (v_mfma_f64_16x16x4f64 / v_mfma_f32_16x16x4f32, 128x128 tiles,
 //      XCD-aware tile order).
@@ -183,18 +184,26 @@ __global__ __launch_bounds__(GTHREADS) __attribute__((amdgpu_waves_per_eu(2, 2))
 }
 
 template <typename T>
-static int gemm(int64_t m, int64_t n, int64_t k, T alpha, const T* A, int64_t lda, const T* B,
-                int64_t ldb, T beta, T* C, int64_t ldc, hipStream_t s) {
+static int gemm_launch(int64_t m, int64_t n, int64_t k, T alpha, const T* A, int64_t lda, const T* B,
+                       int64_t ldb, T beta, T* C, int64_t ldc, hipStream_t s) {
   if (m <= 0 || n <= 0) return RBFX_OK;
   const int64_t tm = ceil_div(m, GBM), tn = ceil_div(n, GBN);
   const int64_t tiles = tm * tn;
   const int64_t grid = ((tiles + 7) / 8) * 8;
-  const int pid = prof_start(sizeof(T) == 8 ? "gemm_f64" : "gemm_f32", s);
   hipLaunchKernelGGL((gemm_kernel<T>), dim3((unsigned)grid), dim3(GTHREADS), 0, s, m, n, k, alpha, A,
                      lda, B, ldb, beta, C, ldc, tm, tn);
-  prof_stop(pid, s, 2.0 * m * n * k, (double)sizeof(T) * (2.0 * m * n + (double)m * k + (double)k * n));
   RBFX_CHECK_LAUNCH();
   return RBFX_OK;
+}
+
+template <typename T>
+static int gemm(int64_t m, int64_t n, int64_t k, T alpha, const T* A, int64_t lda, const T* B,
+                int64_t ldb, T beta, T* C, int64_t ldc, hipStream_t s) {
+  if (m <= 0 || n <= 0) return RBFX_OK;
+  const int pid = prof_start(sizeof(T) == 8 ? "gemm_f64" : "gemm_f32", s);
+  const int st = gemm_launch<T>(m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, s);
+  prof_stop(pid, s, 2.0 * m * n * k, (double)sizeof(T) * (2.0 * m * n + (double)m * k + (double)k * n));
+  return st;
 }
 
 // ---------------------------------------------------------------------------
@@ -550,29 +559,134 @@ __global__ __launch_bounds__(256) void swap_scatter_kernel(T* __restrict__ A, in
 }
 
 // ---------------------------------------------------------------------------
-// U12 = L11^-1 A12 (L11 unit lower W x W), strips of 32 columns staged in LDS.
-constexpr int kTrsmCols = 32;
-constexpr int kTrsmThreads = 256;
+// U12 = inv(L11) A12 as one MFMA GEMM.  inv(L11) (unit lower, W <= kNB) is
+// built in two launches: the 64 x 64 diagonal inverses by column-per-thread
+// substitution (in FP64, rounded to T), then per block column j the
+// recurrence X_ij = -X_ii * sum_{k=j}^{i-1} L_ik X_kj (i > j), in order of i.
+constexpr int kTri = 64;
+constexpr int kNB = 256;  // outer panel width of the blocked LU
 
 template <typename T>
-__global__ __launch_bounds__(kTrsmThreads) void trsm_llnu_kernel(const T* __restrict__ L, int64_t ldl,
-                                                                 T* __restrict__ B, int64_t ldb, int W,
-                                                                 int64_t ncols) {
-  extern __shared__ __align__(16) unsigned char smem_raw[];
-  T* X = reinterpret_cast<T*>(smem_raw);  // [W][kTrsmCols]
-  const int64_t c0 = (int64_t)blockIdx.x * kTrsmCols;
-  const int col = threadIdx.x % kTrsmCols, grp = threadIdx.x / kTrsmCols;
-  constexpr int NG = kTrsmThreads / kTrsmCols;
-  const bool valid = c0 + col < ncols;
-  for (int i = grp; i < W; i += NG) X[i * kTrsmCols + col] = valid ? B[i * ldb + c0 + col] : T(0);
+__global__ __launch_bounds__(kTri) void trtri_diag_kernel(const T* __restrict__ L, int64_t ldl, int W,
+                                                         T* __restrict__ X, int64_t ldx) {
+  __shared__ double S[kTri][kTri + 1];
+  __shared__ double Y[kTri][kTri + 1];
+  const int o = blockIdx.x * kTri;
+  const int sz = min(kTri, W - o);
+  const int c = threadIdx.x;
+  for (int i = 0; i < kTri; ++i) S[i][c] = (i < sz && c < i) ? (double)L[(int64_t)(o + i) * ldl + o + c] : 0.0;
   __syncthreads();
-  for (int kk = 0; kk < W - 1; ++kk) {
-    const T xk = X[kk * kTrsmCols + col];
-    for (int i = kk + 1 + grp; i < W; i += NG) X[i * kTrsmCols + col] -= L[(int64_t)i * ldl + kk] * xk;
+  // column c of the inverse: y_i = -sum_{c<=k<i} L_ik y_k, y_c = 1
+  for (int i = 0; i < kTri; ++i) Y[i][c] = (i == c) ? 1.0 : 0.0;
+  for (int i = c + 1; i < sz; ++i) {
+    double acc = 0.0;
+    for (int k = c; k < i; ++k) acc += S[i][k] * Y[k][c];
+    Y[i][c] = -acc;
+  }
+  // block row: the diagonal inverse and zeros to its right (W columns in all)
+  if (c < sz)
+    for (int i = 0; i < sz; ++i) X[(int64_t)(o + i) * ldx + o + c] = (T)Y[i][c];
+  for (int i = 0; i < sz; ++i)
+    for (int j = o + kTri + c; j < W; j += kTri) X[(int64_t)(o + i) * ldx + j] = T(0);
+}
+
+// One workgroup per block column j < nbk-1; 256 threads, each owning a 4 x 4
+// piece of the 64 x 64 block; products staged through LDS in k-chunks of 32.
+template <typename T>
+__global__ __launch_bounds__(256) void trtri_offdiag_kernel(const T* __restrict__ L, int64_t ldl, int W,
+                                                           T* __restrict__ X, int64_t ldx) {
+  __shared__ double As[kTri][33];
+  __shared__ double Bs[32][kTri + 1];
+  const int nbk = (W + kTri - 1) / kTri;
+  const int j = blockIdx.x;
+  const int oj = j * kTri;
+  const int t = threadIdx.x;
+  const int r0 = (t >> 4) * 4, c0 = (t & 15) * 4;
+  for (int i = j + 1; i < nbk; ++i) {
+    const int oi = i * kTri;
+    const int szi = min(kTri, W - oi);
+    double acc[4][4];
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+      for (int b = 0; b < 4; ++b) acc[a][b] = 0.0;
+    // S = sum_k L_ik X_kj
+    for (int k = j; k < i; ++k) {
+      const int ok = k * kTri;
+      const int szk = min(kTri, W - ok);
+      for (int kc = 0; kc < kTri; kc += 32) {
+        for (int e = t; e < kTri * 32; e += 256) {
+          const int r = e >> 5, q = e & 31;
+          As[r][q] = (r < szi && kc + q < szk) ? (double)L[(int64_t)(oi + r) * ldl + ok + kc + q] : 0.0;
+        }
+        for (int e = t; e < 32 * kTri; e += 256) {
+          const int q = e >> 6, cc = e & 63;
+          Bs[q][cc] = (kc + q < szk) ? (double)X[(int64_t)(ok + kc + q) * ldx + oj + cc] : 0.0;
+        }
+        __syncthreads();
+#pragma unroll 4
+        for (int q = 0; q < 32; ++q) {
+          double a[4], b[4];
+#pragma unroll
+          for (int u = 0; u < 4; ++u) a[u] = As[r0 + u][q];
+#pragma unroll
+          for (int u = 0; u < 4; ++u) b[u] = Bs[q][c0 + u];
+#pragma unroll
+          for (int u = 0; u < 4; ++u)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) acc[u][v] += a[u] * b[v];
+        }
+        __syncthreads();
+      }
+    }
+    // X_ij = -X_ii S
+    double out[4][4];
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+      for (int b = 0; b < 4; ++b) out[a][b] = 0.0;
+    for (int kc = 0; kc < kTri; kc += 32) {
+      for (int e = t; e < kTri * 32; e += 256) {
+        const int r = e >> 5, q = e & 31;
+        As[r][q] = (r < szi && kc + q < szi) ? (double)X[(int64_t)(oi + r) * ldx + oi + kc + q] : 0.0;
+      }
+      if (r0 >= kc && r0 < kc + 32)
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+#pragma unroll
+          for (int v = 0; v < 4; ++v) Bs[r0 - kc + u][c0 + v] = acc[u][v];
+      __syncthreads();
+#pragma unroll 4
+      for (int q = 0; q < 32; ++q) {
+        double a[4], b[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) a[u] = As[r0 + u][q];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) b[u] = Bs[q][c0 + u];
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+#pragma unroll
+          for (int v = 0; v < 4; ++v) out[u][v] += a[u] * b[v];
+      }
+      __syncthreads();
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u)
+      if (r0 + u < szi)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) X[(int64_t)(oi + r0 + u) * ldx + oj + c0 + v] = (T)(-out[u][v]);
+    // the block just written is read back (as X_kj) for the next i
     __syncthreads();
   }
-  if (valid)
-    for (int i = grp; i < W; i += NG) B[i * ldb + c0 + col] = X[i * kTrsmCols + col];
+}
+
+// dst (rows x cols, ld ldd) = src (ld lds)
+template <typename T>
+__global__ __launch_bounds__(256) void copy_block_kernel(const T* __restrict__ src, int64_t lds, int rows,
+                                                        int64_t cols, T* __restrict__ dst, int64_t ldd) {
+  const int r = blockIdx.y;
+  for (int64_t c = (int64_t)blockIdx.x * 256 + threadIdx.x; c < cols; c += (int64_t)gridDim.x * 256)
+    dst[(int64_t)r * ldd + c] = src[(int64_t)r * lds + c];
 }
 
 __global__ void zero_u32_kernel(unsigned* p, int n) {
@@ -606,6 +720,7 @@ struct LuCtx {
   int* plan_count;
   T* swap_tmp;
   int64_t swap_tmp_elems;
+  T* tri_inv;  // [kNB][kNB] inverse of the current unit-lower diagonal block
   int G_max;
   int wmax;
   size_t lds_budget;
@@ -697,14 +812,29 @@ static int apply_swaps(LuCtx<T>& c, int64_t s, int64_t e, int64_t c0, int64_t nc
 template <typename T>
 static int trsm(LuCtx<T>& c, int64_t r0, int W, int64_t col, int64_t ncols) {
   if (ncols <= 0) return RBFX_OK;
-  const size_t smem = (size_t)W * kTrsmCols * sizeof(T);
-  RBFX_REQUIRE(smem <= c.lds_budget, "trsm strip does not fit LDS");
+  RBFX_REQUIRE(W <= kNB, "trsm block wider than the panel");
   const int pid = prof_start(sizeof(T) == 8 ? "trsm_f64" : "trsm_f32", c.s);
-  hipLaunchKernelGGL((trsm_llnu_kernel<T>), dim3((unsigned)ceil_div(ncols, kTrsmCols)),
-                     dim3(kTrsmThreads), smem, c.s, c.A + r0 * c.lda + r0, c.lda,
-                     c.A + r0 * c.lda + col, c.lda, W, ncols);
-  prof_stop(pid, c.s, (double)W * W * ncols, 2.0 * sizeof(T) * W * ncols);
+  const T* L = c.A + r0 * c.lda + r0;
+  T* B = c.A + r0 * c.lda + col;
+  const int nbk = (int)ceil_div(W, kTri);
+  hipLaunchKernelGGL((trtri_diag_kernel<T>), dim3(nbk), dim3(kTri), 0, c.s, L, c.lda, W, c.tri_inv, (int64_t)kNB);
   RBFX_CHECK_LAUNCH();
+  if (nbk > 1) {
+    hipLaunchKernelGGL((trtri_offdiag_kernel<T>), dim3(nbk - 1), dim3(256), 0, c.s, L, c.lda, W, c.tri_inv,
+                       (int64_t)kNB);
+    RBFX_CHECK_LAUNCH();
+  }
+  // B is both an operand and the result: stage it, then B = inv(L11) * staged
+  const int64_t slice = std::max<int64_t>((int64_t)GBN, c.swap_tmp_elems / W);
+  int st;
+  for (int64_t off = 0; off < ncols; off += slice) {
+    const int64_t w = std::min<int64_t>(slice, ncols - off);
+    hipLaunchKernelGGL((copy_block_kernel<T>), dim3((unsigned)std::min<int64_t>(ceil_div(w, 256), 64), W),
+                       dim3(256), 0, c.s, B + off, c.lda, W, w, c.swap_tmp, w);
+    RBFX_CHECK_LAUNCH();
+    if ((st = gemm_launch<T>(W, w, W, T(1), c.tri_inv, kNB, c.swap_tmp, w, T(0), B + off, c.lda, c.s))) return st;
+  }
+  prof_stop(pid, c.s, 2.0 * W * W * ncols, 3.0 * sizeof(T) * W * ncols);
   return RBFX_OK;
 }
 
@@ -728,8 +858,6 @@ static int panel(LuCtx<T>& c, int64_t r0, int W) {
   return apply_swaps(c, r0 + W1, r0 + W, r0, W1);
 }
 
-constexpr int kNB = 256;
-
 static size_t getrf_ws_bytes(int64_t n, int bytes) {
   const int G = 1024;  // upper bound on workgroups of a leaf
   size_t b = 0;
@@ -739,8 +867,10 @@ static size_t getrf_ws_bytes(int64_t n, int bytes) {
   b += ws_round((size_t)G * 4);                   // panel step flags
   b += ws_round((size_t)2 * kSwapMax * 8) * 2;    // plan dst / src
   b += ws_round(64);                              // plan count
-  // swap staging: 2*NB rows x min(n, 8192) columns
-  b += ws_round((size_t)2 * kNB * std::min<int64_t>(n, 8192) * bytes);
+  // staging of row swaps (2*NB rows x min(n, 8192) columns) and of the
+  // triangular solve's right-hand side (NB x n)
+  b += ws_round((size_t)std::max<int64_t>(2 * kNB * std::min<int64_t>(n, 8192), (int64_t)kNB * n) * bytes);
+  b += ws_round((size_t)kNB * kNB * bytes);  // inverse of the diagonal block
   return b;
 }
 
@@ -769,9 +899,10 @@ static int getrf(int64_t n, T* A, int64_t lda, int64_t* ipiv, void* work, size_t
   c.plan_dst = ws.take<int64_t>(2 * kSwapMax);
   c.plan_src = ws.take<int64_t>(2 * kSwapMax);
   c.plan_count = ws.take<int>(16);
-  c.swap_tmp_elems = (int64_t)2 * kNB * std::min<int64_t>(n, 8192);
+  c.swap_tmp_elems = std::max<int64_t>(2 * kNB * std::min<int64_t>(n, 8192), (int64_t)kNB * n);
   c.swap_tmp = ws.take<T>(c.swap_tmp_elems);
-  RBFX_REQUIRE(c.swap_tmp != nullptr, "workspace carve failed");
+  c.tri_inv = ws.take<T>((size_t)kNB * kNB);
+  RBFX_REQUIRE(c.swap_tmp != nullptr && c.tri_inv != nullptr, "workspace carve failed");
   c.seq = 0;
   c.G_max = G;
   c.lds_budget = std::min<size_t>(di.lds_per_block, 65536);
