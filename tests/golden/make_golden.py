@@ -59,8 +59,44 @@ def eval_fixture():
     np.savez_compressed(os.path.join(HERE, "eval_ops.npz"), **out)
 
 
+def poisson_100():
+    """Poisson 100 x 100, interior jitter U(-0.2h, 0.2h) (seed 0), r^3, degree 2,
+    all Dirichlet from u = sin(pi x) sin(pi y); FP64 augmented-system solve."""
+    nx = 100
+    ft = {"North": "d", "South": "d", "East": "d", "West": "d"}
+    oc = OC.SquareCloud(nx, nx, ft, jitter=C.jitter(nx, nx, 0.2, 0))
+    x, y = oc.sorted_nodes.T
+    ex = np.sin(np.pi * x) * np.sin(np.pi * y)
+    q = np.where(np.arange(oc.N) < oc.Ni, -2 * np.pi ** 2 * ex, ex)
+    u, c, _ = OS.pde_solver_kpath(C.LAPLACIAN, oc, R.POLYHARMONIC, 1.0, 6, q)
+    np.savez_compressed(os.path.join(HERE, "poisson_100.npz"), nodes=oc.sorted_nodes, q=q, vals=u, exact=ex)
+
+
+def advdiff_80(steps=5):
+    """Advection-diffusion 80 x 80, periodic West/East, Dirichlet 0 North/South,
+    gaussian eps=5, degree 1, dt=1e-3, nu=0.01, v=(1, 0.5); rhs of each implicit
+    Euler step = value(x_i, coeffs(u^n)) (= u^n + lambda for the gaussian)."""
+    nx = 80
+    ft = {"North": "d", "South": "d", "West": "p1", "East": "p1"}
+    oc = OC.SquareCloud(nx, nx, ft)
+    xy = oc.sorted_nodes
+    un = np.exp(-((xy[:, 0] - 0.3) ** 2 + (xy[:, 1] - 0.5) ** 2) / (2 * 0.1 ** 2))
+    out = {"nodes": xy, "u0": un.copy()}
+    desc = C.adv_diff_desc()
+    bc = {f: np.zeros(len(oc.facet_nodes[f])) for f in ft}
+    for s in range(steps):
+        cn = OS.field_coefficients(un, oc, R.GAUSSIAN, 5.0, 3)
+        qi = OS.value(xy[:oc.Ni], cn, xy, R.GAUSSIAN, 5.0)
+        q = OA.assemble_q(oc, qi, bc)
+        un, _, _ = OS.pde_solver_kpath(desc, oc, R.GAUSSIAN, 5.0, 3, q)
+        out["u%d" % (s + 1)] = un.copy()
+    np.savez_compressed(os.path.join(HERE, "advdiff_80.npz"), **out)
+
+
 if __name__ == "__main__":
     readme_laplace()
     small_cases()
     eval_fixture()
+    poisson_100()
+    advdiff_80()
     print("written to", HERE)

@@ -20,6 +20,7 @@
 //      (v_mfma_f64_16x16x4f64 / v_mfma_f32_16x16x4f32, 128x128 tiles,
 //      XCD-aware tile order).
 #include <algorithm>
+#include <mutex>
 #include <type_traits>
 #include <vector>
 
@@ -39,7 +40,7 @@ struct Mma;
 template <>
 struct Mma<double> {
   typedef v4d acc_t;
-  static constexpr int BK = 16;
+  static constexpr int BK = 8;
   __device__ static inline acc_t mma(double a, double b, acc_t c) {
     return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
   }
@@ -48,7 +49,7 @@ struct Mma<double> {
 template <>
 struct Mma<float> {
   typedef v4f acc_t;
-  static constexpr int BK = 32;
+  static constexpr int BK = 16;
   __device__ static inline acc_t mma(float a, float b, acc_t c) {
     return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
   }
@@ -61,8 +62,12 @@ struct Mma<float> {
 constexpr int GBM = 128, GBN = 128, GTHREADS = 256;
 
 // Two waves per SIMD (<= 256 registers each) so one wave's barrier or load
-// wait overlaps the other's MFMAs.
-template <typename T>
+// wait overlaps the other's MFMAs.  The k-tile (BK) keeps the LDS of one
+// workgroup at 32 KB, so two GEMM workgroups leave 96 KB of a CU's LDS to
+// the panel kernels that run beside the trailing update (look-ahead).
+// USE only separates the kernel symbols of the trailing update (0) and of
+// the triangular solve (1) in profiles; the code is the same.
+template <typename T, int USE>
 __global__ __launch_bounds__(GTHREADS) __attribute__((amdgpu_waves_per_eu(2, 2))) void gemm_kernel(
                                                         int64_t m, int64_t n, int64_t k, T alpha,
                                                         const T* __restrict__ A, int64_t lda,
@@ -76,14 +81,22 @@ __global__ __launch_bounds__(GTHREADS) __attribute__((amdgpu_waves_per_eu(2, 2))
   __shared__ T As[2][BK][GBM];
   __shared__ T Bs[2][BK][GBN];
 
-  // XCD-aware order: consecutive workgroups land on different XCDs (round
-  // robin), so give each XCD a contiguous band of tile rows.
+  // Tile order.  Tiles are numbered in groups of 8 tile rows, column by
+  // column inside a group, so the ~64 workgroups resident on one XCD at a
+  // time cover an 8 x 8 patch and share its A and B tiles in that XCD's L2.
+  // Consecutive workgroup ids land on different XCDs (round robin), so XCD x
+  // takes the x-th contiguous eighth of that numbering.
   const int64_t T_ = tiles_m * tiles_n;
   const int64_t L = blockIdx.x;
   const int64_t per = (T_ + 7) / 8;
   const int64_t tile = (L % 8) * per + L / 8;
   if (tile >= T_) return;
-  const int64_t tm = tile / tiles_n, tn = tile % tiles_n;
+  constexpr int64_t kGroupM = 8;
+  const int64_t gwidth = kGroupM * tiles_n;
+  const int64_t first_m = (tile / gwidth) * kGroupM;
+  const int64_t gm = tiles_m - first_m < kGroupM ? tiles_m - first_m : kGroupM;
+  const int64_t tm = first_m + (tile % gwidth) % gm;
+  const int64_t tn = (tile % gwidth) / gm;
   const int64_t row0 = tm * GBM, col0 = tn * GBN;
 
   const int tid = threadIdx.x;
@@ -183,14 +196,14 @@ __global__ __launch_bounds__(GTHREADS) __attribute__((amdgpu_waves_per_eu(2, 2))
   }
 }
 
-template <typename T>
+template <typename T, int USE = 0>
 static int gemm_launch(int64_t m, int64_t n, int64_t k, T alpha, const T* A, int64_t lda, const T* B,
                        int64_t ldb, T beta, T* C, int64_t ldc, hipStream_t s) {
   if (m <= 0 || n <= 0) return RBFX_OK;
   const int64_t tm = ceil_div(m, GBM), tn = ceil_div(n, GBN);
   const int64_t tiles = tm * tn;
   const int64_t grid = ((tiles + 7) / 8) * 8;
-  hipLaunchKernelGGL((gemm_kernel<T>), dim3((unsigned)grid), dim3(GTHREADS), 0, s, m, n, k, alpha, A,
+  hipLaunchKernelGGL((gemm_kernel<T, USE>), dim3((unsigned)grid), dim3(GTHREADS), 0, s, m, n, k, alpha, A,
                      lda, B, ldb, beta, C, ldc, tm, tn);
   RBFX_CHECK_LAUNCH();
   return RBFX_OK;
@@ -474,14 +487,18 @@ __global__ __launch_bounds__(64) void swap_plan_kernel(const int64_t* __restrict
   __shared__ int64_t in_src[kSwapMax];
   __shared__ int64_t out_pos[kSwapMax];
   __shared__ int64_t out_src[kSwapMax];
+  __shared__ int64_t piv[kSwapMax];
   __shared__ int nout;
   const int lane = threadIdx.x;
   const int k = (int)(e - s);
-  for (int t = lane; t < k; t += 64) in_src[t] = s + t;
+  for (int t = lane; t < k; t += 64) {
+    in_src[t] = s + t;
+    piv[t] = ipiv[s + t];
+  }
   if (lane == 0) nout = 0;
   __syncthreads();
   for (int i = 0; i < k; ++i) {
-    const int64_t p = ipiv[s + i];
+    const int64_t p = piv[i];
     if (p < e) {
       if (lane == 0 && p != s + i) {
         const int64_t tmp = in_src[i];
@@ -560,121 +577,127 @@ __global__ __launch_bounds__(256) void swap_scatter_kernel(T* __restrict__ A, in
 
 // ---------------------------------------------------------------------------
 // U12 = inv(L11) A12 as one MFMA GEMM.  inv(L11) (unit lower, W <= kNB) is
-// built in two launches: the 64 x 64 diagonal inverses by column-per-thread
-// substitution (in FP64, rounded to T), then per block column j the
-// recurrence X_ij = -X_ii * sum_{k=j}^{i-1} L_ik X_kj (i > j), in order of i.
+// built in two launches (FP64 arithmetic, rounded to T): the 64 x 64
+// diagonal inverses (rows of each column held in registers by 4 waves), then
+// the blocks below the diagonal from X_ij = -X_ii sum_{k=j}^{i-1} L_ik X_kj.
 constexpr int kTri = 64;
 constexpr int kNB = 256;  // outer panel width of the blocked LU
 
 template <typename T>
-__global__ __launch_bounds__(kTri) void trtri_diag_kernel(const T* __restrict__ L, int64_t ldl, int W,
-                                                         T* __restrict__ X, int64_t ldx) {
-  __shared__ double S[kTri][kTri + 1];
-  __shared__ double Y[kTri][kTri + 1];
+__global__ __launch_bounds__(256) void trtri_diag_kernel(const T* __restrict__ L, int64_t ldl, int W,
+                                                        T* __restrict__ X, int64_t ldx) {
+  // strictly lower part of the block, packed by rows: L_ik at i(i-1)/2 + k
+  __shared__ double S[kTri * (kTri - 1) / 2];
+  __shared__ double Yk[2][kTri];  // row k of the inverse, double-buffered
   const int o = blockIdx.x * kTri;
   const int sz = min(kTri, W - o);
-  const int c = threadIdx.x;
-  for (int i = 0; i < kTri; ++i) S[i][c] = (i < sz && c < i) ? (double)L[(int64_t)(o + i) * ldl + o + c] : 0.0;
+  const int t = threadIdx.x;
+  const int c = t & (kTri - 1), g = t / kTri;  // column c, rows i = 4q + g
+  for (int i = 1; i < kTri; ++i)
+    for (int k = t; k < i; k += 256)
+      S[i * (i - 1) / 2 + k] = i < sz ? (double)L[(int64_t)(o + i) * ldl + o + k] : 0.0;
+  double y[kTri / 4];
+#pragma unroll
+  for (int q = 0; q < kTri / 4; ++q) y[q] = (4 * q + g == c) ? 1.0 : 0.0;
   __syncthreads();
-  // column c of the inverse: y_i = -sum_{c<=k<i} L_ik y_k, y_c = 1
-  for (int i = 0; i < kTri; ++i) Y[i][c] = (i == c) ? 1.0 : 0.0;
-  for (int i = c + 1; i < sz; ++i) {
-    double acc = 0.0;
-    for (int k = c; k < i; ++k) acc += S[i][k] * Y[k][c];
-    Y[i][c] = -acc;
+  // L Y = I by rows: once row k is final, every row i > k drops L_ik Y_k
+  for (int k = 0; k + 1 < sz; ++k) {
+    if (g == (k & 3)) {
+      double v = 0.0;
+#pragma unroll
+      for (int q = 0; q < kTri / 4; ++q)
+        if (q == (k >> 2)) v = y[q];
+      Yk[k & 1][c] = v;
+    }
+    __syncthreads();
+    const double yk = Yk[k & 1][c];
+#pragma unroll
+    for (int q = 0; q < kTri / 4; ++q) {
+      const int i = 4 * q + g;
+      if (i > k && i < sz) y[q] -= S[i * (i - 1) / 2 + k] * yk;
+    }
   }
   // block row: the diagonal inverse and zeros to its right (W columns in all)
-  if (c < sz)
-    for (int i = 0; i < sz; ++i) X[(int64_t)(o + i) * ldx + o + c] = (T)Y[i][c];
+  if (c < sz) {
+#pragma unroll
+    for (int q = 0; q < kTri / 4; ++q)
+      if (4 * q + g < sz) X[(int64_t)(o + 4 * q + g) * ldx + o + c] = (T)y[q];
+  }
   for (int i = 0; i < sz; ++i)
-    for (int j = o + kTri + c; j < W; j += kTri) X[(int64_t)(o + i) * ldx + j] = T(0);
+    for (int j = o + kTri + t; j < W; j += 256) X[(int64_t)(o + i) * ldx + j] = T(0);
 }
 
 // One workgroup per block column j < nbk-1; 256 threads, each owning a 4 x 4
-// piece of the 64 x 64 block; products staged through LDS in k-chunks of 32.
+// piece of the 64 x 64 block; X_ij for i = j+1, ... in order, products staged
+// through LDS in k-chunks of 16 (17 KB of LDS, so the kernel fits beside the
+// trailing-update GEMM).
+constexpr int kTriK = 16;
+
 template <typename T>
 __global__ __launch_bounds__(256) void trtri_offdiag_kernel(const T* __restrict__ L, int64_t ldl, int W,
                                                            T* __restrict__ X, int64_t ldx) {
-  __shared__ double As[kTri][33];
-  __shared__ double Bs[32][kTri + 1];
+  __shared__ double As[kTri][kTriK + 1];
+  __shared__ double Bs[kTriK][kTri + 1];
   const int nbk = (W + kTri - 1) / kTri;
   const int j = blockIdx.x;
   const int oj = j * kTri;
   const int t = threadIdx.x;
   const int r0 = (t >> 4) * 4, c0 = (t & 15) * 4;
-  for (int i = j + 1; i < nbk; ++i) {
-    const int oi = i * kTri;
-    const int szi = min(kTri, W - oi);
-    double acc[4][4];
-#pragma unroll
-    for (int a = 0; a < 4; ++a)
-#pragma unroll
-      for (int b = 0; b < 4; ++b) acc[a][b] = 0.0;
-    // S = sum_k L_ik X_kj
-    for (int k = j; k < i; ++k) {
-      const int ok = k * kTri;
-      const int szk = min(kTri, W - ok);
-      for (int kc = 0; kc < kTri; kc += 32) {
-        for (int e = t; e < kTri * 32; e += 256) {
-          const int r = e >> 5, q = e & 31;
-          As[r][q] = (r < szi && kc + q < szk) ? (double)L[(int64_t)(oi + r) * ldl + ok + kc + q] : 0.0;
-        }
-        for (int e = t; e < 32 * kTri; e += 256) {
-          const int q = e >> 6, cc = e & 63;
-          Bs[q][cc] = (kc + q < szk) ? (double)X[(int64_t)(ok + kc + q) * ldx + oj + cc] : 0.0;
-        }
-        __syncthreads();
-#pragma unroll 4
-        for (int q = 0; q < 32; ++q) {
-          double a[4], b[4];
-#pragma unroll
-          for (int u = 0; u < 4; ++u) a[u] = As[r0 + u][q];
-#pragma unroll
-          for (int u = 0; u < 4; ++u) b[u] = Bs[q][c0 + u];
+  // acc += A(rows ai.., cols ak..) * B(rows bk.., cols oj..), both 64 x 64
+  auto mult = [&](const T* Ab, int64_t lda_, int arows, int acols, const T* Bb, int64_t ldb_, int brows,
+                  bool b_from_acc, double (&acc)[4][4], const double (&src)[4][4]) {
+    for (int kc = 0; kc < kTri; kc += kTriK) {
+      for (int e = t; e < kTri * kTriK; e += 256) {
+        const int r = e / kTriK, q = e % kTriK;
+        As[r][q] = (r < arows && kc + q < acols) ? (double)Ab[(int64_t)r * lda_ + kc + q] : 0.0;
+      }
+      if (b_from_acc) {
+        if (r0 >= kc && r0 < kc + kTriK)
 #pragma unroll
           for (int u = 0; u < 4; ++u)
 #pragma unroll
-            for (int v = 0; v < 4; ++v) acc[u][v] += a[u] * b[v];
+            for (int v = 0; v < 4; ++v) Bs[r0 - kc + u][c0 + v] = src[u][v];
+      } else {
+        for (int e = t; e < kTriK * kTri; e += 256) {
+          const int q = e / kTri, cc = e % kTri;
+          Bs[q][cc] = (kc + q < brows) ? (double)Bb[(int64_t)(kc + q) * ldb_ + cc] : 0.0;
         }
-        __syncthreads();
       }
-    }
-    // X_ij = -X_ii S
-    double out[4][4];
-#pragma unroll
-    for (int a = 0; a < 4; ++a)
-#pragma unroll
-      for (int b = 0; b < 4; ++b) out[a][b] = 0.0;
-    for (int kc = 0; kc < kTri; kc += 32) {
-      for (int e = t; e < kTri * 32; e += 256) {
-        const int r = e >> 5, q = e & 31;
-        As[r][q] = (r < szi && kc + q < szi) ? (double)X[(int64_t)(oi + r) * ldx + oi + kc + q] : 0.0;
-      }
-      if (r0 >= kc && r0 < kc + 32)
-#pragma unroll
-        for (int u = 0; u < 4; ++u)
-#pragma unroll
-          for (int v = 0; v < 4; ++v) Bs[r0 - kc + u][c0 + v] = acc[u][v];
       __syncthreads();
 #pragma unroll 4
-      for (int q = 0; q < 32; ++q) {
+      for (int q = 0; q < kTriK; ++q) {
         double a[4], b[4];
 #pragma unroll
         for (int u = 0; u < 4; ++u) a[u] = As[r0 + u][q];
 #pragma unroll
-        for (int u = 0; u < 4; ++u) b[u] = Bs[q][c0 + u];
+        for (int v = 0; v < 4; ++v) b[v] = Bs[q][c0 + v];
 #pragma unroll
         for (int u = 0; u < 4; ++u)
 #pragma unroll
-          for (int v = 0; v < 4; ++v) out[u][v] += a[u] * b[v];
+          for (int v = 0; v < 4; ++v) acc[u][v] += a[u] * b[v];
       }
       __syncthreads();
     }
+  };
+  for (int i = j + 1; i < nbk; ++i) {
+    const int oi = i * kTri;
+    const int szi = min(kTri, W - oi);
+    double s_[4][4], x_[4][4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) s_[u][v] = x_[u][v] = 0.0;
+    // S = sum_{k=j}^{i-1} L_ik X_kj   (blocks k < i are full)
+    for (int k = j; k < i; ++k)
+      mult(L + (int64_t)oi * ldl + k * kTri, ldl, szi, kTri, X + (int64_t)(k * kTri) * ldx + oj, ldx, kTri,
+           false, s_, s_);
+    // X_ij = -X_ii S
+    mult(X + (int64_t)oi * ldx + oi, ldx, szi, szi, nullptr, 0, 0, true, x_, s_);
 #pragma unroll
     for (int u = 0; u < 4; ++u)
       if (r0 + u < szi)
 #pragma unroll
-        for (int v = 0; v < 4; ++v) X[(int64_t)(oi + r0 + u) * ldx + oj + c0 + v] = (T)(-out[u][v]);
+        for (int v = 0; v < 4; ++v) X[(int64_t)(oi + r0 + u) * ldx + oj + c0 + v] = (T)(-x_[u][v]);
     // the block just written is read back (as X_kj) for the next i
     __syncthreads();
   }
@@ -702,6 +725,21 @@ __global__ void status_to_info_kernel(const int* status, int32_t* info) {
 
 // ---------------------------------------------------------------------------
 // Driver.
+// Scratch of one stream of the factorisation (row-swap plan and staging,
+// inverse of the diagonal block).
+template <typename T>
+struct LuScratch {
+  int64_t* plan_dst;
+  int64_t* plan_src;
+  int* plan_count;
+  int64_t plan_s = -1, plan_e = -1;  // swap range whose plan is in plan_*
+  T* swap_tmp;
+  int64_t swap_tmp_elems;
+  T* tri_inv;  // [kNB][kNB]
+  int64_t inv_r0 = -1;  // diagonal block whose inverse is in tri_inv
+  int inv_W = 0;
+};
+
 template <typename T>
 struct LuCtx {
   T* A;
@@ -715,12 +753,7 @@ struct LuCtx {
   unsigned* flags;  // [G_max] step flags of the panel launches
   int* status;
   unsigned seq;     // steps taken by the panel launches so far
-  int64_t* plan_dst;
-  int64_t* plan_src;
-  int* plan_count;
-  T* swap_tmp;
-  int64_t swap_tmp_elems;
-  T* tri_inv;  // [kNB][kNB] inverse of the current unit-lower diagonal block
+  LuScratch<T> sc;
   int G_max;
   int wmax;
   size_t lds_budget;
@@ -783,56 +816,83 @@ static int leaf(LuCtx<T>& c, int64_t r0, int w) {
   return RBFX_OK;
 }
 
+// Net row-interchange plan of the swaps ipiv[s..e) into dst's plan buffers,
+// enqueued on c's stream.
+template <typename T>
+static int make_plan(LuCtx<T>& c, LuScratch<T>& dst, int64_t s, int64_t e) {
+  RBFX_REQUIRE(e - s <= kSwapMax, "too many swaps in one plan");
+  hipLaunchKernelGGL(swap_plan_kernel, dim3(1), dim3(64), 0, c.s, c.ipiv, s, e, dst.plan_dst, dst.plan_src,
+                     dst.plan_count);
+  RBFX_CHECK_LAUNCH();
+  dst.plan_s = s;
+  dst.plan_e = e;
+  return RBFX_OK;
+}
+
+// inv(L11) of the unit-lower block at (r0, r0), W wide, into dst.tri_inv,
+// enqueued on c's stream.
+template <typename T>
+static int make_inverse(LuCtx<T>& c, LuScratch<T>& dst, int64_t r0, int W) {
+  RBFX_REQUIRE(W <= kNB, "diagonal block wider than the panel");
+  const int pid = prof_start(sizeof(T) == 8 ? "trtri_f64" : "trtri_f32", c.s);
+  const T* L = c.A + r0 * c.lda + r0;
+  const int nbk = (int)ceil_div(W, kTri);
+  hipLaunchKernelGGL((trtri_diag_kernel<T>), dim3(nbk), dim3(256), 0, c.s, L, c.lda, W, dst.tri_inv,
+                     (int64_t)kNB);
+  RBFX_CHECK_LAUNCH();
+  if (nbk > 1) {
+    hipLaunchKernelGGL((trtri_offdiag_kernel<T>), dim3(nbk - 1), dim3(256), 0, c.s, L, c.lda, W,
+                       dst.tri_inv, (int64_t)kNB);
+    RBFX_CHECK_LAUNCH();
+  }
+  prof_stop(pid, c.s, (double)W * W * W / 3.0, (double)sizeof(T) * W * W);
+  dst.inv_r0 = r0;
+  dst.inv_W = W;
+  return RBFX_OK;
+}
+
 // Apply the swaps ipiv[s..e) to columns [c0, c0+nc).
 template <typename T>
 static int apply_swaps(LuCtx<T>& c, int64_t s, int64_t e, int64_t c0, int64_t nc) {
   if (nc <= 0 || e <= s) return RBFX_OK;
-  RBFX_REQUIRE(e - s <= kSwapMax, "too many swaps in one plan");
-  hipLaunchKernelGGL(swap_plan_kernel, dim3(1), dim3(64), 0, c.s, c.ipiv, s, e, c.plan_dst,
-                     c.plan_src, c.plan_count);
-  RBFX_CHECK_LAUNCH();
+  int st;
+  if ((c.sc.plan_s != s || c.sc.plan_e != e) && (st = make_plan(c, c.sc, s, e))) return st;
   const int maxcnt = (int)(2 * (e - s));
   const int pid = prof_start("row_swaps", c.s);
   // process the columns in slices that fit the staging buffer
-  const int64_t slice = std::max<int64_t>(256, c.swap_tmp_elems / maxcnt);
+  const int64_t slice = std::max<int64_t>(256, c.sc.swap_tmp_elems / maxcnt);
   for (int64_t off = 0; off < nc; off += slice) {
     const int64_t w = std::min<int64_t>(slice, nc - off);
     dim3 grid((unsigned)std::min<int64_t>(ceil_div(w, 256), 64), (unsigned)maxcnt);
     hipLaunchKernelGGL((swap_gather_kernel<T>), grid, dim3(256), 0, c.s, c.A, c.lda, c0 + off, w,
-                       c.plan_src, c.plan_count, c.swap_tmp);
+                       c.sc.plan_src, c.sc.plan_count, c.sc.swap_tmp);
     RBFX_CHECK_LAUNCH();
     hipLaunchKernelGGL((swap_scatter_kernel<T>), grid, dim3(256), 0, c.s, c.A, c.lda, c0 + off, w,
-                       c.plan_dst, c.plan_count, c.swap_tmp);
+                       c.sc.plan_dst, c.sc.plan_count, c.sc.swap_tmp);
     RBFX_CHECK_LAUNCH();
   }
   prof_stop(pid, c.s, 0.0, 4.0 * sizeof(T) * (e - s) * nc);
   return RBFX_OK;
 }
 
+// U12 = inv(L11) A12 for the W x W diagonal block at r0 and columns
+// [col, col+ncols): staged copy of A12, then one GEMM.
 template <typename T>
 static int trsm(LuCtx<T>& c, int64_t r0, int W, int64_t col, int64_t ncols) {
   if (ncols <= 0) return RBFX_OK;
-  RBFX_REQUIRE(W <= kNB, "trsm block wider than the panel");
-  const int pid = prof_start(sizeof(T) == 8 ? "trsm_f64" : "trsm_f32", c.s);
-  const T* L = c.A + r0 * c.lda + r0;
-  T* B = c.A + r0 * c.lda + col;
-  const int nbk = (int)ceil_div(W, kTri);
-  hipLaunchKernelGGL((trtri_diag_kernel<T>), dim3(nbk), dim3(kTri), 0, c.s, L, c.lda, W, c.tri_inv, (int64_t)kNB);
-  RBFX_CHECK_LAUNCH();
-  if (nbk > 1) {
-    hipLaunchKernelGGL((trtri_offdiag_kernel<T>), dim3(nbk - 1), dim3(256), 0, c.s, L, c.lda, W, c.tri_inv,
-                       (int64_t)kNB);
-    RBFX_CHECK_LAUNCH();
-  }
-  // B is both an operand and the result: stage it, then B = inv(L11) * staged
-  const int64_t slice = std::max<int64_t>((int64_t)GBN, c.swap_tmp_elems / W);
   int st;
+  if ((c.sc.inv_r0 != r0 || c.sc.inv_W != W) && (st = make_inverse(c, c.sc, r0, W))) return st;
+  const int pid = prof_start(sizeof(T) == 8 ? "trsm_f64" : "trsm_f32", c.s);
+  T* B = c.A + r0 * c.lda + col;
+  const int64_t slice = std::max<int64_t>((int64_t)GBN, c.sc.swap_tmp_elems / W);
   for (int64_t off = 0; off < ncols; off += slice) {
     const int64_t w = std::min<int64_t>(slice, ncols - off);
     hipLaunchKernelGGL((copy_block_kernel<T>), dim3((unsigned)std::min<int64_t>(ceil_div(w, 256), 64), W),
-                       dim3(256), 0, c.s, B + off, c.lda, W, w, c.swap_tmp, w);
+                       dim3(256), 0, c.s, B + off, c.lda, W, w, c.sc.swap_tmp, w);
     RBFX_CHECK_LAUNCH();
-    if ((st = gemm_launch<T>(W, w, W, T(1), c.tri_inv, kNB, c.swap_tmp, w, T(0), B + off, c.lda, c.s))) return st;
+    if ((st = gemm_launch<T, 1>(W, w, W, T(1), c.sc.tri_inv, kNB, c.sc.swap_tmp, w, T(0), B + off, c.lda,
+                                c.s)))
+      return st;
   }
   prof_stop(pid, c.s, 2.0 * W * W * ncols, 3.0 * sizeof(T) * W * ncols);
   return RBFX_OK;
@@ -865,13 +925,55 @@ static size_t getrf_ws_bytes(int64_t n, int bytes) {
   b += ws_round((size_t)2 * G * kPanelWMax * bytes);  // cand_row
   b += ws_round(64);                              // status
   b += ws_round((size_t)G * 4);                   // panel step flags
-  b += ws_round((size_t)2 * kSwapMax * 8) * 2;    // plan dst / src
-  b += ws_round(64);                              // plan count
-  // staging of row swaps (2*NB rows x min(n, 8192) columns) and of the
-  // triangular solve's right-hand side (NB x n)
+  for (int q = 0; q < 3; ++q) {                   // two slots of the main stream, one of the panel stream
+    b += ws_round((size_t)2 * kSwapMax * 8) * 2;  // plan dst / src
+    b += ws_round(64);                            // plan count
+    b += ws_round((size_t)kNB * kNB * bytes);     // inverse of the diagonal block
+  }
+  // main stream: staging of row swaps (2*NB rows x min(n, 8192) columns) and
+  // of the triangular solve's right-hand side (NB x n)
   b += ws_round((size_t)std::max<int64_t>(2 * kNB * std::min<int64_t>(n, 8192), (int64_t)kNB * n) * bytes);
-  b += ws_round((size_t)kNB * kNB * bytes);  // inverse of the diagonal block
+  // panel stream: swaps and solves inside one panel (at most 2*NB x NB)
+  b += ws_round((size_t)2 * kNB * kNB * bytes);
   return b;
+}
+
+template <typename T>
+static bool carve_scratch(Workspace& ws, LuScratch<T>& sc, int64_t tmp_elems) {
+  sc.plan_dst = ws.take<int64_t>(2 * kSwapMax);
+  sc.plan_src = ws.take<int64_t>(2 * kSwapMax);
+  sc.plan_count = ws.take<int>(16);
+  sc.tri_inv = ws.take<T>((size_t)kNB * kNB);
+  sc.swap_tmp_elems = tmp_elems;
+  sc.swap_tmp = ws.take<T>(tmp_elems);
+  sc.plan_s = sc.plan_e = -1;
+  sc.inv_r0 = -1;
+  sc.inv_W = 0;
+  return sc.plan_dst && sc.plan_src && sc.plan_count && sc.tri_inv && (tmp_elems == 0 || sc.swap_tmp);
+}
+
+// The panels run on a second stream of this library (default priority, one
+// per device) so that panel k+1 overlaps the trailing update of step k
+// (look-ahead).  Enqueue sequences of concurrent calls on one device are
+// serialised by a mutex, so the shared stream and events pair up correctly.
+struct PanelStream {
+  hipStream_t s = nullptr;
+  hipEvent_t ev[2] = {nullptr, nullptr};
+};
+static std::mutex g_ps_mu;
+static PanelStream g_ps[64];
+
+static int panel_stream(PanelStream*& out) {
+  int dev = 0;
+  RBFX_HIP(hipGetDevice(&dev));
+  RBFX_REQUIRE(dev >= 0 && dev < 64, "device index %d out of range", dev);
+  PanelStream& p = g_ps[dev];
+  if (!p.s) {
+    RBFX_HIP(hipStreamCreateWithFlags(&p.s, hipStreamNonBlocking));
+    for (int i = 0; i < 2; ++i) RBFX_HIP(hipEventCreateWithFlags(&p.ev[i], hipEventDisableTiming));
+  }
+  out = &p;
+  return RBFX_OK;
 }
 
 template <typename T>
@@ -882,6 +984,10 @@ static int getrf(int64_t n, T* A, int64_t lda, int64_t* ipiv, void* work, size_t
   RBFX_REQUIRE(n < (int64_t)INT32_MAX, "n too large");
   RBFX_REQUIRE(work && work_bytes >= getrf_ws_bytes(n, sizeof(T)), "getrf workspace too small");
   const DeviceInfo& di = device_info();
+  std::lock_guard<std::mutex> lock(g_ps_mu);
+  PanelStream* ps = nullptr;
+  int st;
+  if ((st = panel_stream(ps))) return st;
   LuCtx<T> c;
   c.A = A;
   c.n = n;
@@ -896,13 +1002,15 @@ static int getrf(int64_t n, T* A, int64_t lda, int64_t* ipiv, void* work, size_t
   unsigned* ctl = ws.take<unsigned>(16);
   c.status = reinterpret_cast<int*>(ctl + 1);
   c.flags = ws.take<unsigned>(G);
-  c.plan_dst = ws.take<int64_t>(2 * kSwapMax);
-  c.plan_src = ws.take<int64_t>(2 * kSwapMax);
-  c.plan_count = ws.take<int>(16);
-  c.swap_tmp_elems = std::max<int64_t>(2 * kNB * std::min<int64_t>(n, 8192), (int64_t)kNB * n);
-  c.swap_tmp = ws.take<T>(c.swap_tmp_elems);
-  c.tri_inv = ws.take<T>((size_t)kNB * kNB);
-  RBFX_REQUIRE(c.swap_tmp != nullptr && c.tri_inv != nullptr, "workspace carve failed");
+  // main stream: two slots of (plan, inverse) filled by the panel stream
+  // for alternate steps, one staging buffer; panel stream: its own scratch
+  LuScratch<T> slot[2], sc_panel;
+  const int64_t tmp_main = std::max<int64_t>(2 * kNB * std::min<int64_t>(n, 8192), (int64_t)kNB * n);
+  const bool ok = carve_scratch(ws, slot[0], tmp_main) && carve_scratch(ws, slot[1], 0) &&
+                  carve_scratch(ws, sc_panel, (int64_t)2 * kNB * kNB);
+  slot[1].swap_tmp = slot[0].swap_tmp;
+  slot[1].swap_tmp_elems = slot[0].swap_tmp_elems;
+  RBFX_REQUIRE(ok && c.cand && c.cand_row && c.flags, "workspace carve failed");
   c.seq = 0;
   c.G_max = G;
   c.lds_budget = std::min<size_t>(di.lds_per_block, 65536);
@@ -913,20 +1021,53 @@ static int getrf(int64_t n, T* A, int64_t lda, int64_t* ipiv, void* work, size_t
   RBFX_CHECK_LAUNCH();
   hipLaunchKernelGGL(zero_u32_kernel, dim3(G / 64), dim3(64), 0, s, c.flags, G);
   RBFX_CHECK_LAUNCH();
-  RBFX_HIP(hipMemsetAsync(info, 0, sizeof(int32_t), s));
-  int st;
-  for (int64_t k0 = 0; k0 < n; k0 += kNB) {
+  hipLaunchKernelGGL(zero_u32_kernel, dim3(1), dim3(64), 0, s, reinterpret_cast<unsigned*>(info), 1);
+  RBFX_CHECK_LAUNCH();
+
+  // panel context: same matrix and pivot records, own stream and scratch
+  LuCtx<T> cp = c;
+  cp.s = ps->s;
+  cp.sc = sc_panel;
+  hipEvent_t ready = ps->ev[0], done = ps->ev[1];
+  // Panel stream, per step: factor the panel, then the row-interchange plan
+  // and the inverse of its diagonal block for the main stream's slot.
+  auto panel_step = [&](int64_t r0, int W, LuScratch<T>& out) -> int {
+    int e;
+    if ((e = panel(cp, r0, W))) return e;
+    if ((e = make_plan(cp, out, r0, r0 + W))) return e;
+    if (r0 + W < n && (e = make_inverse(cp, out, r0, W))) return e;
+    return RBFX_OK;
+  };
+  // panel 0 after everything queued so far on the caller's stream
+  RBFX_HIP(hipEventRecord(ready, s));
+  RBFX_HIP(hipStreamWaitEvent(cp.s, ready, 0));
+  if ((st = panel_step(0, (int)std::min<int64_t>(kNB, n), slot[0]))) return st;
+  RBFX_HIP(hipEventRecord(done, cp.s));
+  int par = 0;
+  for (int64_t k0 = 0; k0 < n; k0 += kNB, par ^= 1) {
     const int W = (int)std::min<int64_t>(kNB, n - k0);
-    if ((st = panel(c, k0, W))) return st;
-    if ((st = apply_swaps(c, k0, k0 + W, 0, k0))) return st;
-    const int64_t rest = n - k0 - W;
-    if (rest > 0) {
-      if ((st = apply_swaps(c, k0, k0 + W, k0 + W, rest))) return st;
-      if ((st = trsm(c, k0, W, k0 + W, rest))) return st;
-      if ((st = gemm<T>(rest, rest, W, T(-1), A + (k0 + W) * lda + k0, lda, A + k0 * lda + k0 + W, lda,
-                        T(1), A + (k0 + W) * lda + k0 + W, lda, s)))
-        return st;
-    }
+    const int64_t k1 = k0 + W;
+    const int64_t rest = n - k1;
+    c.sc = slot[par];
+    RBFX_HIP(hipStreamWaitEvent(s, done, 0));  // panel k0, its plan and inverse are ready
+    if ((st = apply_swaps(c, k0, k1, 0, k0))) return st;
+    if (rest <= 0) break;
+    if ((st = apply_swaps(c, k0, k1, k1, rest))) return st;
+    if ((st = trsm(c, k0, W, k1, rest))) return st;
+    const T* L21 = A + k1 * lda + k0;
+    const T* U12 = A + k0 * lda + k1;
+    T* A22 = A + k1 * lda + k1;
+    // columns of the next panel first, then that panel on the panel stream
+    // while the rest of the trailing matrix is updated here (look-ahead)
+    const int Wn = (int)std::min<int64_t>(kNB, rest);
+    if ((st = gemm<T>(rest, Wn, W, T(-1), L21, lda, U12, lda, T(1), A22, lda, s))) return st;
+    RBFX_HIP(hipEventRecord(ready, s));
+    RBFX_HIP(hipStreamWaitEvent(cp.s, ready, 0));
+    if ((st = panel_step(k1, Wn, slot[par ^ 1]))) return st;
+    RBFX_HIP(hipEventRecord(done, cp.s));
+    if (rest > Wn &&
+        (st = gemm<T>(rest, rest - Wn, W, T(-1), L21, lda, U12 + Wn, lda, T(1), A22 + Wn, lda, s)))
+      return st;
   }
   hipLaunchKernelGGL(status_to_info_kernel, dim3(1), dim3(64), 0, s, c.status, info);
   RBFX_CHECK_LAUNCH();

@@ -256,7 +256,7 @@ class KFactor:
     refinement stalls, an FP64 LU is built and used instead (fell_back).
     """
 
-    def __init__(self, problem, precision="fp64", ir_tol=1e-12, ir_restart=200, ir_max_inner=600):
+    def __init__(self, problem, precision="fp64", ir_tol=1e-11, ir_restart=120, ir_max_inner=120):
         self.problem = problem
         self.n = problem.n
         self.precision = precision
@@ -280,6 +280,7 @@ class KFactor:
         st = SolveStats()
         st.precision = self.precision
         if self.precision == "fp64" or (self.lu64 is not None and self.lu32 is None):
+            st.fell_back = self.precision != "fp64"
             x = self.lu64.solve(b, trans)
         else:
             x = self._ir(b, trans, st)
@@ -300,8 +301,12 @@ class KFactor:
                                    int(self.ir_max_inner), L.ptr(ws), ws.numel(), ctypes.byref(stats), L.stream())
         st.ir_outer, st.ir_inner, st.ir_relres = stats.outer_iters, stats.inner_iters, stats.final_relres
         if status == L.RBFX_ERR_STALL:
+            # later solves with this factorisation (e.g. the adjoint) go straight
+            # to the FP64 factors; the FP32 copy is released
             st.fell_back = True
-            return self._fallback().solve(b, trans)
+            lu64 = self._fallback()
+            self.lu32 = None
+            return lu64.solve(b, trans)
         L.check(status)
         return x
 
