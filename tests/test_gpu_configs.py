@@ -1,0 +1,268 @@
+"""GPU tests of the benchmark configurations at their full sizes.
+
+* Poisson 100 x 100 (jittered, degree 2): FP64 and FP32-LU + GMRES-IR against
+  the oracle's FP64 solution (committed fixture tests/golden/poisson_100.npz).
+* Adjoint Laplace control 60 x 60: dL/dc against the closed form B^-T dL/du of
+  the oracle and central finite differences (exact for this quadratic loss;
+  h = 1e-2 because h = 1e-6 is below the forward solves' rounding noise at
+  this conditioning), FP64 and FP32+IR.
+* Advection-diffusion 80 x 80 (periodic West/East): 5 implicit steps against
+  the oracle (tests/golden/advdiff_80.npz); 200 steps of backprop through time
+  with checkpoints every 20 against the full graph and finite differences.
+  The kernel is r^3: with the gaussian eps = 5 the system is numerically
+  singular under the reference's semantics, which is tested as such.
+* Large Poisson 200 x 200 (N = 40,000): size-independent checks -- the FP64
+  residual of the augmented system, and the source-coefficient gradient
+  against finite differences.
+"""
+import os
+
+import numpy as np
+import pytest
+import torch
+
+from oracle import assembly as OA, cloud as OC, rbf as R, solver as OS
+from tests import _cases as C
+
+pytestmark = pytest.mark.gpu
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+def _lap(x, center=None, rbf=None, monomial=None, fields=None):
+    import updes_amd as u
+    return u.nodal_laplacian(x, center, rbf, monomial)
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32_ir"])
+def test_poisson_100_matches_oracle(precision):
+    import updes_amd as u
+    g = np.load(os.path.join(GOLDEN, "poisson_100.npz"))
+    nx = 100
+    ft = {"North": "d", "South": "d", "East": "d", "West": "d"}
+    J = C.jitter(nx, nx, 0.2, 0)
+    cloud = u.SquareCloud(Nx=nx, Ny=nx, facet_types=ft, jitter=J)
+    assert np.array_equal(cloud.sorted_nodes.cpu().numpy(), g["nodes"])
+    ex = lambda x: torch.sin(torch.pi * x[0]) * torch.sin(torch.pi * x[1])
+    rhs = lambda x, centers, rbf, fields: -2 * torch.pi ** 2 * torch.sin(torch.pi * x[0]) * torch.sin(torch.pi * x[1])
+    u.clear_cache()
+    sol = u.pde_solver_jit(_lap, rhs, cloud, {f: ex for f in ft}, u.polyharmonic, 2, precision=precision)
+    vals = sol.vals.cpu().numpy()
+    ref = g["vals"]
+    err = np.linalg.norm(vals - ref) / np.linalg.norm(ref)
+    st = u.last_solve_stats()
+    print("precision %s: rel-L2 vs oracle %.3e, stats %s" % (precision, err, st))
+    assert err <= 1e-8
+    # discretisation error against the exact solution equals the oracle's
+    e_gpu = np.linalg.norm(vals - g["exact"]) / np.linalg.norm(g["exact"])
+    e_ref = np.linalg.norm(ref - g["exact"]) / np.linalg.norm(g["exact"])
+    assert abs(e_gpu - e_ref) <= 1e-8 + 1e-6 * e_ref
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32_ir"])
+def test_laplace_control_60_gradient(precision):
+    import updes_amd as u
+    nx = 60
+    ft = {"North": "d", "South": "d", "East": "d", "West": "d"}
+    cloud = u.SquareCloud(Nx=nx, Ny=nx, facet_types=ft)
+    nN = len(cloud.facet_nodes["North"])
+    c0 = torch.as_tensor(np.random.default_rng(0).normal(0, 0.1, nN), device="cuda")
+    xi = cloud.sorted_nodes[:cloud.Ni].cuda()
+    target = torch.sin(torch.pi * xi[:, 0]) * xi[:, 1]
+    zero = {"South": 0.0, "East": 0.0, "West": 0.0}
+
+    def loss_of(c):
+        sol = u.pde_solver(_lap, lambda x, ce, r, f: 0.0, cloud, dict(North=c, **zero), u.polyharmonic, 1,
+                           precision=precision)
+        return ((sol.vals[:cloud.Ni] - target) ** 2).mean()
+
+    u.clear_cache()
+    c = c0.clone().requires_grad_(True)
+    loss_of(c).backward()
+    g = c.grad.detach().cpu().numpy()
+    oc = OC.SquareCloud(nx, nx, ft)
+    q = OA.assemble_q(oc, 0.0, {"North": c0.cpu().numpy(), "South": np.zeros(nx), "East": np.zeros(nx - 2),
+                                "West": np.zeros(nx - 2)})
+    u_ref, _, B = OS.pde_solver_bpath(C.LAPLACIAN, oc, R.POLYHARMONIC, 1.0, 3, q)
+    gu = np.zeros(oc.N)
+    gu[:oc.Ni] = 2 * (u_ref[:oc.Ni] - target.cpu().numpy()) / oc.Ni
+    g_ref = np.linalg.solve(B.T, gu)[oc.facet_nodes["North"]]
+    rel = np.linalg.norm(g - g_ref) / np.linalg.norm(g_ref)
+    print("precision %s: gradient rel err vs closed form %.3e" % (precision, rel))
+    assert rel <= 1e-7
+    # L is quadratic in c (linear PDE), so central differences carry no
+    # truncation error at any h.  At h = 1e-6 the difference of two forward
+    # solves is below their rounding noise here (kappa(K) ~ 1e10 at 60 x 60:
+    # kappa eps |u| / h ~ 1e-6 |u| swamps the signal), so h = 1e-2.
+    h = 1e-2
+    for k in [0, nN // 3, nN - 1]:
+        e = torch.zeros_like(c0)
+        e[k] = h
+        with torch.no_grad():
+            fd = (loss_of(c0 + e) - loss_of(c0 - e)).item() / (2 * h)
+        assert abs(fd - g[k]) <= 1e-6 * max(1.0, np.abs(g).max()), (k, fd, g[k])
+
+
+def _advdiff_80(rbf=None):
+    import updes_amd as u
+    nx = 80
+    ft = {"North": "d", "South": "d", "West": "p1", "East": "p1"}
+    cloud = u.SquareCloud(Nx=nx, Ny=nx, facet_types=ft)
+    rbf = rbf or u.polyharmonic
+    V = torch.tensor((1.0, 0.5), dtype=torch.float64)
+    dt, nu = 1e-3, 0.01
+
+    def diff(x, center=None, rbf=None, monomial=None, fields=None):
+        return (u.nodal_value(x, center, rbf, monomial) + dt * torch.dot(V, u.nodal_gradient(x, center, rbf, monomial))
+                - dt * nu * u.nodal_laplacian(x, center, rbf, monomial))
+
+    def rhs(x, centers=None, rbf=None, fields=None):
+        return u.value(x, fields[:, 0], centers, rbf)
+
+    bcs = {f: 0.0 for f in ft}
+
+    def step(un):
+        return u.pde_solver(diff, rhs, cloud, bcs, rbf, 1, rhs_args=[un]).vals
+
+    return cloud, step
+
+
+def test_advdiff_80_steps_match_oracle():
+    """Five implicit steps (r^3, the kernel of the reference's own periodic
+    advection demo) against the reference algorithm's solves."""
+    import updes_amd as u
+    g = np.load(os.path.join(GOLDEN, "advdiff_80.npz"))
+    cloud, step = _advdiff_80()
+    assert np.array_equal(cloud.sorted_nodes.cpu().numpy(), g["nodes"])
+    u.clear_cache()
+    v = torch.as_tensor(g["u0"], device="cuda")
+    with torch.no_grad():
+        for s in range(5):
+            v = step(v)
+            ref = g["u%d" % (s + 1)]
+            err = np.linalg.norm(v.cpu().numpy() - ref) / np.linalg.norm(ref)
+            print("step %d: rel-L2 vs oracle %.3e (oracle's own K/B spread %.1e)" % (s + 1, err, g["spread%d" % (s + 1)]))
+            # each step's rhs value(x, inv(A) [u^n; 0]) goes through a solve with
+            # A (kappa(A) = 6.9e9 for r^3 at 80 x 80), so two FP64 paths agree to
+            # ~kappa(A) eps; the oracle's two solves of one q differ by 6.2e-9
+            assert err <= float(g["cond_A"]) * np.finfo(float).eps
+
+
+def test_advdiff_80_gaussian_eps5_is_singular_backward_stable():
+    """With the gaussian eps = 5 the periodic system is numerically singular
+    under the reference's semantics (cond(B) = 1.7e19 at 80 x 80; the oracle's
+    B/QR and augmented LU solutions differ by O(1)), so its solution is not
+    reproducible by any FP64 algorithm.  What holds is backward stability: the
+    computed coefficients solve K c = b to a small relative backward error."""
+    import functools
+    import updes_amd as u
+    from updes_amd import engine as E, _lib as L
+    cloud, step = _advdiff_80(functools.partial(u.gaussian, eps=5.0))
+    u.clear_cache()
+    xy = cloud.sorted_nodes.cuda()
+    u0 = torch.exp(-((xy[:, 0] - 0.3) ** 2 + (xy[:, 1] - 0.5) ** 2) / (2 * 0.1 ** 2))
+    with torch.no_grad():
+        step(u0)
+        fac = list(E._K_CACHE.d.values())[-1]
+        b = torch.randn(fac.n, dtype=torch.float64, device="cuda", generator=torch.Generator("cuda").manual_seed(1))
+        c = fac.solve(b)
+        r = E.matvec(fac.problem, c) - b
+        K = E.assemble(fac.problem, L.MAT_K, torch.float64)[:, :fac.n]
+        eta = (torch.linalg.norm(r) / (torch.linalg.norm(K) * torch.linalg.norm(c) + torch.linalg.norm(b))).item()
+    print("gaussian eps=5, 80x80: backward error %.3e, |c| %.3e" % (eta, torch.linalg.norm(c).item()))
+    # partial-pivoting LU is backward stable with a bound ~ n eps times the
+    # growth factor (measured 1.1e-10 here, where |c| ~ 1e27 shows the
+    # singularity); the bound below is n eps * 1e3
+    assert eta <= fac.n * np.finfo(float).eps * 1e3
+
+
+def test_advdiff_80_bptt_200_steps():
+    import updes_amd as u
+    cloud, step = _advdiff_80()
+    u.clear_cache()
+    xy = cloud.sorted_nodes.cuda()
+    u0 = torch.exp(-((xy[:, 0] - 0.3) ** 2 + (xy[:, 1] - 0.5) ** 2) / (2 * 0.1 ** 2))
+    target = 0.5 * torch.exp(-((xy[:, 0] - 0.5) ** 2 + (xy[:, 1] - 0.4) ** 2) / (2 * 0.1 ** 2))
+    nsteps = 200
+    b = u0.clone().requires_grad_(True)
+    w = u.checkpointed_rollout(step, b, nsteps, every=20)
+    Lb = ((w - target) ** 2).sum()
+    Lb.backward()
+    a = u0.clone().requires_grad_(True)
+    v = a
+    for _ in range(nsteps):
+        v = step(v)
+    La = ((v - target) ** 2).sum()
+    La.backward()
+    print("BPTT 200 steps: L %.6e, |grad| %.3e" % (La.item(), torch.linalg.norm(a.grad).item()))
+    assert abs(La.item() - Lb.item()) <= 1e-12 * abs(La.item())
+    assert torch.linalg.norm(a.grad - b.grad) <= 1e-10 * torch.linalg.norm(a.grad)
+    # L is quadratic in u0 (every step is linear in u^n), so central differences
+    # carry no truncation error; h = 1.  Each step's rhs passes through a solve
+    # with A (kappa(A) = 6.9e9 here), so both the rollout and its adjoint carry
+    # ~kappa(A) eps of relative error per step: after n steps the comparison
+    # can only hold to ~n kappa(A) eps (measured 2.9e-6 of max|g| at n = 200,
+    # bound 3e-4), and it is checked tightly on a 5-step rollout.
+    g = np.load(os.path.join(GOLDEN, "advdiff_80.npz"))
+    keps = float(g["cond_A"]) * np.finfo(float).eps
+    h = 1.0
+    k = int(cloud.Ni // 2)
+    e = torch.zeros_like(u0)
+    e[k] = h
+
+    def L(x, n):
+        for _ in range(n):
+            x = step(x)
+        return ((x - target) ** 2).sum()
+
+    with torch.no_grad():
+        fd = (L(u0 + e, nsteps) - L(u0 - e, nsteps)).item() / (2 * h)
+    gmax = b.grad.abs().max().item()
+    print("n=200: FD %.8e adjoint %.8e" % (fd, b.grad[k].item()))
+    assert abs(fd - b.grad[k].item()) <= nsteps * keps * gmax
+    c = u0.clone().requires_grad_(True)
+    L(c, 5).backward()
+    with torch.no_grad():
+        fd5 = (L(u0 + e, 5) - L(u0 - e, 5)).item() / (2 * h)
+    print("n=5: FD %.10e adjoint %.10e" % (fd5, c.grad[k].item()))
+    assert abs(fd5 - c.grad[k].item()) <= 5 * keps * c.grad.abs().max().item()
+
+
+def test_large_poisson_40k_residual_and_gradient():
+    """N = 40,000: the FP64 solution has a small normwise backward error in the
+    augmented system (matrix-free FP64 residual, ||K||_F estimated with random
+    probes), and the adjoint gradient of source coefficients matches central
+    differences (the loss is quadratic in the coefficients, h = 1)."""
+    import updes_amd as u
+    import bench
+    from updes_amd import engine as E
+    cloud, bcs, b, basis = bench.fourier_problem(200, seed=0)
+    u.clear_cache()
+    vals, loss, g = bench.solve_and_grad(cloud, bcs, b, basis, "fp64")
+    fac = list(E._K_CACHE.d.values())[-1]
+    # rebuild the right-hand side the solve used and check K c = [q; 0]
+    with torch.no_grad():
+        f_int = basis @ b
+        q = u.assemble_q(lambda x, centers, rbf, fields: -f_int, bcs, cloud, u.polyharmonic, 6, None)
+        rhs = torch.cat([q, torch.zeros(6, dtype=torch.float64, device=q.device)])
+        c = fac.solve(rhs)
+        r = E.matvec(fac.problem, c) - rhs
+        # ||K||_F from E|K z|^2 = ||K||_F^2 over a few Gaussian probes (matrix-free)
+        gen = torch.Generator("cuda").manual_seed(0)
+        kf = np.sqrt(np.mean([torch.linalg.norm(E.matvec(fac.problem, torch.randn(fac.n, dtype=torch.float64,
+                                                                               device="cuda", generator=gen))).item() ** 2
+                              for _ in range(4)]))
+    eta = torch.linalg.norm(r).item() / (kf * torch.linalg.norm(c).item() + torch.linalg.norm(rhs).item())
+    print("N=40k: backward error %.3e (residual / |b| %.3e), loss %.6e"
+          % (eta, (torch.linalg.norm(r) / torch.linalg.norm(rhs)).item(), loss.item()))
+    assert eta <= fac.n * np.finfo(float).eps * 100
+    # quadratic loss: central differences are exact up to rounding at any h;
+    # h = 1 keeps the forward solves' rounding noise / h far below the bound
+    h = 1.0
+    gk = g.cpu().numpy()
+    for k in [0, 27]:
+        e = torch.zeros_like(b)
+        e[k] = h
+        Lp = bench.solve_and_grad(cloud, bcs, b + e, basis, "fp64", want_grad=False)[1].item()
+        Lm = bench.solve_and_grad(cloud, bcs, b - e, basis, "fp64", want_grad=False)[1].item()
+        fd = (Lp - Lm) / (2 * h)
+        assert abs(fd - gk[k]) <= 1e-6 * max(1e-3, np.abs(gk).max()), (k, fd, gk[k])

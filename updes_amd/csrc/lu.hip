@@ -809,6 +809,9 @@ static int leaf(LuCtx<T>& c, int64_t r0, int w) {
   const size_t smem = need(R);
   void* args[] = {&a};
   const int pid = prof_start(sizeof(T) == 8 ? "panel_f64" : "panel_f32", c.s);
+  if (smem > 65536)
+    RBFX_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&panel_kernel<T>),
+                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem));
   RBFX_HIP(hipLaunchCooperativeKernel(reinterpret_cast<const void*>(&panel_kernel<T>), dim3(G),
                                       dim3(kPanelThreads), args, (unsigned)smem, c.s));
   prof_stop(pid, c.s, (double)m * w * w, 2.0 * sizeof(T) * m * w);
@@ -1013,10 +1016,12 @@ static int getrf(int64_t n, T* A, int64_t lda, int64_t* ipiv, void* work, size_t
   RBFX_REQUIRE(ok && c.cand && c.cand_row && c.flags, "workspace carve failed");
   c.seq = 0;
   c.G_max = G;
-  c.lds_budget = std::min<size_t>(di.lds_per_block, 65536);
+  // A leaf keeps its rows in LDS.  96 KB leaves room for two trailing-update
+  // GEMM workgroups (32 KB each) on the same CU, so the look-ahead panel runs
+  // beside the update.
+  c.lds_budget = std::min<size_t>(di.lds_per_block, 96 * 1024);
   c.wmax = panel_width_max(n, sizeof(T), di, c.lds_budget);
   RBFX_REQUIRE(c.wmax > 0, "matrix too large for the LDS-resident panel (n=%lld)", (long long)n);
-  c.wmax = std::min(c.wmax, sizeof(T) == 8 ? 32 : 64);
   hipLaunchKernelGGL(zero_u32_kernel, dim3(1), dim3(64), 0, s, ctl, 16);
   RBFX_CHECK_LAUNCH();
   hipLaunchKernelGGL(zero_u32_kernel, dim3(G / 64), dim3(64), 0, s, c.flags, G);
