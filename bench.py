@@ -131,6 +131,19 @@ def cpu_baseline(seconds_budget=30.0):
                       "scaled to N=40,000 by (40006/3606)^3 = %.0f" % (len(times), t, scale)}
 
 
+def measured_traffic(kernel, side):
+    """HBM bytes per launch of `kernel` from the committed counter summary of
+    this workload (two rocprofv3 --pmc runs scaled by a copy of known size),
+    or None when no summary covers it."""
+    path = os.path.join(HERE, "profiles", "%s_40k_traffic.md" % kernel)
+    if side != 200 or not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith("traffic_bytes_per_launch:"):
+            return float(line.split(":")[1])
+    return None
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -232,6 +245,11 @@ def main():
                            "unit": "TFLOP/s", "frac": achieved / peak, "traffic": None,
                            "algorithmic_bytes_per_launch": pk["bytes"] / pk["launches"],
                            "launches": pk["launches"], "avg_launch_ms": pk["total_ms"] / pk["launches"]}
+        traffic = measured_traffic(gname, args.side)
+        if traffic is not None:
+            out["roofline"]["traffic"] = traffic
+            out["roofline"]["traffic_source"] = "profiles/%s_40k_traffic.md (FETCH_SIZE + WRITE_SIZE runs, " \
+                                                "copy-scaled)" % gname
         out["kernels"] = {p: {k: {"launches": v["launches"], "total_ms": round(v["total_ms"], 3),
                                   "tflops": (v["flops"] / (v["total_ms"] * 1e-3) / 1e12) if v["flops"] else None,
                                   "gbps": (v["bytes"] / (v["total_ms"] * 1e-3) / 1e9) if v["bytes"] else None}
