@@ -280,7 +280,12 @@ class KFactor:
         st = SolveStats()
         st.precision = self.precision
         if self.precision == "fp64" or (self.lu64 is not None and self.lu32 is None):
-            st.fell_back = self.precision != "fp64"
+            if self.precision != "fp64":
+                # refinement stalled on an earlier solve with these factors: keep
+                # its statistics, this solve uses the FP64 factors directly
+                prev = self.last_stats
+                st.fell_back = True
+                st.ir_outer, st.ir_inner, st.ir_relres = prev.ir_outer, prev.ir_inner, prev.ir_relres
             x = self.lu64.solve(b, trans)
         else:
             x = self._ir(b, trans, st)
