@@ -219,41 +219,6 @@ static int gemm(int64_t m, int64_t n, int64_t k, T alpha, const T* A, int64_t ld
   return st;
 }
 
-// ---------------------------------------------------------------------------
-// Grid-wide step barrier for the cooperative panel kernel (all workgroups
-// co-resident).  Every thread's coherent stores of the step are complete
-// (s_waitcnt 0) before workgroup g announces step `target` in its own flag;
-// then every workgroup polls all flags in parallel.  Flags only grow (targets
-// are cumulative per factorisation).
-__device__ inline bool grid_step(unsigned* flags, unsigned target, int* status) {
-  __builtin_amdgcn_s_waitcnt(0);
-  __syncthreads();
-  __shared__ int bad;
-  if (threadIdx.x == 0) {
-    bad = 0;
-    st_coh(flags + blockIdx.x, target);
-  }
-  __syncthreads();
-  for (unsigned q = threadIdx.x; q < gridDim.x; q += blockDim.x) {
-    uint32_t spins = 0;
-    while (ld_coh(flags + q) < target) {
-      __builtin_amdgcn_s_sleep(1);
-      ++spins;
-      if ((spins & 255u) == 0 &&
-          __hip_atomic_load(status, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) {
-        bad = 1;
-        break;
-      }
-      if (spins > kSpinLimit) {
-        __hip_atomic_store(status, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        bad = 1;
-        break;
-      }
-    }
-  }
-  __syncthreads();
-  return bad == 0;
-}
 
 template <typename T>
 struct PanelArgs {
@@ -265,9 +230,8 @@ struct PanelArgs {
   int64_t RP;     // padded LDS column stride
   int64_t r0;     // absolute row of the panel's first row
   int64_t* ipiv;  // (w) absolute pivot rows
-  double* cand;   // [2][G] (|value|, row) pairs
+  double* cand;   // [2][G] tagged (|value|, row) records (two 64-bit words each)
   T* cand_row;    // [2][G][w]
-  unsigned* flags;    // [G] step flags (grid_step)
   unsigned seq_base;  // step count before this launch
   int* status;
   int32_t* info;
@@ -338,6 +302,7 @@ __global__ __launch_bounds__(kPanelThreads) void panel_kernel(PanelArgs<T> a) {
   __shared__ int64_t s_i[kPanelThreads / 64];
   __shared__ int64_t s_win[kPanelWMax];
   __shared__ int s_zero[kPanelWMax];
+  __shared__ int s_bad;
   __shared__ MoveList ml;
 
   const int G = gridDim.x, g = blockIdx.x, tid = threadIdx.x;
@@ -365,24 +330,53 @@ __global__ __launch_bounds__(kPanelThreads) void panel_kernel(PanelArgs<T> a) {
     double bv;
     int64_t bi;
     block_argmax(best, bidx, s_v, s_i, bv, bi);
-    // 2. publish it: (|value|, row) and the row's w entries
-    double* rec = a.cand + 2 * ((int64_t)par * G + g);
-    if (tid == 0) {
-      st_coh(rec, bv);
-      st_coh(rec + 1, (double)bi);
-    }
+    // 2. publish it.  The candidate row goes out first; once every thread's
+    //    stores of it have completed, the tagged record
+    //      w0 = bits of |value| with the 16 low mantissa bits set to the tag,
+    //      w1 = tag << 48 | row.
+    //    A reader takes a record only when both words carry the tag of column
+    //    j, so the record needs no separate flag and its two stores no order.
+    const unsigned long long tag = (unsigned long long)((a.seq_base + (unsigned)(j + 1)) & 0xFFFFu);
     if (bv >= 0.0 && tid < w) st_coh(a.cand_row + ((int64_t)par * G + g) * w + tid, P[(int64_t)tid * a.RP + (bi - rb)]);
-    // 3. wait until every workgroup has published column j
-    if (!grid_step(a.flags, a.seq_base + (unsigned)(j + 1), a.status)) return;
-    // 4. the winner across workgroups
+    __builtin_amdgcn_s_waitcnt(0);
+    __syncthreads();
+    unsigned long long* recs = reinterpret_cast<unsigned long long*>(a.cand);
+    if (tid == 0) {
+      unsigned long long* rec = recs + 2 * ((int64_t)par * G + g);
+      st_coh(rec, (unsigned long long)(__double_as_longlong(bv) & ~0xFFFFll) | tag);
+      st_coh(rec + 1, (tag << 48) | ((unsigned long long)bi & 0xFFFFFFFFFFFFull));
+    }
+    // 3.-4. every workgroup's record of column j (one thread each) and the
+    //    winner across workgroups
     double v = -1.0;
     int64_t vi = INT64_MAX;
+    if (tid == 0) s_bad = 0;
+    __syncthreads();
     for (int q = tid; q < G; q += kPanelThreads) {
-      const double* o = a.cand + 2 * ((int64_t)par * G + q);
-      const double ov = ld_coh(o);
-      const double oi = ld_coh(o + 1);
-      if (ov >= 0.0) argmax_merge(v, vi, ov, (int64_t)oi);
+      const unsigned long long* o = recs + 2 * ((int64_t)par * G + q);
+      unsigned long long w0 = ld_coh(o), w1 = ld_coh(o + 1);
+      uint32_t spins = 0;
+      while ((w0 & 0xFFFFull) != tag || (w1 >> 48) != tag) {
+        __builtin_amdgcn_s_sleep(1);
+        ++spins;
+        if ((spins & 255u) == 0 &&
+            __hip_atomic_load(a.status, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) {
+          s_bad = 1;
+          break;
+        }
+        if (spins > kSpinLimit) {
+          __hip_atomic_store(a.status, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          s_bad = 1;
+          break;
+        }
+        w0 = ld_coh(o);
+        w1 = ld_coh(o + 1);
+      }
+      const double ov = __longlong_as_double((long long)(w0 & ~0xFFFFull));
+      if (ov >= 0.0) argmax_merge(v, vi, ov, (int64_t)(w1 & 0xFFFFFFFFFFFFull));
     }
+    __syncthreads();
+    if (s_bad) return;
     double wv;
     int64_t wi;
     block_argmax(v, vi, s_v, s_i, wv, wi);
@@ -750,7 +744,6 @@ struct LuCtx {
   // workspace
   double* cand;     // [2][G_max] (|value|, row) records of the panel launches
   T* cand_row;      // [2][G_max][kPanelWMax]
-  unsigned* flags;  // [G_max] step flags of the panel launches
   int* status;
   unsigned seq;     // steps taken by the panel launches so far
   LuScratch<T> sc;
@@ -801,7 +794,6 @@ static int leaf(LuCtx<T>& c, int64_t r0, int w) {
   a.ipiv = c.ipiv + r0;
   a.cand = c.cand;
   a.cand_row = c.cand_row;
-  a.flags = c.flags;
   a.seq_base = c.seq;
   a.status = c.status;
   a.info = c.info;
@@ -927,7 +919,6 @@ static size_t getrf_ws_bytes(int64_t n, int bytes) {
   b += ws_round((size_t)4 * G * 8);               // cand records
   b += ws_round((size_t)2 * G * kPanelWMax * bytes);  // cand_row
   b += ws_round(64);                              // status
-  b += ws_round((size_t)G * 4);                   // panel step flags
   for (int q = 0; q < 3; ++q) {                   // two slots of the main stream, one of the panel stream
     b += ws_round((size_t)2 * kSwapMax * 8) * 2;  // plan dst / src
     b += ws_round(64);                            // plan count
@@ -1004,7 +995,6 @@ static int getrf(int64_t n, T* A, int64_t lda, int64_t* ipiv, void* work, size_t
   c.cand_row = ws.take<T>((size_t)2 * G * kPanelWMax);
   unsigned* ctl = ws.take<unsigned>(16);
   c.status = reinterpret_cast<int*>(ctl + 1);
-  c.flags = ws.take<unsigned>(G);
   // main stream: two slots of (plan, inverse) filled by the panel stream
   // for alternate steps, one staging buffer; panel stream: its own scratch
   LuScratch<T> slot[2], sc_panel;
@@ -1013,7 +1003,7 @@ static int getrf(int64_t n, T* A, int64_t lda, int64_t* ipiv, void* work, size_t
                   carve_scratch(ws, sc_panel, (int64_t)2 * kNB * kNB);
   slot[1].swap_tmp = slot[0].swap_tmp;
   slot[1].swap_tmp_elems = slot[0].swap_tmp_elems;
-  RBFX_REQUIRE(ok && c.cand && c.cand_row && c.flags, "workspace carve failed");
+  RBFX_REQUIRE(ok && c.cand && c.cand_row, "workspace carve failed");
   c.seq = 0;
   c.G_max = G;
   // A leaf keeps its rows in LDS.  96 KB leaves room for two trailing-update
@@ -1024,7 +1014,8 @@ static int getrf(int64_t n, T* A, int64_t lda, int64_t* ipiv, void* work, size_t
   RBFX_REQUIRE(c.wmax > 0, "matrix too large for the LDS-resident panel (n=%lld)", (long long)n);
   hipLaunchKernelGGL(zero_u32_kernel, dim3(1), dim3(64), 0, s, ctl, 16);
   RBFX_CHECK_LAUNCH();
-  hipLaunchKernelGGL(zero_u32_kernel, dim3(G / 64), dim3(64), 0, s, c.flags, G);
+  // pivot records start with tag 0, which no column uses
+  hipLaunchKernelGGL(zero_u32_kernel, dim3(8 * G / 64), dim3(64), 0, s, reinterpret_cast<unsigned*>(c.cand), 8 * G);
   RBFX_CHECK_LAUNCH();
   hipLaunchKernelGGL(zero_u32_kernel, dim3(1), dim3(64), 0, s, reinterpret_cast<unsigned*>(info), 1);
   RBFX_CHECK_LAUNCH();
