@@ -230,7 +230,7 @@ struct PanelArgs {
   int64_t RP;     // padded LDS column stride
   int64_t r0;     // absolute row of the panel's first row
   int64_t* ipiv;  // (w) absolute pivot rows
-  double* cand;   // [2][G] tagged (|value|, row) records (two 64-bit words each)
+  double* cand;   // [2][G][kRecWords] tagged (|value|, row) records (first two words used)
   T* cand_row;    // [2][G][w]
   unsigned seq_base;  // step count before this launch
   int* status;
@@ -239,6 +239,9 @@ struct PanelArgs {
 
 constexpr int kPanelThreads = 256;
 constexpr int kPanelWMax = 64;
+// A pivot record (two 64-bit words) owns a 128-byte line: every workgroup
+// polls every record, and records sharing a line would serialise the polls.
+constexpr int kRecWords = 16;
 
 // (value, index) maximum with the smaller index winning ties.
 __device__ inline void argmax_merge(double& v, int64_t& i, double ov, int64_t oi) {
@@ -342,7 +345,7 @@ __global__ __launch_bounds__(kPanelThreads) void panel_kernel(PanelArgs<T> a) {
     __syncthreads();
     unsigned long long* recs = reinterpret_cast<unsigned long long*>(a.cand);
     if (tid == 0) {
-      unsigned long long* rec = recs + 2 * ((int64_t)par * G + g);
+      unsigned long long* rec = recs + kRecWords * ((int64_t)par * G + g);
       st_coh(rec, (unsigned long long)(__double_as_longlong(bv) & ~0xFFFFll) | tag);
       st_coh(rec + 1, (tag << 48) | ((unsigned long long)bi & 0xFFFFFFFFFFFFull));
     }
@@ -353,7 +356,7 @@ __global__ __launch_bounds__(kPanelThreads) void panel_kernel(PanelArgs<T> a) {
     if (tid == 0) s_bad = 0;
     __syncthreads();
     for (int q = tid; q < G; q += kPanelThreads) {
-      const unsigned long long* o = recs + 2 * ((int64_t)par * G + q);
+      const unsigned long long* o = recs + kRecWords * ((int64_t)par * G + q);
       unsigned long long w0 = ld_coh(o), w1 = ld_coh(o + 1);
       uint32_t spins = 0;
       while ((w0 & 0xFFFFull) != tag || (w1 >> 48) != tag) {
@@ -770,17 +773,17 @@ template <typename T>
 static int leaf(LuCtx<T>& c, int64_t r0, int w) {
   const DeviceInfo& di = device_info();
   const int64_t m = c.n - r0;
-  int G = (int)std::min<int64_t>(di.num_cus, std::max<int64_t>(1, ceil_div(m, 128)));
-  int64_t R = ceil_div(m, G);
   auto need = [&](int64_t Rr) {
     const int64_t RP = Rr + 1;
     return (size_t)w * RP * sizeof(T) + kPanelWMax * sizeof(T) + (size_t)RP * 4;
   };
   const size_t budget = c.lds_budget - kPanelStaticLds;  // room for the kernel's static LDS
-  while (need(R) > budget && G < di.num_cus) {
-    G = std::min<int>(di.num_cus, G * 2);
-    R = ceil_div(m, G);
-  }
+  // As many rows per workgroup as the LDS budget takes: every column's
+  // exchange polls all G records, so fewer workgroups exchange faster.
+  int64_t Rmax = 1;
+  while (need(Rmax + 1) <= budget) ++Rmax;
+  int G = (int)std::min<int64_t>(di.num_cus, ceil_div(m, Rmax));
+  int64_t R = ceil_div(m, G);
   RBFX_REQUIRE(need(R) <= budget, "panel does not fit in LDS (m=%lld, w=%d)", (long long)m, w);
   G = (int)ceil_div(m, R);
   PanelArgs<T> a;
@@ -916,7 +919,7 @@ static int panel(LuCtx<T>& c, int64_t r0, int W) {
 static size_t getrf_ws_bytes(int64_t n, int bytes) {
   const int G = 1024;  // upper bound on workgroups of a leaf
   size_t b = 0;
-  b += ws_round((size_t)4 * G * 8);               // cand records
+  b += ws_round((size_t)2 * G * kRecWords * 8);   // cand records
   b += ws_round((size_t)2 * G * kPanelWMax * bytes);  // cand_row
   b += ws_round(64);                              // status
   for (int q = 0; q < 3; ++q) {                   // two slots of the main stream, one of the panel stream
@@ -991,7 +994,7 @@ static int getrf(int64_t n, T* A, int64_t lda, int64_t* ipiv, void* work, size_t
   c.s = s;
   Workspace ws(work, work_bytes);
   const int G = 1024;
-  c.cand = ws.take<double>((size_t)4 * G);
+  c.cand = ws.take<double>((size_t)2 * G * kRecWords);
   c.cand_row = ws.take<T>((size_t)2 * G * kPanelWMax);
   unsigned* ctl = ws.take<unsigned>(16);
   c.status = reinterpret_cast<int*>(ctl + 1);
@@ -1015,7 +1018,8 @@ static int getrf(int64_t n, T* A, int64_t lda, int64_t* ipiv, void* work, size_t
   hipLaunchKernelGGL(zero_u32_kernel, dim3(1), dim3(64), 0, s, ctl, 16);
   RBFX_CHECK_LAUNCH();
   // pivot records start with tag 0, which no column uses
-  hipLaunchKernelGGL(zero_u32_kernel, dim3(8 * G / 64), dim3(64), 0, s, reinterpret_cast<unsigned*>(c.cand), 8 * G);
+  hipLaunchKernelGGL(zero_u32_kernel, dim3(4 * G * kRecWords / 64), dim3(64), 0, s,
+                     reinterpret_cast<unsigned*>(c.cand), 4 * G * kRecWords);
   RBFX_CHECK_LAUNCH();
   hipLaunchKernelGGL(zero_u32_kernel, dim3(1), dim3(64), 0, s, reinterpret_cast<unsigned*>(info), 1);
   RBFX_CHECK_LAUNCH();
