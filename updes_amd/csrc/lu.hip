@@ -475,7 +475,7 @@ __global__ __launch_bounds__(kPanelThreads) void panel_kernel(PanelArgs<T> a) {
 // ---------------------------------------------------------------------------
 // Row interchange plan: the net permutation of the sequential swaps
 // ipiv[s..e) (absolute rows, ipiv[i] >= i) as (dst, src) pairs.
-constexpr int kSwapMax = 256;
+constexpr int kSwapMax = 512;
 
 __global__ __launch_bounds__(64) void swap_plan_kernel(const int64_t* __restrict__ ipiv, int64_t s,
                                                        int64_t e, int64_t* __restrict__ dst,
@@ -578,7 +578,7 @@ __global__ __launch_bounds__(256) void swap_scatter_kernel(T* __restrict__ A, in
 // diagonal inverses (rows of each column held in registers by 4 waves), then
 // the blocks below the diagonal from X_ij = -X_ii sum_{k=j}^{i-1} L_ik X_kj.
 constexpr int kTri = 64;
-constexpr int kNB = 256;  // outer panel width of the blocked LU
+constexpr int kNB = 512;  // outer panel width of the blocked LU
 
 template <typename T>
 __global__ __launch_bounds__(256) void trtri_diag_kernel(const T* __restrict__ L, int64_t ldl, int W,
