@@ -124,9 +124,11 @@ __global__ __launch_bounds__(GTHREADS) __attribute__((amdgpu_waves_per_eu(2, 2))
   const int ar = tid % GBM, ak = (tid / GBM) * AV;
   // B tile (BK x GBN): thread -> k row bk = tid / (GBN / BV), col chunk bc
   const int bk = tid / (GBN / BV), bc = (tid % (GBN / BV)) * BV;
-  T ra[AV], rb[BV];
+  // Two register slots: the loads of k-tile t+2 are issued while tile t is
+  // multiplied, so a load has two tiles of MFMA work to arrive.
+  T ra0[AV], rb0[BV], ra1[AV], rb1[BV];
 
-  auto load = [&](int64_t kt) {
+  auto load = [&](int64_t kt, T (&ra)[AV], T (&rb)[BV]) {
     const int64_t gr = row0 + ar;
     const int64_t gk = kt + ak;
     const T* pa = A + gr * lda + gk;
@@ -148,20 +150,13 @@ __global__ __launch_bounds__(GTHREADS) __attribute__((amdgpu_waves_per_eu(2, 2))
       for (int q = 0; q < BV; ++q) rb[q] = (gkb < k && gc + q < n) ? pb[q] : T(0);
     }
   };
-  auto store = [&](int buf) {
+  auto store = [&](int buf, const T (&ra)[AV], const T (&rb)[BV]) {
 #pragma unroll
     for (int q = 0; q < AV; ++q) As[buf][ak + q][ar] = alpha * ra[q];
 #pragma unroll
     for (int q = 0; q < BV; ++q) Bs[buf][bk][bc + q] = rb[q];
   };
-
-  load(0);
-  store(0);
-  __syncthreads();
-  int buf = 0;
-  for (int64_t kt = 0; kt < k; kt += BK) {
-    const bool more = kt + BK < k;
-    if (more) load(kt + BK);
+  auto compute = [&](int buf) {
 #pragma unroll
     for (int kk = 0; kk < BK; kk += 4) {
       T af[4], bf[4];
@@ -174,11 +169,25 @@ __global__ __launch_bounds__(GTHREADS) __attribute__((amdgpu_waves_per_eu(2, 2))
 #pragma unroll
         for (int j = 0; j < 4; ++j) acc[i][j] = Mma<T>::mma(af[i], bf[j], acc[i][j]);
     }
-    // the other buffer was last read in the previous iteration, which every
-    // wave finished before the barrier that ended it
-    if (more) store(buf ^ 1);
+  };
+
+  load(0, ra0, rb0);
+  if (BK < k) load(BK, ra1, rb1);
+  store(0, ra0, rb0);
+  __syncthreads();
+  // LDS buffer 0 holds the even k-tiles, buffer 1 the odd ones; slot 0 the
+  // register copy of the next even tile, slot 1 of the next odd one.  A
+  // buffer is rewritten only after the barrier that follows its last reads.
+  for (int64_t kt = 0; kt < k; kt += 2 * BK) {
+    if (kt + 2 * BK < k) load(kt + 2 * BK, ra0, rb0);
+    compute(0);
+    if (kt + BK < k) store(1, ra1, rb1);
     __syncthreads();
-    buf ^= 1;
+    if (kt + BK >= k) break;
+    if (kt + 3 * BK < k) load(kt + 3 * BK, ra1, rb1);
+    compute(1);
+    if (kt + 2 * BK < k) store(0, ra0, rb0);
+    __syncthreads();
   }
 
   // epilogue: plain stores
