@@ -144,6 +144,49 @@ def measured_traffic(kernel, side):
     return None
 
 
+def gemm_replay(n, nb=512, leaf=64):
+    """The GEMM launches of the blocked LU of an n x n matrix (panel width nb,
+    recursive panels halved down to `leaf` columns): trailing update of the
+    next panel's columns, of the rest, and the in-panel updates, each on the
+    same sub-blocks of one row-major matrix as in the factorisation."""
+    from updes_amd import _lib as L, engine as E
+    lib = L.lib()
+    ld = E._ld(n)
+    M = torch.randn(n, ld, dtype=torch.float64, device="cuda") * 1e-3
+    base = M.data_ptr()
+    el = 8
+
+    def gemm(m_, n_, k_, a_r, a_c, b_r, b_c, c_r, c_c):
+        L.check(lib.rbfx_gemm_f64(m_, n_, k_, -1.0, ctypes.c_void_p(base + el * (a_r * ld + a_c)), ld,
+                                  ctypes.c_void_p(base + el * (b_r * ld + b_c)), ld, 1.0,
+                                  ctypes.c_void_p(base + el * (c_r * ld + c_c)), ld, L.stream()))
+
+    def panel(r0, W):
+        if W <= leaf:
+            return
+        W1 = max(8, (W // 2 + 7) // 8 * 8)
+        W2 = W - W1
+        panel(r0, W1)
+        gemm(n - r0 - W1, W2, W1, r0 + W1, r0, r0, r0 + W1, r0 + W1, r0 + W1)
+        panel(r0 + W1, W2)
+
+    count = 0
+    for k0 in range(0, n, nb):
+        W = min(nb, n - k0)
+        panel(k0, W)
+        k1 = k0 + W
+        rest = n - k1
+        if rest <= 0:
+            break
+        Wn = min(nb, rest)
+        gemm(rest, Wn, W, k1, k0, k0, k1, k1, k1)
+        if rest > Wn:
+            gemm(rest, rest - Wn, W, k1, k0, k0, k1 + Wn, k1, k1 + Wn)
+        count += 1
+    torch.cuda.synchronize()
+    print("[bench] gemm replay: n=%d, %d panel steps" % (n, count), file=sys.stderr, flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -154,6 +197,9 @@ def main():
     ap.add_argument("--precisions", default="fp64,fp32_ir")
     ap.add_argument("--copy-probe-mib", type=int, default=0)
     ap.add_argument("--no-kernel-profile", action="store_true")
+    ap.add_argument("--gemm-replay", action="store_true",
+                    help="run only the FP64 factorisation's GEMM launches (same shapes and operand blocks) "
+                         "on one matrix, for counter runs of the GEMM alone")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: the dense solve of one collocation system runs on a single MI355X; "
@@ -175,6 +221,9 @@ def main():
         torch.cuda.synchronize()
         print("[bench] copy probe: %d bytes read, %d bytes written" % (8 * nel, 8 * nel), file=sys.stderr, flush=True)
         del src, dst
+    if args.gemm_replay:
+        gemm_replay(args.side * args.side + 6)
+        return
     cloud, bcs, b, basis = fourier_problem(args.side)
     n = cloud.N + 6
     print("[bench] N=%d n=%d, precisions %s" % (cloud.N, n, args.precisions), file=sys.stderr, flush=True)
@@ -248,8 +297,8 @@ def main():
         traffic = measured_traffic(gname, args.side)
         if traffic is not None:
             out["roofline"]["traffic"] = traffic
-            out["roofline"]["traffic_source"] = "profiles/%s_40k_traffic.md (FETCH_SIZE + WRITE_SIZE runs, " \
-                                                "copy-scaled)" % gname
+            out["roofline"]["traffic_source"] = "profiles/%s_40k_traffic.md (FETCH_SIZE + WRITE_SIZE runs on " \
+                                                "the factorisation's GEMM launches replayed alone, copy-scaled)" % gname
         out["kernels"] = {p: {k: {"launches": v["launches"], "total_ms": round(v["total_ms"], 3),
                                   "tflops": (v["flops"] / (v["total_ms"] * 1e-3) / 1e12) if v["flops"] else None,
                                   "gbps": (v["bytes"] / (v["total_ms"] * 1e-3) / 1e9) if v["bytes"] else None}
