@@ -9,9 +9,9 @@
 // Structure per outer panel of NB columns (k0 .. k0+NB):
 //   1. recursive panel factorisation: halves down to WMAX-wide leaves; each
 //      leaf runs as ONE cooperative launch whose workgroups keep their rows of
-//      the leaf in LDS and agree on every pivot through a grid barrier (one
-//      barrier per column: every workgroup publishes its best row, all of them
-//      pick the same winner);
+//      the leaf in LDS and agree on every pivot through one exchange per
+//      column (every workgroup publishes its best row and a tagged record,
+//      all of them poll the records and pick the same winner);
 //   2. row interchanges of the other columns through a gather/scatter plan
 //      (row-major rows are contiguous, so a swap is a coalesced copy);
 //   3. U12 = inv(L11) A12: the unit-lower diagonal block is inverted (64 x 64
