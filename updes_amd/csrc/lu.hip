@@ -8,7 +8,7 @@
 //
 // Structure per outer panel of NB columns (k0 .. k0+NB):
 //   1. recursive panel factorisation: halves down to WMAX-wide leaves; each
-//      leaf runs as ONE cooperative launch whose workgroups keep their rows of
+//      leaf runs as ONE launch of co-resident workgroups that keep their rows of
 //      the leaf in LDS and agree on every pivot through one exchange per
 //      column (every workgroup publishes its best row and a tagged record,
 //      all of them poll the records and pick the same winner);
@@ -65,8 +65,9 @@ constexpr int GBM = 128, GBN = 128, GTHREADS = 256;
 // wait overlaps the other's MFMAs.  The k-tile (BK) keeps the LDS of one
 // workgroup at 32 KB, so two GEMM workgroups leave 96 KB of a CU's LDS to
 // the panel kernels that run beside the trailing update (look-ahead).
-// USE only separates the kernel symbols of the trailing update (0) and of
-// the triangular solve (1) in profiles; the code is the same.
+// USE only separates the kernel symbols of the trailing update (0), of the
+// triangular solve (1) and of the diagonal-block inverse (2) in profiles;
+// the code is the same.
 template <typename T, int USE>
 __global__ __launch_bounds__(GTHREADS) __attribute__((amdgpu_waves_per_eu(2, 2))) void gemm_kernel(
                                                         int64_t m, int64_t n, int64_t k, T alpha,
@@ -238,6 +239,7 @@ struct PanelArgs {
   int64_t R;      // rows per workgroup
   int64_t RP;     // padded LDS column stride
   int64_t r0;     // absolute row of the panel's first row
+  int64_t left, right;  // columns of the enclosing outer panel left / right of the leaf
   int64_t* ipiv;  // (w) absolute pivot rows
   double* cand;   // [2][G][kRecWords] tagged (|value|, row) records (first two words used)
   T* cand_row;    // [2][G][w]
@@ -479,6 +481,35 @@ __global__ __launch_bounds__(kPanelThreads) void panel_kernel(PanelArgs<T> a) {
     const int64_t r = idx / w, c = idx % w;
     a.A[(int64_t)retired[r] * a.lda + c] = P[c * a.RP + r];
   }
+  // The same interchanges on the outer panel's other columns (left and right
+  // of this leaf), so the recursion above needs no row swaps of its own.
+  // Each workgroup takes a disjoint slice of those columns, reads the moved
+  // rows of its slice into LDS (P is free again), then writes them to their
+  // final positions.
+  const int64_t nout = a.left + a.right;
+  if (cnt == 0 || nout == 0) return;
+  __syncthreads();
+  const int64_t per = (nout + G - 1) / G;
+  const int64_t o0 = (int64_t)g * per;
+  const int64_t o1 = o0 + per < nout ? o0 + per : nout;
+  const int64_t chunk = (int64_t)w * a.RP / cnt;  // columns per pass that fit P
+  for (int64_t c0 = o0; c0 < o1; c0 += chunk) {
+    const int64_t nc = c0 + chunk < o1 ? chunk : o1 - c0;
+    for (int64_t e = tid; e < (int64_t)cnt * nc; e += kPanelThreads) {
+      const int k = (int)(e / nc);
+      const int64_t oc = c0 + e % nc;
+      const int64_t col = oc < a.left ? oc - a.left : oc - a.left + w;
+      P[e] = a.A[ml.phys[k] * a.lda + col];
+    }
+    __syncthreads();
+    for (int64_t e = tid; e < (int64_t)cnt * nc; e += kPanelThreads) {
+      const int k = (int)(e / nc);
+      const int64_t oc = c0 + e % nc;
+      const int64_t col = oc < a.left ? oc - a.left : oc - a.left + w;
+      a.A[ml.pos[k] * a.lda + col] = P[e];
+    }
+    __syncthreads();
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -583,25 +614,35 @@ __global__ __launch_bounds__(256) void swap_scatter_kernel(T* __restrict__ A, in
 
 // ---------------------------------------------------------------------------
 // U12 = inv(L11) A12 as one MFMA GEMM.  inv(L11) (unit lower, W <= kNB) is
-// built in two launches (FP64 arithmetic, rounded to T): the 64 x 64
-// diagonal inverses (rows of each column held in registers by 4 waves), then
-// the blocks below the diagonal from X_ij = -X_ii sum_{k=j}^{i-1} L_ik X_kj.
+// built by the panel recursion itself: every leaf inverts its own diagonal
+// block (trtri_diag_kernel, FP64 arithmetic rounded to T) and every split
+// of the recursion joins the inverses of its halves with two GEMMs,
+// X21 = -X22 (L21 X11).
 constexpr int kTri = 64;
 constexpr int kNB = 512;  // outer panel width of the blocked LU
 
+// Inverse of the sz x sz (sz <= 64) unit-lower block at L into X, and zeros
+// in X's columns [sz, zc) of the same rows (the upper blocks of the inverse).
 template <typename T>
-__global__ __launch_bounds__(256) void trtri_diag_kernel(const T* __restrict__ L, int64_t ldl, int W,
-                                                        T* __restrict__ X, int64_t ldx) {
+__global__ __launch_bounds__(256) void trtri_diag_kernel(const T* __restrict__ L, int64_t ldl, int sz,
+                                                        T* __restrict__ X, int64_t ldx, int zc) {
   // strictly lower part of the block, packed by rows: L_ik at i(i-1)/2 + k
   __shared__ double S[kTri * (kTri - 1) / 2];
   __shared__ double Yk[2][kTri];  // row k of the inverse, double-buffered
-  const int o = blockIdx.x * kTri;
-  const int sz = min(kTri, W - o);
   const int t = threadIdx.x;
   const int c = t & (kTri - 1), g = t / kTri;  // column c, rows i = 4q + g
-  for (int i = 1; i < kTri; ++i)
-    for (int k = t; k < i; k += 256)
-      S[i * (i - 1) / 2 + k] = i < sz ? (double)L[(int64_t)(o + i) * ldl + o + k] : 0.0;
+  // whole rows per pass (coalesced), all 16 loads in flight
+  T lv[kTri * kTri / 256];
+#pragma unroll
+  for (int q = 0; q < kTri * kTri / 256; ++q) {
+    const int e = q * 256 + t, i = e / kTri, k = e % kTri;
+    lv[q] = (k < i && i < sz) ? L[(int64_t)i * ldl + k] : T(0);
+  }
+#pragma unroll
+  for (int q = 0; q < kTri * kTri / 256; ++q) {
+    const int e = q * 256 + t, i = e / kTri, k = e % kTri;
+    if (k < i) S[i * (i - 1) / 2 + k] = (double)lv[q];
+  }
   double y[kTri / 4];
 #pragma unroll
   for (int q = 0; q < kTri / 4; ++q) y[q] = (4 * q + g == c) ? 1.0 : 0.0;
@@ -623,90 +664,13 @@ __global__ __launch_bounds__(256) void trtri_diag_kernel(const T* __restrict__ L
       if (i > k && i < sz) y[q] -= S[i * (i - 1) / 2 + k] * yk;
     }
   }
-  // block row: the diagonal inverse and zeros to its right (W columns in all)
   if (c < sz) {
 #pragma unroll
     for (int q = 0; q < kTri / 4; ++q)
-      if (4 * q + g < sz) X[(int64_t)(o + 4 * q + g) * ldx + o + c] = (T)y[q];
+      if (4 * q + g < sz) X[(int64_t)(4 * q + g) * ldx + c] = (T)y[q];
   }
   for (int i = 0; i < sz; ++i)
-    for (int j = o + kTri + t; j < W; j += 256) X[(int64_t)(o + i) * ldx + j] = T(0);
-}
-
-// One workgroup per block column j < nbk-1; 256 threads, each owning a 4 x 4
-// piece of the 64 x 64 block; X_ij for i = j+1, ... in order, products staged
-// through LDS in k-chunks of 16 (17 KB of LDS, so the kernel fits beside the
-// trailing-update GEMM).
-constexpr int kTriK = 16;
-
-template <typename T>
-__global__ __launch_bounds__(256) void trtri_offdiag_kernel(const T* __restrict__ L, int64_t ldl, int W,
-                                                           T* __restrict__ X, int64_t ldx) {
-  __shared__ double As[kTri][kTriK + 1];
-  __shared__ double Bs[kTriK][kTri + 1];
-  const int nbk = (W + kTri - 1) / kTri;
-  const int j = blockIdx.x;
-  const int oj = j * kTri;
-  const int t = threadIdx.x;
-  const int r0 = (t >> 4) * 4, c0 = (t & 15) * 4;
-  // acc += A(rows ai.., cols ak..) * B(rows bk.., cols oj..), both 64 x 64
-  auto mult = [&](const T* Ab, int64_t lda_, int arows, int acols, const T* Bb, int64_t ldb_, int brows,
-                  bool b_from_acc, double (&acc)[4][4], const double (&src)[4][4]) {
-    for (int kc = 0; kc < kTri; kc += kTriK) {
-      for (int e = t; e < kTri * kTriK; e += 256) {
-        const int r = e / kTriK, q = e % kTriK;
-        As[r][q] = (r < arows && kc + q < acols) ? (double)Ab[(int64_t)r * lda_ + kc + q] : 0.0;
-      }
-      if (b_from_acc) {
-        if (r0 >= kc && r0 < kc + kTriK)
-#pragma unroll
-          for (int u = 0; u < 4; ++u)
-#pragma unroll
-            for (int v = 0; v < 4; ++v) Bs[r0 - kc + u][c0 + v] = src[u][v];
-      } else {
-        for (int e = t; e < kTriK * kTri; e += 256) {
-          const int q = e / kTri, cc = e % kTri;
-          Bs[q][cc] = (kc + q < brows) ? (double)Bb[(int64_t)(kc + q) * ldb_ + cc] : 0.0;
-        }
-      }
-      __syncthreads();
-#pragma unroll 4
-      for (int q = 0; q < kTriK; ++q) {
-        double a[4], b[4];
-#pragma unroll
-        for (int u = 0; u < 4; ++u) a[u] = As[r0 + u][q];
-#pragma unroll
-        for (int v = 0; v < 4; ++v) b[v] = Bs[q][c0 + v];
-#pragma unroll
-        for (int u = 0; u < 4; ++u)
-#pragma unroll
-          for (int v = 0; v < 4; ++v) acc[u][v] += a[u] * b[v];
-      }
-      __syncthreads();
-    }
-  };
-  for (int i = j + 1; i < nbk; ++i) {
-    const int oi = i * kTri;
-    const int szi = min(kTri, W - oi);
-    double s_[4][4], x_[4][4];
-#pragma unroll
-    for (int u = 0; u < 4; ++u)
-#pragma unroll
-      for (int v = 0; v < 4; ++v) s_[u][v] = x_[u][v] = 0.0;
-    // S = sum_{k=j}^{i-1} L_ik X_kj   (blocks k < i are full)
-    for (int k = j; k < i; ++k)
-      mult(L + (int64_t)oi * ldl + k * kTri, ldl, szi, kTri, X + (int64_t)(k * kTri) * ldx + oj, ldx, kTri,
-           false, s_, s_);
-    // X_ij = -X_ii S
-    mult(X + (int64_t)oi * ldx + oi, ldx, szi, szi, nullptr, 0, 0, true, x_, s_);
-#pragma unroll
-    for (int u = 0; u < 4; ++u)
-      if (r0 + u < szi)
-#pragma unroll
-        for (int v = 0; v < 4; ++v) X[(int64_t)(oi + r0 + u) * ldx + oj + c0 + v] = (T)(-x_[u][v]);
-    // the block just written is read back (as X_kj) for the next i
-    __syncthreads();
-  }
+    for (int j = sz + t; j < zc; j += 256) X[(int64_t)i * ldx + j] = T(0);
 }
 
 // dst (rows x cols, ld ldd) = src (ld lds)
@@ -723,7 +687,7 @@ __global__ void zero_u32_kernel(unsigned* p, int n) {
   if (i < n) p[i] = 0u;
 }
 
-// A failed or timed-out cooperative panel leaves status != 0: report it as
+// A failed or timed-out panel launch leaves status != 0: report it as
 // info = -1 (the factors are then invalid).
 __global__ void status_to_info_kernel(const int* status, int32_t* info) {
   if (threadIdx.x == 0 && *status != 0) *info = -1;
@@ -741,9 +705,7 @@ struct LuScratch {
   int64_t plan_s = -1, plan_e = -1;  // swap range whose plan is in plan_*
   T* swap_tmp;
   int64_t swap_tmp_elems;
-  T* tri_inv;  // [kNB][kNB]
-  int64_t inv_r0 = -1;  // diagonal block whose inverse is in tri_inv
-  int inv_W = 0;
+  T* tri_inv;  // [kNB][kNB] inverse of the panel's diagonal block
 };
 
 template <typename T>
@@ -762,6 +724,8 @@ struct LuCtx {
   int G_max;
   int wmax;
   size_t lds_budget;
+  int64_t pc0 = 0;  // outer panel being factored: columns [pc0, pc0 + pw)
+  int pw = 0;
 };
 
 // Static LDS of panel_kernel (argmax slots, winners, move list), kept free
@@ -803,6 +767,9 @@ static int leaf(LuCtx<T>& c, int64_t r0, int w) {
   a.R = R;
   a.RP = R + 1;
   a.r0 = r0;
+  a.left = r0 - c.pc0;
+  a.right = c.pc0 + c.pw - (r0 + w);
+  RBFX_REQUIRE(a.left >= 0 && a.right >= 0, "leaf outside its outer panel");
   a.ipiv = c.ipiv + r0;
   a.cand = c.cand;
   a.cand_row = c.cand_row;
@@ -811,13 +778,19 @@ static int leaf(LuCtx<T>& c, int64_t r0, int w) {
   a.info = c.info;
   RBFX_REQUIRE(G <= c.G_max, "too many panel workgroups");
   const size_t smem = need(R);
-  void* args[] = {&a};
   const int pid = prof_start(sizeof(T) == 8 ? "panel_f64" : "panel_f32", c.s);
   if (smem > 65536)
     RBFX_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&panel_kernel<T>),
                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem));
-  RBFX_HIP(hipLaunchCooperativeKernel(reinterpret_cast<const void*>(&panel_kernel<T>), dim3(G),
-                                      dim3(kPanelThreads), args, (unsigned)smem, c.s));
+  // A plain launch on the panel stream.  The workgroups wait for each
+  // other, which needs all G of them resident at once: G is at most the CU
+  // count and each workgroup fits a CU on its own, so the ones not yet
+  // resident start as soon as the trailing-update workgroups on some CU
+  // retire; every wait is bounded in any case.  (A cooperative launch
+  // guarantees the same but goes through a separate device queue, which
+  // added ~60 us of dispatch latency per leaf, measured.)
+  hipLaunchKernelGGL((panel_kernel<T>), dim3(G), dim3(kPanelThreads), smem, c.s, a);
+  RBFX_CHECK_LAUNCH();
   prof_stop(pid, c.s, (double)m * w * w, 2.0 * sizeof(T) * m * w);
   c.seq += (unsigned)w;
   return RBFX_OK;
@@ -836,25 +809,16 @@ static int make_plan(LuCtx<T>& c, LuScratch<T>& dst, int64_t s, int64_t e) {
   return RBFX_OK;
 }
 
-// inv(L11) of the unit-lower block at (r0, r0), W wide, into dst.tri_inv,
-// enqueued on c's stream.
+// Inverse of a leaf's w x w unit-lower diagonal block (at (r0, r0)) into X,
+// with zeros in X's columns [w, zc) of its rows, enqueued on c's stream.
 template <typename T>
-static int make_inverse(LuCtx<T>& c, LuScratch<T>& dst, int64_t r0, int W) {
-  RBFX_REQUIRE(W <= kNB, "diagonal block wider than the panel");
+static int leaf_inverse(LuCtx<T>& c, int64_t r0, int w, T* X, int64_t ldx, int zc) {
+  RBFX_REQUIRE(w <= kTri && zc >= w, "leaf wider than the diagonal inverse kernel");
   const int pid = prof_start(sizeof(T) == 8 ? "trtri_f64" : "trtri_f32", c.s);
-  const T* L = c.A + r0 * c.lda + r0;
-  const int nbk = (int)ceil_div(W, kTri);
-  hipLaunchKernelGGL((trtri_diag_kernel<T>), dim3(nbk), dim3(256), 0, c.s, L, c.lda, W, dst.tri_inv,
-                     (int64_t)kNB);
+  hipLaunchKernelGGL((trtri_diag_kernel<T>), dim3(1), dim3(256), 0, c.s, c.A + r0 * c.lda + r0, c.lda, w, X,
+                     ldx, zc);
   RBFX_CHECK_LAUNCH();
-  if (nbk > 1) {
-    hipLaunchKernelGGL((trtri_offdiag_kernel<T>), dim3(nbk - 1), dim3(256), 0, c.s, L, c.lda, W,
-                       dst.tri_inv, (int64_t)kNB);
-    RBFX_CHECK_LAUNCH();
-  }
-  prof_stop(pid, c.s, (double)W * W * W / 3.0, (double)sizeof(T) * W * W);
-  dst.inv_r0 = r0;
-  dst.inv_W = W;
+  prof_stop(pid, c.s, (double)w * w * w / 3.0, (double)sizeof(T) * w * zc);
   return RBFX_OK;
 }
 
@@ -882,13 +846,12 @@ static int apply_swaps(LuCtx<T>& c, int64_t s, int64_t e, int64_t c0, int64_t nc
   return RBFX_OK;
 }
 
-// U12 = inv(L11) A12 for the W x W diagonal block at r0 and columns
-// [col, col+ncols): staged copy of A12, then one GEMM.
+// U12 = X A12 with X = inv(L11) of the W x W diagonal block at r0 (ld ldx)
+// and columns [col, col+ncols): staged copy of A12, then one GEMM.
 template <typename T>
-static int trsm(LuCtx<T>& c, int64_t r0, int W, int64_t col, int64_t ncols) {
+static int trsm(LuCtx<T>& c, int64_t r0, int W, const T* X, int64_t ldx, int64_t col, int64_t ncols) {
   if (ncols <= 0) return RBFX_OK;
   int st;
-  if ((c.sc.inv_r0 != r0 || c.sc.inv_W != W) && (st = make_inverse(c, c.sc, r0, W))) return st;
   const int pid = prof_start(sizeof(T) == 8 ? "trsm_f64" : "trsm_f32", c.s);
   T* B = c.A + r0 * c.lda + col;
   const int64_t slice = std::max<int64_t>((int64_t)GBN, c.sc.swap_tmp_elems / W);
@@ -897,8 +860,7 @@ static int trsm(LuCtx<T>& c, int64_t r0, int W, int64_t col, int64_t ncols) {
     hipLaunchKernelGGL((copy_block_kernel<T>), dim3((unsigned)std::min<int64_t>(ceil_div(w, 256), 64), W),
                        dim3(256), 0, c.s, B + off, c.lda, W, w, c.sc.swap_tmp, w);
     RBFX_CHECK_LAUNCH();
-    if ((st = gemm_launch<T, 1>(W, w, W, T(1), c.sc.tri_inv, kNB, c.sc.swap_tmp, w, T(0), B + off, c.lda,
-                                c.s)))
+    if ((st = gemm_launch<T, 1>(W, w, W, T(1), X, ldx, c.sc.swap_tmp, w, T(0), B + off, c.lda, c.s)))
       return st;
   }
   prof_stop(pid, c.s, 2.0 * W * W * ncols, 3.0 * sizeof(T) * W * ncols);
@@ -906,23 +868,38 @@ static int trsm(LuCtx<T>& c, int64_t r0, int W, int64_t col, int64_t ncols) {
 }
 
 // Recursive factorisation of the panel rows [r0, n) x columns [r0, r0+W).
+// It also leaves X = inv(L11) of the panel's W x W unit-lower diagonal block
+// (ld ldx), with zeros in X's columns [W, zc) of the same rows.
 template <typename T>
-static int panel(LuCtx<T>& c, int64_t r0, int W) {
+static int panel(LuCtx<T>& c, int64_t r0, int W, T* X, int64_t ldx, int zc) {
   int st;
-  if (W <= c.wmax) return leaf(c, r0, W);
+  if (W <= c.wmax) {
+    if ((st = leaf(c, r0, W))) return st;
+    return leaf_inverse(c, r0, W, X, ldx, zc);
+  }
   const int W1 = std::max(8, (W / 2 + 7) / 8 * 8);
   const int W2 = W - W1;
-  if ((st = panel(c, r0, W1))) return st;
-  // swaps of the left half applied to the right half's columns
-  if ((st = apply_swaps(c, r0, r0 + W1, r0 + W1, W2))) return st;
-  if ((st = trsm(c, r0, W1, r0 + W1, W2))) return st;
+  // (every leaf applies its row interchanges to all columns of the outer
+  // panel itself, so the halves need no swaps here)
+  if ((st = panel(c, r0, W1, X, ldx, zc))) return st;
+  if ((st = trsm(c, r0, W1, X, ldx, r0 + W1, W2))) return st;
   const int64_t m2 = c.n - r0 - W1;
-  if ((st = gemm<T>(m2, W2, W1, T(-1), c.A + (r0 + W1) * c.lda + r0, c.lda, c.A + r0 * c.lda + r0 + W1,
-                    c.lda, T(1), c.A + (r0 + W1) * c.lda + r0 + W1, c.lda, c.s)))
+  const T* L21 = c.A + (r0 + W1) * c.lda + r0;
+  if ((st = gemm<T>(m2, W2, W1, T(-1), L21, c.lda, c.A + r0 * c.lda + r0 + W1, c.lda, T(1),
+                    c.A + (r0 + W1) * c.lda + r0 + W1, c.lda, c.s)))
     return st;
-  if ((st = panel(c, r0 + W1, W2))) return st;
-  // swaps of the right half applied to the left half's columns
-  return apply_swaps(c, r0 + W1, r0 + W, r0, W1);
+  T* X22 = X + (int64_t)W1 * ldx + W1;
+  if ((st = panel(c, r0 + W1, W2, X22, ldx, zc - W1))) return st;
+  // L21 (rows r0+W1 .. r0+W) is final now: later pivots only move rows
+  // >= r0+W.  X21 = -X22 (L21 X11), the product staged in the panel's
+  // swap buffer.
+  RBFX_REQUIRE((int64_t)W1 * W2 <= c.sc.swap_tmp_elems, "inverse staging too small");
+  T* Tm = c.sc.swap_tmp;
+  const int pid = prof_start(sizeof(T) == 8 ? "trtri_f64" : "trtri_f32", c.s);
+  if ((st = gemm_launch<T, 2>(W2, W1, W1, T(1), L21, c.lda, X, ldx, T(0), Tm, W1, c.s))) return st;
+  st = gemm_launch<T, 2>(W2, W1, W2, T(-1), X22, ldx, Tm, W1, T(0), X + (int64_t)W1 * ldx, ldx, c.s);
+  prof_stop(pid, c.s, 2.0 * W2 * W1 * (double)(W1 + W2), (double)sizeof(T) * 3.0 * W1 * W2);
+  return st;
 }
 
 static size_t getrf_ws_bytes(int64_t n, int bytes) {
@@ -953,8 +930,6 @@ static bool carve_scratch(Workspace& ws, LuScratch<T>& sc, int64_t tmp_elems) {
   sc.swap_tmp_elems = tmp_elems;
   sc.swap_tmp = ws.take<T>(tmp_elems);
   sc.plan_s = sc.plan_e = -1;
-  sc.inv_r0 = -1;
-  sc.inv_W = 0;
   return sc.plan_dst && sc.plan_src && sc.plan_count && sc.tri_inv && (tmp_elems == 0 || sc.swap_tmp);
 }
 
@@ -1038,14 +1013,17 @@ static int getrf(int64_t n, T* A, int64_t lda, int64_t* ipiv, void* work, size_t
   cp.s = ps->s;
   cp.sc = sc_panel;
   hipEvent_t ready = ps->ev[0], done = ps->ev[1];
-  // Panel stream, per step: factor the panel, then the row-interchange plan
-  // and the inverse of its diagonal block for the main stream's slot.
+  // Panel stream, per step: factor the panel (which also leaves the inverse
+  // of its diagonal block in the main stream's slot), then the
+  // row-interchange plan for that slot.
   auto panel_step = [&](int64_t r0, int W, LuScratch<T>& out) -> int {
     int e;
-    if ((e = panel(cp, r0, W))) return e;
-    if ((e = make_plan(cp, out, r0, r0 + W))) return e;
-    if (r0 + W < n && (e = make_inverse(cp, out, r0, W))) return e;
-    return RBFX_OK;
+    const int pid = prof_start(sizeof(T) == 8 ? "panel_step_f64" : "panel_step_f32", cp.s);
+    cp.pc0 = r0;
+    cp.pw = W;
+    if ((e = panel(cp, r0, W, out.tri_inv, (int64_t)kNB, W))) return e;
+    prof_stop(pid, cp.s, (double)(n - r0) * W * W, 0.0);
+    return make_plan(cp, out, r0, r0 + W);
   };
   // panel 0 after everything queued so far on the caller's stream
   RBFX_HIP(hipEventRecord(ready, s));
@@ -1062,7 +1040,7 @@ static int getrf(int64_t n, T* A, int64_t lda, int64_t* ipiv, void* work, size_t
     if ((st = apply_swaps(c, k0, k1, 0, k0))) return st;
     if (rest <= 0) break;
     if ((st = apply_swaps(c, k0, k1, k1, rest))) return st;
-    if ((st = trsm(c, k0, W, k1, rest))) return st;
+    if ((st = trsm(c, k0, W, c.sc.tri_inv, (int64_t)kNB, k1, rest))) return st;
     const T* L21 = A + k1 * lda + k0;
     const T* U12 = A + k0 * lda + k1;
     T* A22 = A + k1 * lda + k1;
