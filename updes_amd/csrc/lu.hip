@@ -621,16 +621,21 @@ __global__ __launch_bounds__(256) void swap_scatter_kernel(T* __restrict__ A, in
 constexpr int kTri = 64;
 constexpr int kNB = 512;  // outer panel width of the blocked LU
 
-// Inverse of the sz x sz (sz <= 64) unit-lower block at L into X, and zeros
-// in X's columns [sz, zc) of the same rows (the upper blocks of the inverse).
+// Inverse of the sz x sz (sz <= 64) unit-lower block at L into X[0:sz, 0:sz]
+// (zeros above the diagonal included), in FP64 arithmetic.  Blocked by 16:
+//   1. each wave inverts one 16 x 16 diagonal block by forward substitution,
+//      the solved row broadcast across the wave by lane shuffles;
+//   2. block rows I = 1..3 in turn: S = L[I, 0:I] X[0:I, 0:I], then
+//      X[I, 0:I] = -X_II S (two barriers per block row).
+// Rows and columns from sz to 64 are padded with the identity.
 template <typename T>
 __global__ __launch_bounds__(256) void trtri_diag_kernel(const T* __restrict__ L, int64_t ldl, int sz,
-                                                        T* __restrict__ X, int64_t ldx, int zc) {
-  // strictly lower part of the block, packed by rows: L_ik at i(i-1)/2 + k
-  __shared__ double S[kTri * (kTri - 1) / 2];
-  __shared__ double Yk[2][kTri];  // row k of the inverse, double-buffered
-  const int t = threadIdx.x;
-  const int c = t & (kTri - 1), g = t / kTri;  // column c, rows i = 4q + g
+                                                        T* __restrict__ X, int64_t ldx) {
+  __shared__ double Ls[kTri * (kTri - 1) / 2];  // strictly lower part, packed by rows
+  __shared__ double Xs[kTri][kTri + 1];
+  __shared__ double Ss[16][kTri + 1];
+  const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
+  auto lidx = [](int i, int k) { return i * (i - 1) / 2 + k; };
   // whole rows per pass (coalesced), all 16 loads in flight
   T lv[kTri * kTri / 256];
 #pragma unroll
@@ -641,36 +646,49 @@ __global__ __launch_bounds__(256) void trtri_diag_kernel(const T* __restrict__ L
 #pragma unroll
   for (int q = 0; q < kTri * kTri / 256; ++q) {
     const int e = q * 256 + t, i = e / kTri, k = e % kTri;
-    if (k < i) S[i * (i - 1) / 2 + k] = (double)lv[q];
+    if (k < i) Ls[lidx(i, k)] = (double)lv[q];
+    Xs[i][k] = 0.0;
   }
-  double y[kTri / 4];
-#pragma unroll
-  for (int q = 0; q < kTri / 4; ++q) y[q] = (4 * q + g == c) ? 1.0 : 0.0;
   __syncthreads();
-  // L Y = I by rows: once row k is final, every row i > k drops L_ik Y_k
-  for (int k = 0; k + 1 < sz; ++k) {
-    if (g == (k & 3)) {
-      double v = 0.0;
+  {
+    const int c = lane & 15, r4 = lane >> 4, o = 16 * wv;
+    double y[4];
 #pragma unroll
-      for (int q = 0; q < kTri / 4; ++q)
-        if (q == (k >> 2)) v = y[q];
-      Yk[k & 1][c] = v;
+    for (int q = 0; q < 4; ++q) y[q] = (4 * q + r4 == c) ? 1.0 : 0.0;
+#pragma unroll
+    for (int k = 0; k < 15; ++k) {
+      const double xk = __shfl(y[k >> 2], ((k & 3) << 4) | c, 64);  // X[k][c]
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const int i = 4 * q + r4;
+        if (i > k) y[q] -= Ls[lidx(o + i, o + k)] * xk;
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < 4; ++q) Xs[o + 4 * q + r4][o + c] = y[q];
+  }
+  __syncthreads();
+  for (int I = 1; I < 4; ++I) {
+    const int nI = 16 * I, oI = 16 * I;
+    for (int e = t; e < 16 * nI; e += 256) {
+      const int i = e / nI, j = e % nI;
+      double acc = 0.0;
+      for (int k = j; k < nI; ++k) acc += Ls[lidx(oI + i, k)] * Xs[k][j];  // X is lower: k >= j
+      Ss[i][j] = acc;
     }
     __syncthreads();
-    const double yk = Yk[k & 1][c];
-#pragma unroll
-    for (int q = 0; q < kTri / 4; ++q) {
-      const int i = 4 * q + g;
-      if (i > k && i < sz) y[q] -= S[i * (i - 1) / 2 + k] * yk;
+    for (int e = t; e < 16 * nI; e += 256) {
+      const int i = e / nI, j = e % nI;
+      double acc = 0.0;
+      for (int k = 0; k <= i; ++k) acc += Xs[oI + i][oI + k] * Ss[k][j];
+      Xs[oI + i][j] = -acc;
     }
+    __syncthreads();
   }
-  if (c < sz) {
-#pragma unroll
-    for (int q = 0; q < kTri / 4; ++q)
-      if (4 * q + g < sz) X[(int64_t)(4 * q + g) * ldx + c] = (T)y[q];
+  for (int e = t; e < sz * sz; e += 256) {
+    const int i = e / sz, c = e % sz;
+    X[(int64_t)i * ldx + c] = (T)Xs[i][c];
   }
-  for (int i = 0; i < sz; ++i)
-    for (int j = sz + t; j < zc; j += 256) X[(int64_t)i * ldx + j] = T(0);
 }
 
 // dst (rows x cols, ld ldd) = src (ld lds)
@@ -810,15 +828,15 @@ static int make_plan(LuCtx<T>& c, LuScratch<T>& dst, int64_t s, int64_t e) {
 }
 
 // Inverse of a leaf's w x w unit-lower diagonal block (at (r0, r0)) into X,
-// with zeros in X's columns [w, zc) of its rows, enqueued on c's stream.
+// enqueued on c's stream.
 template <typename T>
-static int leaf_inverse(LuCtx<T>& c, int64_t r0, int w, T* X, int64_t ldx, int zc) {
-  RBFX_REQUIRE(w <= kTri && zc >= w, "leaf wider than the diagonal inverse kernel");
+static int leaf_inverse(LuCtx<T>& c, int64_t r0, int w, T* X, int64_t ldx) {
+  RBFX_REQUIRE(w <= kTri, "leaf wider than the diagonal inverse kernel");
   const int pid = prof_start(sizeof(T) == 8 ? "trtri_f64" : "trtri_f32", c.s);
   hipLaunchKernelGGL((trtri_diag_kernel<T>), dim3(1), dim3(256), 0, c.s, c.A + r0 * c.lda + r0, c.lda, w, X,
-                     ldx, zc);
+                     ldx);
   RBFX_CHECK_LAUNCH();
-  prof_stop(pid, c.s, (double)w * w * w / 3.0, (double)sizeof(T) * w * zc);
+  prof_stop(pid, c.s, (double)w * w * w / 3.0, (double)sizeof(T) * w * w);
   return RBFX_OK;
 }
 
@@ -869,19 +887,21 @@ static int trsm(LuCtx<T>& c, int64_t r0, int W, const T* X, int64_t ldx, int64_t
 
 // Recursive factorisation of the panel rows [r0, n) x columns [r0, r0+W).
 // It also leaves X = inv(L11) of the panel's W x W unit-lower diagonal block
-// (ld ldx), with zeros in X's columns [W, zc) of the same rows.
+// (ld ldx) in X's lower triangle; X's strictly upper part must hold zeros
+// (the buffer is cleared once per factorisation and only ever receives
+// lower-triangular entries).
 template <typename T>
-static int panel(LuCtx<T>& c, int64_t r0, int W, T* X, int64_t ldx, int zc) {
+static int panel(LuCtx<T>& c, int64_t r0, int W, T* X, int64_t ldx) {
   int st;
   if (W <= c.wmax) {
     if ((st = leaf(c, r0, W))) return st;
-    return leaf_inverse(c, r0, W, X, ldx, zc);
+    return leaf_inverse(c, r0, W, X, ldx);
   }
   const int W1 = std::max(8, (W / 2 + 7) / 8 * 8);
   const int W2 = W - W1;
   // (every leaf applies its row interchanges to all columns of the outer
   // panel itself, so the halves need no swaps here)
-  if ((st = panel(c, r0, W1, X, ldx, zc))) return st;
+  if ((st = panel(c, r0, W1, X, ldx))) return st;
   if ((st = trsm(c, r0, W1, X, ldx, r0 + W1, W2))) return st;
   const int64_t m2 = c.n - r0 - W1;
   const T* L21 = c.A + (r0 + W1) * c.lda + r0;
@@ -889,7 +909,7 @@ static int panel(LuCtx<T>& c, int64_t r0, int W, T* X, int64_t ldx, int zc) {
                     c.A + (r0 + W1) * c.lda + r0 + W1, c.lda, c.s)))
     return st;
   T* X22 = X + (int64_t)W1 * ldx + W1;
-  if ((st = panel(c, r0 + W1, W2, X22, ldx, zc - W1))) return st;
+  if ((st = panel(c, r0 + W1, W2, X22, ldx))) return st;
   // L21 (rows r0+W1 .. r0+W) is final now: later pivots only move rows
   // >= r0+W.  X21 = -X22 (L21 X11), the product staged in the panel's
   // swap buffer.
@@ -1007,6 +1027,13 @@ static int getrf(int64_t n, T* A, int64_t lda, int64_t* ipiv, void* work, size_t
   RBFX_CHECK_LAUNCH();
   hipLaunchKernelGGL(zero_u32_kernel, dim3(1), dim3(64), 0, s, reinterpret_cast<unsigned*>(info), 1);
   RBFX_CHECK_LAUNCH();
+  // the diagonal-block inverses: strictly upper parts stay zero from here on
+  for (int q = 0; q < 2; ++q) {
+    const int words = (int)((size_t)kNB * kNB * sizeof(T) / 4);
+    hipLaunchKernelGGL(zero_u32_kernel, dim3((unsigned)ceil_div(words, 256)), dim3(256), 0, s,
+                       reinterpret_cast<unsigned*>(slot[q].tri_inv), words);
+    RBFX_CHECK_LAUNCH();
+  }
 
   // panel context: same matrix and pivot records, own stream and scratch
   LuCtx<T> cp = c;
@@ -1021,7 +1048,7 @@ static int getrf(int64_t n, T* A, int64_t lda, int64_t* ipiv, void* work, size_t
     const int pid = prof_start(sizeof(T) == 8 ? "panel_step_f64" : "panel_step_f32", cp.s);
     cp.pc0 = r0;
     cp.pw = W;
-    if ((e = panel(cp, r0, W, out.tri_inv, (int64_t)kNB, W))) return e;
+    if ((e = panel(cp, r0, W, out.tri_inv, (int64_t)kNB))) return e;
     prof_stop(pid, cp.s, (double)(n - r0) * W * W, 0.0);
     return make_plan(cp, out, r0, r0 + W);
   };
@@ -1037,24 +1064,34 @@ static int getrf(int64_t n, T* A, int64_t lda, int64_t* ipiv, void* work, size_t
     const int64_t rest = n - k1;
     c.sc = slot[par];
     RBFX_HIP(hipStreamWaitEvent(s, done, 0));  // panel k0, its plan and inverse are ready
-    if ((st = apply_swaps(c, k0, k1, 0, k0))) return st;
-    if (rest <= 0) break;
-    if ((st = apply_swaps(c, k0, k1, k1, rest))) return st;
-    if ((st = trsm(c, k0, W, c.sc.tri_inv, (int64_t)kNB, k1, rest))) return st;
+    if (rest <= 0) {
+      if ((st = apply_swaps(c, k0, k1, 0, k0))) return st;
+      break;
+    }
     const T* L21 = A + k1 * lda + k0;
     const T* U12 = A + k0 * lda + k1;
     T* A22 = A + k1 * lda + k1;
-    // columns of the next panel first, then that panel on the panel stream
-    // while the rest of the trailing matrix is updated here (look-ahead)
+    // The next panel's columns first (interchanges, U12 block, update): the
+    // panel stream waits for exactly these.  It then factors that panel
+    // while this stream does the rest of the step (look-ahead): the
+    // interchanges of the columns left of the panel, which nothing in the
+    // factorisation reads again, and those, U12 and the update of the
+    // remaining columns.
     const int Wn = (int)std::min<int64_t>(kNB, rest);
+    if ((st = apply_swaps(c, k0, k1, k1, Wn))) return st;
+    if ((st = trsm(c, k0, W, c.sc.tri_inv, (int64_t)kNB, k1, Wn))) return st;
     if ((st = gemm<T>(rest, Wn, W, T(-1), L21, lda, U12, lda, T(1), A22, lda, s))) return st;
     RBFX_HIP(hipEventRecord(ready, s));
     RBFX_HIP(hipStreamWaitEvent(cp.s, ready, 0));
     if ((st = panel_step(k1, Wn, slot[par ^ 1]))) return st;
     RBFX_HIP(hipEventRecord(done, cp.s));
-    if (rest > Wn &&
-        (st = gemm<T>(rest, rest - Wn, W, T(-1), L21, lda, U12 + Wn, lda, T(1), A22 + Wn, lda, s)))
-      return st;
+    if ((st = apply_swaps(c, k0, k1, 0, k0))) return st;
+    if (rest > Wn) {
+      if ((st = apply_swaps(c, k0, k1, k1 + Wn, rest - Wn))) return st;
+      if ((st = trsm(c, k0, W, c.sc.tri_inv, (int64_t)kNB, k1 + Wn, rest - Wn))) return st;
+      if ((st = gemm<T>(rest, rest - Wn, W, T(-1), L21, lda, U12 + Wn, lda, T(1), A22 + Wn, lda, s)))
+        return st;
+    }
   }
   hipLaunchKernelGGL(status_to_info_kernel, dim3(1), dim3(64), 0, s, c.status, info);
   RBFX_CHECK_LAUNCH();
