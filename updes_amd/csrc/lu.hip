@@ -262,22 +262,55 @@ __device__ inline void argmax_merge(double& v, int64_t& i, double ov, int64_t oi
   }
 }
 
-// Block-wide argmax; one barrier.  The shared slots are reused by the next
-// call only after at least one more barrier of the caller.
-__device__ inline void block_argmax(double v, int64_t idx, double* sv, int64_t* si, double& bv, int64_t& bi) {
-  for (int off = 32; off > 0; off >>= 1) {
-    const double ov = __shfl_down(v, off, 64);
-    const int64_t oi = __shfl_down(idx, off, 64);
-    argmax_merge(v, idx, ov, oi);
-  }
-  if ((threadIdx.x & 63) == 0) {
-    sv[threadIdx.x >> 6] = v;
-    si[threadIdx.x >> 6] = idx;
-  }
+// Pivot keys.  A candidate (|value|, row) is compared as one 64-bit key:
+// the bits of |value| with the low kKeyLow bits replaced by the position of
+// the row, inverted so that a lower position wins a tie of the truncated
+// values:  bits 11..19 = 511 - workgroup, bits 0..10 = 2047 - row within the
+// workgroup.  |value| >= 0, so its bits order like the values; key 0 means
+// "no candidate" (no active row, or NaN).  The choice is partial pivoting on
+// values rounded down to 32 mantissa bits, ties to the lowest row.
+constexpr int kKeyLow = 20;
+constexpr unsigned long long kKeyValMask = ~((1ull << kKeyLow) - 1);
+constexpr int kKeyMaxGroups = 512, kKeyMaxRows = 2048;
+
+__device__ inline unsigned long long pivot_key(double absval, int q, int local) {
+  if (!(absval >= 0.0)) return 0ull;
+  return ((unsigned long long)__double_as_longlong(absval) & kKeyValMask) |
+         ((unsigned long long)(kKeyMaxGroups - 1 - q) << 11) | (unsigned long long)(kKeyMaxRows - 1 - local);
+}
+
+// Maximum of a 64-bit key over the wave with DPP row shifts and broadcasts
+// (no LDS traffic): an inclusive max-scan inside each row of 16 lanes, then
+// row 15 -> rows 1, 3 and lane 31 -> rows 2, 3, so lane 63 holds the result.
+template <int CTRL, int ROWMASK>
+__device__ inline unsigned long long dpp_u64(unsigned long long v) {
+  const int lo = __builtin_amdgcn_update_dpp(0, (int)(unsigned)v, CTRL, ROWMASK, 0xf, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, (int)(unsigned)(v >> 32), CTRL, ROWMASK, 0xf, false);
+  return ((unsigned long long)(unsigned)hi << 32) | (unsigned)lo;
+}
+__device__ inline unsigned long long umax64(unsigned long long a, unsigned long long b) { return a > b ? a : b; }
+__device__ inline unsigned long long wave_max_key(unsigned long long v) {
+  v = umax64(v, dpp_u64<0x111, 0xf>(v));  // row_shr:1
+  v = umax64(v, dpp_u64<0x112, 0xf>(v));  // row_shr:2
+  v = umax64(v, dpp_u64<0x114, 0xf>(v));  // row_shr:4
+  v = umax64(v, dpp_u64<0x118, 0xf>(v));  // row_shr:8
+  v = umax64(v, dpp_u64<0x142, 0xa>(v));  // row_bcast:15
+  v = umax64(v, dpp_u64<0x143, 0xc>(v));  // row_bcast:31
+  const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)v, 63);
+  const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(v >> 32), 63);
+  return ((unsigned long long)hi << 32) | lo;
+}
+
+// Block-wide key maximum; one barrier.  s has one slot per wave and is
+// reused by a later call only after at least one more barrier.
+__device__ inline unsigned long long block_max_key(unsigned long long key, unsigned long long* s) {
+  const unsigned long long wm = wave_max_key(key);
+  if ((threadIdx.x & 63) == 0) s[threadIdx.x >> 6] = wm;
   __syncthreads();
-  bv = sv[0];
-  bi = si[0];
-  for (int q = 1; q < kPanelThreads / 64; ++q) argmax_merge(bv, bi, sv[q], si[q]);
+  unsigned long long b = s[0];
+#pragma unroll
+  for (int q = 1; q < kPanelThreads / 64; ++q) b = umax64(b, s[q]);
+  return b;
 }
 
 // Position bookkeeping of the panel's row interchanges, replayed from the
@@ -312,8 +345,7 @@ __global__ __launch_bounds__(kPanelThreads) void panel_kernel(PanelArgs<T> a) {
   T* P = reinterpret_cast<T*>(smem_raw);                     // [w][RP] column-major
   T* urow = P + (int64_t)a.w * a.RP;                          // [kPanelWMax]
   int* retired = reinterpret_cast<int*>(urow + kPanelWMax);  // [RP]
-  __shared__ double s_v[kPanelThreads / 64];
-  __shared__ int64_t s_i[kPanelThreads / 64];
+  __shared__ unsigned long long s_k1[kPanelThreads / 64], s_k2[kPanelThreads / 64];
   __shared__ int64_t s_win[kPanelWMax];
   __shared__ int s_zero[kPanelWMax];
   __shared__ int s_bad;
@@ -341,9 +373,11 @@ __global__ __launch_bounds__(kPanelThreads) void panel_kernel(PanelArgs<T> a) {
   for (int j = 0; j < w; ++j) {
     const int par = j & 1;
     // 1. best active row of this workgroup for column j
-    double bv;
-    int64_t bi;
-    block_argmax(best, bidx, s_v, s_i, bv, bi);
+    if (tid == 0) s_bad = 0;
+    const unsigned long long k1 =
+        block_max_key(bidx == INT64_MAX ? 0ull : pivot_key(best, 0, (int)(bidx - rb)), s_k1);
+    const int64_t bi = k1 ? rb + (kKeyMaxRows - 1 - (int)(k1 & (kKeyMaxRows - 1))) : INT64_MAX;
+    const double bv = k1 ? __longlong_as_double((long long)(k1 & kKeyValMask)) : -1.0;
     // 2. publish it.  The candidate row goes out first; once every thread's
     //    stores of it have completed, the tagged record
     //      w0 = bits of |value| with the 16 low mantissa bits set to the tag,
@@ -362,10 +396,7 @@ __global__ __launch_bounds__(kPanelThreads) void panel_kernel(PanelArgs<T> a) {
     }
     // 3.-4. every workgroup's record of column j (one thread each) and the
     //    winner across workgroups
-    double v = -1.0;
-    int64_t vi = INT64_MAX;
-    if (tid == 0) s_bad = 0;
-    __syncthreads();
+    unsigned long long k2 = 0;
     for (int q = tid; q < G; q += kPanelThreads) {
       const unsigned long long* o = recs + kRecWords * ((int64_t)par * G + q);
       unsigned long long w0 = ld_coh(o), w1 = ld_coh(o + 1);
@@ -387,27 +418,30 @@ __global__ __launch_bounds__(kPanelThreads) void panel_kernel(PanelArgs<T> a) {
         w1 = ld_coh(o + 1);
       }
       const double ov = __longlong_as_double((long long)(w0 & ~0xFFFFull));
-      if (ov >= 0.0) argmax_merge(v, vi, ov, (int64_t)(w1 & 0xFFFFFFFFFFFFull));
+      if (ov >= 0.0)
+        k2 = umax64(k2, pivot_key(ov, q, (int)((int64_t)(w1 & 0xFFFFFFFFFFFFull) - (int64_t)q * a.R)));
     }
-    __syncthreads();
+    k2 = block_max_key(k2, s_k2);
     if (s_bad) return;
-    double wv;
-    int64_t wi;
-    block_argmax(v, vi, s_v, s_i, wv, wi);
-    if (!(wv >= 0.0)) {
+    const int wq = kKeyMaxGroups - 1 - (int)((k2 >> 11) & (kKeyMaxGroups - 1));
+    const int64_t wi = (int64_t)wq * a.R + (kKeyMaxRows - 1 - (int)(k2 & (kKeyMaxRows - 1)));
+    if (k2 != 0 && (wq >= G || wi >= a.m)) {
+      // a key that names no row cannot come from the records: stop
+      if (tid == 0) __hip_atomic_store(a.status, 3, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      return;
+    }
+    if (k2 == 0) {
       // no finite candidate anywhere (NaN column): every workgroup sees the
-      // same winner value, so all of them leave here together
+      // same key, so all of them leave here together
       if (g == 0 && tid == 0) {
         *a.info = -1;
         __hip_atomic_store(a.status, 2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
       }
       return;
     }
-    const int wg = (int)(wi / a.R);
-    if (tid < w) urow[tid] = ld_coh(a.cand_row + ((int64_t)par * G + wg) * w + tid);
+    if (tid < w) urow[tid] = ld_coh(a.cand_row + ((int64_t)par * G + wq) * w + tid);
     if (tid == 0) {
       s_win[j] = wi;
-      s_zero[j] = wv == 0.0;
       if (wi >= rb && wi < re) retired[wi - rb] = 1;
     }
     __syncthreads();
@@ -415,6 +449,7 @@ __global__ __launch_bounds__(kPanelThreads) void panel_kernel(PanelArgs<T> a) {
     //    best candidate of column j+1 on the way
     const T piv = urow[j];
     const T rpiv = piv != T(0) ? T(1) / piv : T(0);
+    if (tid == 0) s_zero[j] = piv == T(0);
     best = -1.0;
     bidx = INT64_MAX;
     for (int r = tid; r < nr; r += kPanelThreads) {
@@ -794,7 +829,7 @@ static int leaf(LuCtx<T>& c, int64_t r0, int w) {
   a.seq_base = c.seq;
   a.status = c.status;
   a.info = c.info;
-  RBFX_REQUIRE(G <= c.G_max, "too many panel workgroups");
+  RBFX_REQUIRE(G <= c.G_max && G <= kKeyMaxGroups && R < kKeyMaxRows, "too many panel workgroups or rows");
   const size_t smem = need(R);
   const int pid = prof_start(sizeof(T) == 8 ? "panel_f64" : "panel_f32", c.s);
   if (smem > 65536)
