@@ -226,6 +226,15 @@ int rbfx_gemm_f32(int64_t m, int64_t n, int64_t k, float alpha, const float* d_A
                   int64_t lda, const float* d_B, int64_t ldb, float beta, float* d_C,
                   int64_t ldc, void* stream);
 
+/* ---- tuning / test hook -----------------------------------------------------
+ * Caps the workgroups of one panel-leaf launch of rbfx_getrf_* (process-wide;
+ * 0 = the device's CU count, the default).  A leaf keeps its rows in
+ * registers, one to three rows per thread of 256-thread workgroups, so a cap
+ * below ceil(m/256) moves it to two or three rows per thread; a cap that
+ * would need more than three fails the factorisation with an error.  No
+ * counterpart in the reference.                                            */
+int rbfx_set_panel_groups(int max_groups);
+
 #ifdef __cplusplus
 }
 #endif

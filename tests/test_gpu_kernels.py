@@ -73,6 +73,36 @@ def test_getrf_getrs(dtype, n):
         assert np.abs(res).max() < bound + 1e-12, (trans, np.abs(res).max(), bound)
 
 
+@pytest.mark.parametrize("dtype", [torch.float64, torch.float32])
+@pytest.mark.parametrize("n,groups", [(500, 1), (700, 1), (1301, 2), (1301, 3)])
+def test_getrf_multirow_leaves(dtype, n, groups):
+    """Leaves with two or three rows per thread (panel launches capped to few
+    workgroups): same pivots as LAPACK and a small P A - L U residual."""
+    from updes_amd import engine as E
+    L = _lib()
+    a = _random_well(n, 11)
+    ld = E._ld(n)
+    m = torch.zeros((n, ld), dtype=dtype)
+    m[:, :n] = torch.as_tensor(a).to(dtype)
+    L.check(L.lib().rbfx_set_panel_groups(groups))
+    try:
+        lu = E.LU(m.cuda(), n)
+        torch.cuda.synchronize()
+    finally:
+        L.check(L.lib().rbfx_set_panel_groups(0))
+    assert lu.singular == 0
+    ipiv = lu.ipiv.cpu().numpy()
+    if dtype == torch.float64:
+        _, piv_ref = sla.lu_factor(a)
+        assert np.array_equal(ipiv, piv_ref)
+    LU = lu.a[:, :n].double().cpu().numpy()
+    pa = a.astype(np.float64 if dtype == torch.float64 else np.float32).astype(np.float64).copy()
+    for i in range(n):
+        pa[[i, ipiv[i]]] = pa[[ipiv[i], i]]
+    rel = np.abs((np.tril(LU, -1) + np.eye(n)) @ np.triu(LU) - pa).max() / np.abs(a).max()
+    assert rel < (1e-13 if dtype == torch.float64 else 1e-6) * n, rel
+
+
 def test_getrf_matches_lapack_pivots():
     """Same pivot sequence as LAPACK getrf on a matrix without ties."""
     from updes_amd import engine as E

@@ -20,6 +20,7 @@
 //      (v_mfma_f64_16x16x4f64 / v_mfma_f32_16x16x4f32, 128x128 tiles,
 //      XCD-aware tile order).
 #include <algorithm>
+#include <atomic>
 #include <mutex>
 #include <type_traits>
 #include <vector>
@@ -237,7 +238,6 @@ struct PanelArgs {
   int64_t m;      // rows in the panel (r0 .. r0+m)
   int w;          // panel width <= kPanelWMax
   int64_t R;      // rows per workgroup
-  int64_t RP;     // padded LDS column stride
   int64_t r0;     // absolute row of the panel's first row
   int64_t left, right;  // columns of the enclosing outer panel left / right of the leaf
   int64_t* ipiv;  // (w) absolute pivot rows
@@ -339,17 +339,26 @@ __device__ inline int ml_find(const int64_t* keys, int cnt, int64_t key) {
   return -1;
 }
 
-template <typename T>
+// Staging of the interchanges applied to the panel's other columns.
+constexpr int kSwapStage = 2048;
+
+// One leaf of the panel: columns [0, w) (w <= kPanelWMax) of rows [0, m) at
+// a.A, factored by G workgroups of kPanelThreads threads.  Thread t of
+// workgroup g holds rows g*R + t + q*kPanelThreads (q < ROWS) in registers.
+// The column loop runs over all kPanelWMax columns and rotates every row's
+// registers left by one after each column, so the current column is always
+// register 0 (no run-time register indexing) and after the loop the rows
+// are back in column order.  Columns >= w are zero and only rotated.
+template <typename T, int ROWS>
 __global__ __launch_bounds__(kPanelThreads) void panel_kernel(PanelArgs<T> a) {
-  extern __shared__ __align__(16) unsigned char smem_raw[];
-  T* P = reinterpret_cast<T*>(smem_raw);                     // [w][RP] column-major
-  T* urow = P + (int64_t)a.w * a.RP;                          // [kPanelWMax]
-  int* retired = reinterpret_cast<int*>(urow + kPanelWMax);  // [RP]
+  constexpr int W = kPanelWMax;
+  __shared__ T urow[2 * W];  // winner row of the current column; zeros from w on
   __shared__ unsigned long long s_k1[kPanelThreads / 64], s_k2[kPanelThreads / 64];
-  __shared__ int64_t s_win[kPanelWMax];
-  __shared__ int s_zero[kPanelWMax];
+  __shared__ int64_t s_win[W];
+  __shared__ int s_zero[W];
   __shared__ int s_bad;
   __shared__ MoveList ml;
+  __shared__ T stage[kSwapStage];
 
   const int G = gridDim.x, g = blockIdx.x, tid = threadIdx.x;
   const int w = a.w;
@@ -357,109 +366,148 @@ __global__ __launch_bounds__(kPanelThreads) void panel_kernel(PanelArgs<T> a) {
   const int64_t re = rb + a.R < a.m ? rb + a.R : a.m;
   const int nr = re > rb ? (int)(re - rb) : 0;
 
-  for (int64_t idx = tid; idx < (int64_t)nr * w; idx += kPanelThreads) {
-    const int64_t r = idx / w, c = idx % w;
-    P[c * a.RP + r] = a.A[(rb + r) * a.lda + c];
+  T P[ROWS][W];
+  bool act[ROWS];
+#pragma unroll
+  for (int q = 0; q < ROWS; ++q) {
+    const int local = tid + q * kPanelThreads;
+    act[q] = local < nr;
+    const T* src = a.A + (rb + local) * a.lda;
+#pragma unroll
+    for (int c = 0; c < W; ++c) P[q][c] = (act[q] && c < w) ? src[c] : T(0);
   }
-  for (int r = tid; r < nr; r += kPanelThreads) retired[r] = 0;
-  __syncthreads();
+  for (int c = tid; c < 2 * W; c += kPanelThreads) urow[c] = T(0);
 
-  // best active row of this workgroup for column 0 (later columns are scanned
-  // inside the elimination loop)
-  double best = -1.0;
-  int64_t bidx = INT64_MAX;
-  for (int r = tid; r < nr; r += kPanelThreads) argmax_merge(best, bidx, (double)fabs(P[r]), rb + r);
+  // key of this thread's best active row in register 0 (the current column)
+  auto thread_key = [&]() -> unsigned long long {
+    unsigned long long k = 0;
+#pragma unroll
+    for (int q = 0; q < ROWS; ++q)
+      if (act[q]) k = umax64(k, pivot_key((double)fabs(P[q][0]), 0, tid + q * kPanelThreads));
+    return k;
+  };
+  unsigned long long* recs = reinterpret_cast<unsigned long long*>(a.cand);
 
+  // rotate every row left by one: column j+1 into register 0, column j to
+  // the top
+  auto rotate = [&]() {
+#pragma unroll
+    for (int q = 0; q < ROWS; ++q) {
+      const T t0 = P[q][0];
+#pragma unroll
+      for (int c = 0; c + 1 < W; ++c) P[q][c] = P[q][c + 1];
+      P[q][W - 1] = t0;
+    }
+  };
   for (int j = 0; j < w; ++j) {
-    const int par = j & 1;
-    // 1. best active row of this workgroup for column j
-    if (tid == 0) s_bad = 0;
-    const unsigned long long k1 =
-        block_max_key(bidx == INT64_MAX ? 0ull : pivot_key(best, 0, (int)(bidx - rb)), s_k1);
-    const int64_t bi = k1 ? rb + (kKeyMaxRows - 1 - (int)(k1 & (kKeyMaxRows - 1))) : INT64_MAX;
-    const double bv = k1 ? __longlong_as_double((long long)(k1 & kKeyValMask)) : -1.0;
-    // 2. publish it.  The candidate row goes out first; once every thread's
-    //    stores of it have completed, the tagged record
-    //      w0 = bits of |value| with the 16 low mantissa bits set to the tag,
-    //      w1 = tag << 48 | row.
-    //    A reader takes a record only when both words carry the tag of column
-    //    j, so the record needs no separate flag and its two stores no order.
-    const unsigned long long tag = (unsigned long long)((a.seq_base + (unsigned)(j + 1)) & 0xFFFFu);
-    if (bv >= 0.0 && tid < w) st_coh(a.cand_row + ((int64_t)par * G + g) * w + tid, P[(int64_t)tid * a.RP + (bi - rb)]);
-    __builtin_amdgcn_s_waitcnt(0);
-    __syncthreads();
-    unsigned long long* recs = reinterpret_cast<unsigned long long*>(a.cand);
-    if (tid == 0) {
-      unsigned long long* rec = recs + kRecWords * ((int64_t)par * G + g);
-      st_coh(rec, (unsigned long long)(__double_as_longlong(bv) & ~0xFFFFll) | tag);
-      st_coh(rec + 1, (tag << 48) | ((unsigned long long)bi & 0xFFFFFFFFFFFFull));
-    }
-    // 3.-4. every workgroup's record of column j (one thread each) and the
-    //    winner across workgroups
-    unsigned long long k2 = 0;
-    for (int q = tid; q < G; q += kPanelThreads) {
-      const unsigned long long* o = recs + kRecWords * ((int64_t)par * G + q);
-      unsigned long long w0 = ld_coh(o), w1 = ld_coh(o + 1);
-      uint32_t spins = 0;
-      while ((w0 & 0xFFFFull) != tag || (w1 >> 48) != tag) {
-        __builtin_amdgcn_s_sleep(1);
-        ++spins;
-        if ((spins & 255u) == 0 &&
-            __hip_atomic_load(a.status, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) {
-          s_bad = 1;
-          break;
-        }
-        if (spins > kSpinLimit) {
-          __hip_atomic_store(a.status, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-          s_bad = 1;
-          break;
-        }
-        w0 = ld_coh(o);
-        w1 = ld_coh(o + 1);
+    {
+      const int par = j & 1;
+      const unsigned long long tag = (unsigned long long)((a.seq_base + (unsigned)(j + 1)) & 0xFFFFu);
+      // 1. best active row of this workgroup for column j
+      if (tid == 0) s_bad = 0;
+      const unsigned long long k1 = block_max_key(thread_key(), s_k1);
+      const int blocal = kKeyMaxRows - 1 - (int)(k1 & (kKeyMaxRows - 1));
+      // 2. publish it.  The owner of that row stores the row (absolute column
+      //    order), waits until the stores have completed, then stores the
+      //    tagged record
+      //      w0 = bits of |value| with the 16 low mantissa bits set to the tag,
+      //      w1 = tag << 48 | row.
+      //    A reader takes a record only when both words carry the tag of
+      //    column j, so the record needs no separate flag and its two stores
+      //    no order.  With no candidate, thread 0 publishes value -1.
+      const bool owner = k1 != 0 && (blocal % kPanelThreads) == tid;
+      if (owner) {
+        T* dst = a.cand_row + ((int64_t)par * G + g) * w;
+        const int qo = blocal / kPanelThreads;
+#pragma unroll
+        for (int q = 0; q < ROWS; ++q)
+          if (q == qo) {
+#pragma unroll
+            for (int c = 0; c < W; ++c)
+              if (j + c < w) st_coh(dst + j + c, P[q][c]);
+          }
+        __builtin_amdgcn_s_waitcnt(0);
       }
-      const double ov = __longlong_as_double((long long)(w0 & ~0xFFFFull));
-      if (ov >= 0.0)
-        k2 = umax64(k2, pivot_key(ov, q, (int)((int64_t)(w1 & 0xFFFFFFFFFFFFull) - (int64_t)q * a.R)));
-    }
-    k2 = block_max_key(k2, s_k2);
-    if (s_bad) return;
-    const int wq = kKeyMaxGroups - 1 - (int)((k2 >> 11) & (kKeyMaxGroups - 1));
-    const int64_t wi = (int64_t)wq * a.R + (kKeyMaxRows - 1 - (int)(k2 & (kKeyMaxRows - 1)));
-    if (k2 != 0 && (wq >= G || wi >= a.m)) {
-      // a key that names no row cannot come from the records: stop
-      if (tid == 0) __hip_atomic_store(a.status, 3, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      return;
-    }
-    if (k2 == 0) {
-      // no finite candidate anywhere (NaN column): every workgroup sees the
-      // same key, so all of them leave here together
-      if (g == 0 && tid == 0) {
-        *a.info = -1;
-        __hip_atomic_store(a.status, 2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      if (owner || (k1 == 0 && tid == 0)) {
+        const double bv = k1 ? __longlong_as_double((long long)(k1 & kKeyValMask)) : -1.0;
+        const int64_t bi = k1 ? rb + blocal : INT64_MAX;
+        unsigned long long* rec = recs + kRecWords * ((int64_t)par * G + g);
+        st_coh(rec, (unsigned long long)(__double_as_longlong(bv) & ~0xFFFFll) | tag);
+        st_coh(rec + 1, (tag << 48) | ((unsigned long long)bi & 0xFFFFFFFFFFFFull));
       }
-      return;
-    }
-    if (tid < w) urow[tid] = ld_coh(a.cand_row + ((int64_t)par * G + wq) * w + tid);
-    if (tid == 0) {
-      s_win[j] = wi;
-      if (wi >= rb && wi < re) retired[wi - rb] = 1;
-    }
-    __syncthreads();
-    // 5. eliminate column j from this workgroup's active rows, tracking the
-    //    best candidate of column j+1 on the way
-    const T piv = urow[j];
-    const T rpiv = piv != T(0) ? T(1) / piv : T(0);
-    if (tid == 0) s_zero[j] = piv == T(0);
-    best = -1.0;
-    bidx = INT64_MAX;
-    for (int r = tid; r < nr; r += kPanelThreads) {
-      if (retired[r]) continue;
-      const T l = P[(int64_t)j * a.RP + r] * rpiv;
-      P[(int64_t)j * a.RP + r] = l;
-      for (int c = j + 1; c < w; ++c) P[(int64_t)c * a.RP + r] -= l * urow[c];
-      if (j + 1 < w) argmax_merge(best, bidx, (double)fabs(P[(int64_t)(j + 1) * a.RP + r]), rb + r);
+      // 3.-4. every workgroup's record of column j (one thread each) and the
+      //    winner across workgroups
+      unsigned long long k2 = 0;
+      for (int q = tid; q < G; q += kPanelThreads) {
+        const unsigned long long* o = recs + kRecWords * ((int64_t)par * G + q);
+        unsigned long long w0 = ld_coh(o), w1 = ld_coh(o + 1);
+        uint32_t spins = 0;
+        while ((w0 & 0xFFFFull) != tag || (w1 >> 48) != tag) {
+          __builtin_amdgcn_s_sleep(1);
+          ++spins;
+          if ((spins & 255u) == 0 &&
+              __hip_atomic_load(a.status, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) {
+            s_bad = 1;
+            break;
+          }
+          if (spins > kSpinLimit) {
+            __hip_atomic_store(a.status, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            s_bad = 1;
+            break;
+          }
+          w0 = ld_coh(o);
+          w1 = ld_coh(o + 1);
+        }
+        const double ov = __longlong_as_double((long long)(w0 & ~0xFFFFull));
+        if (ov >= 0.0)
+          k2 = umax64(k2, pivot_key(ov, q, (int)((int64_t)(w1 & 0xFFFFFFFFFFFFull) - (int64_t)q * a.R)));
+      }
+      k2 = block_max_key(k2, s_k2);
+      if (s_bad) return;
+      const int wq = kKeyMaxGroups - 1 - (int)((k2 >> 11) & (kKeyMaxGroups - 1));
+      const int64_t wi = (int64_t)wq * a.R + (kKeyMaxRows - 1 - (int)(k2 & (kKeyMaxRows - 1)));
+      if (k2 != 0 && (wq >= G || wi >= a.m)) {
+        // a key that names no row cannot come from the records: stop
+        if (tid == 0) __hip_atomic_store(a.status, 3, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        return;
+      }
+      if (k2 == 0) {
+        // no finite candidate anywhere (NaN column): every workgroup sees the
+        // same key, so all of them leave here together
+        if (g == 0 && tid == 0) {
+          *a.info = -1;
+          __hip_atomic_store(a.status, 2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+        return;
+      }
+      if (tid < w) urow[tid] = ld_coh(a.cand_row + ((int64_t)par * G + wq) * w + tid);
+      if (tid == 0) s_win[j] = wi;
+#pragma unroll
+      for (int q = 0; q < ROWS; ++q)
+        if (rb + tid + q * kPanelThreads == wi) act[q] = false;
+      __syncthreads();
+      // 5. eliminate column j from the active rows: register c holds column
+      //    j + c (urow beyond w is zero, which also leaves the finished
+      //    columns rotated to the top untouched)
+      const T piv = urow[j];
+      const T rpiv = piv != T(0) ? T(1) / piv : T(0);
+      if (tid == 0) s_zero[j] = piv == T(0);
+      // Branch-free: an inactive row takes multiplier 0 (a divergent branch
+      // here makes the compiler keep a second copy of the row registers).
+      // Eliminated and rotated in one pass: register c takes column j+1+c,
+      // the top register the multiplier of column j.
+#pragma unroll
+      for (int q = 0; q < ROWS; ++q) {
+        const T p0 = P[q][0];
+        const T l = act[q] ? p0 * rpiv : T(0);
+#pragma unroll
+        for (int c = 0; c + 1 < W; ++c) P[q][c] = P[q][c + 1] - l * urow[j + 1 + c];
+        P[q][W - 1] = act[q] ? l : p0;
+      }
     }
   }
+  // the remaining rotations bring the columns back into register order
+  for (int j = w; j < W; ++j) rotate();
 
   // Replay the interchanges (LAPACK order: at step j rows at positions j and
   // pos(winner) swap) to find every row's final position; wave 0 only.
@@ -499,49 +547,47 @@ __global__ __launch_bounds__(kPanelThreads) void panel_kernel(PanelArgs<T> a) {
     }
   }
   __syncthreads();
-  // final position of every row of this workgroup (kept in the retired[]
-  // slots, no longer needed), then a row-contiguous write-back
+  // every row of this workgroup to its final position
   const int cnt = ml.cnt;
-  for (int r = tid; r < nr; r += kPanelThreads) {
-    int64_t dest = rb + r;
+#pragma unroll
+  for (int q = 0; q < ROWS; ++q) {
+    const int local = tid + q * kPanelThreads;
+    if (local >= nr) continue;
+    int64_t dest = rb + local;
     for (int k = 0; k < cnt; ++k)
-      if (ml.phys[k] == rb + r) {
+      if (ml.phys[k] == rb + local) {
         dest = ml.pos[k];
         break;
       }
-    retired[r] = (int)dest;
-  }
-  __syncthreads();
-  for (int64_t idx = tid; idx < (int64_t)nr * w; idx += kPanelThreads) {
-    const int64_t r = idx / w, c = idx % w;
-    a.A[(int64_t)retired[r] * a.lda + c] = P[c * a.RP + r];
+    T* dst = a.A + dest * a.lda;
+#pragma unroll
+    for (int c = 0; c < W; ++c)
+      if (c < w) dst[c] = P[q][c];
   }
   // The same interchanges on the outer panel's other columns (left and right
   // of this leaf), so the recursion above needs no row swaps of its own.
   // Each workgroup takes a disjoint slice of those columns, reads the moved
-  // rows of its slice into LDS (P is free again), then writes them to their
-  // final positions.
+  // rows of its slice into LDS, then writes them to their final positions.
   const int64_t nout = a.left + a.right;
   if (cnt == 0 || nout == 0) return;
-  __syncthreads();
   const int64_t per = (nout + G - 1) / G;
   const int64_t o0 = (int64_t)g * per;
   const int64_t o1 = o0 + per < nout ? o0 + per : nout;
-  const int64_t chunk = (int64_t)w * a.RP / cnt;  // columns per pass that fit P
+  const int64_t chunk = kSwapStage / cnt;  // columns per pass that fit the stage
   for (int64_t c0 = o0; c0 < o1; c0 += chunk) {
     const int64_t nc = c0 + chunk < o1 ? chunk : o1 - c0;
     for (int64_t e = tid; e < (int64_t)cnt * nc; e += kPanelThreads) {
       const int k = (int)(e / nc);
       const int64_t oc = c0 + e % nc;
       const int64_t col = oc < a.left ? oc - a.left : oc - a.left + w;
-      P[e] = a.A[ml.phys[k] * a.lda + col];
+      stage[e] = a.A[ml.phys[k] * a.lda + col];
     }
     __syncthreads();
     for (int64_t e = tid; e < (int64_t)cnt * nc; e += kPanelThreads) {
       const int k = (int)(e / nc);
       const int64_t oc = c0 + e % nc;
       const int64_t col = oc < a.left ? oc - a.left : oc - a.left + w;
-      a.A[ml.pos[k] * a.lda + col] = P[e];
+      a.A[ml.pos[k] * a.lda + col] = stage[e];
     }
     __syncthreads();
   }
@@ -775,42 +821,28 @@ struct LuCtx {
   unsigned seq;     // steps taken by the panel launches so far
   LuScratch<T> sc;
   int G_max;
-  int wmax;
-  size_t lds_budget;
   int64_t pc0 = 0;  // outer panel being factored: columns [pc0, pc0 + pw)
   int pw = 0;
 };
 
-// Static LDS of panel_kernel (argmax slots, winners, move list), kept free
-// of the dynamic panel storage.
-constexpr size_t kPanelStaticLds = 8192;
-
-static int panel_width_max(int64_t n, int bytes, const DeviceInfo& di, size_t lds_budget) {
-  // widest leaf whose LDS share (rows per workgroup x w) fits with num_cus groups
-  for (int w = kPanelWMax; w >= 8; w /= 2) {
-    const int64_t R = ceil_div(n, di.num_cus);
-    const size_t need = (size_t)w * (size_t)(R + 1) * bytes + kPanelWMax * bytes + (size_t)(R + 1) * 4;
-    if (need + kPanelStaticLds <= lds_budget) return w;
-  }
-  return 0;
-}
+// Upper bound on the workgroups of one leaf launch (0: the CU count).
+static std::atomic<int> g_panel_groups{0};
+constexpr int kPanelMaxRows = 3;  // rows per thread of the register-resident leaf
 
 template <typename T>
 static int leaf(LuCtx<T>& c, int64_t r0, int w) {
   const DeviceInfo& di = device_info();
   const int64_t m = c.n - r0;
-  auto need = [&](int64_t Rr) {
-    const int64_t RP = Rr + 1;
-    return (size_t)w * RP * sizeof(T) + kPanelWMax * sizeof(T) + (size_t)RP * 4;
-  };
-  const size_t budget = c.lds_budget - kPanelStaticLds;  // room for the kernel's static LDS
-  // As many rows per workgroup as the LDS budget takes: every column's
-  // exchange polls all G records, so fewer workgroups exchange faster.
-  int64_t Rmax = 1;
-  while (need(Rmax + 1) <= budget) ++Rmax;
-  int G = (int)std::min<int64_t>(di.num_cus, ceil_div(m, Rmax));
-  int64_t R = ceil_div(m, G);
-  RBFX_REQUIRE(need(R) <= budget, "panel does not fit in LDS (m=%lld, w=%d)", (long long)m, w);
+  const int cap = g_panel_groups.load(std::memory_order_relaxed);
+  const int64_t gmax =
+      std::min<int64_t>(cap > 0 ? std::min(cap, di.num_cus) : di.num_cus, kKeyMaxGroups - 1);
+  // As few workgroups as one row per thread allows (every column's exchange
+  // polls all G records); more rows per thread only when the CUs run out.
+  const int rows = (int)ceil_div(m, (int64_t)kPanelThreads * gmax);
+  RBFX_REQUIRE(rows <= kPanelMaxRows, "matrix too large for the register-resident panel (m=%lld)",
+               (long long)m);
+  int G = (int)ceil_div(m, (int64_t)kPanelThreads * rows);
+  const int64_t R = ceil_div(m, G);
   G = (int)ceil_div(m, R);
   PanelArgs<T> a;
   a.A = c.A + r0 * c.lda + r0;
@@ -818,31 +850,33 @@ static int leaf(LuCtx<T>& c, int64_t r0, int w) {
   a.m = m;
   a.w = w;
   a.R = R;
-  a.RP = R + 1;
   a.r0 = r0;
   a.left = r0 - c.pc0;
   a.right = c.pc0 + c.pw - (r0 + w);
   RBFX_REQUIRE(a.left >= 0 && a.right >= 0, "leaf outside its outer panel");
+  RBFX_REQUIRE(w >= 1 && w <= kPanelWMax, "leaf width %d out of range", w);
   a.ipiv = c.ipiv + r0;
   a.cand = c.cand;
   a.cand_row = c.cand_row;
   a.seq_base = c.seq;
   a.status = c.status;
   a.info = c.info;
-  RBFX_REQUIRE(G <= c.G_max && G <= kKeyMaxGroups && R < kKeyMaxRows, "too many panel workgroups or rows");
-  const size_t smem = need(R);
+  RBFX_REQUIRE(G <= c.G_max && G < kKeyMaxGroups && R <= (int64_t)kPanelThreads * rows && R < kKeyMaxRows,
+               "too many panel workgroups or rows");
   const int pid = prof_start(sizeof(T) == 8 ? "panel_f64" : "panel_f32", c.s);
-  if (smem > 65536)
-    RBFX_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&panel_kernel<T>),
-                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem));
   // A plain launch on the panel stream.  The workgroups wait for each
   // other, which needs all G of them resident at once: G is at most the CU
-  // count and each workgroup fits a CU on its own, so the ones not yet
-  // resident start as soon as the trailing-update workgroups on some CU
-  // retire; every wait is bounded in any case.  (A cooperative launch
-  // guarantees the same but goes through a separate device queue, which
-  // added ~60 us of dispatch latency per leaf, measured.)
-  hipLaunchKernelGGL((panel_kernel<T>), dim3(G), dim3(kPanelThreads), smem, c.s, a);
+  // count and a workgroup fits an empty CU, so the ones not yet resident
+  // start as soon as the trailing-update workgroups on some CU retire;
+  // every wait is bounded in any case.  (A cooperative launch guarantees
+  // the same but goes through a separate device queue, which added ~60 us
+  // of dispatch latency per leaf, measured.)
+  if (rows == 1)
+    hipLaunchKernelGGL((panel_kernel<T, 1>), dim3(G), dim3(kPanelThreads), 0, c.s, a);
+  else if (rows == 2)
+    hipLaunchKernelGGL((panel_kernel<T, 2>), dim3(G), dim3(kPanelThreads), 0, c.s, a);
+  else
+    hipLaunchKernelGGL((panel_kernel<T, 3>), dim3(G), dim3(kPanelThreads), 0, c.s, a);
   RBFX_CHECK_LAUNCH();
   prof_stop(pid, c.s, (double)m * w * w, 2.0 * sizeof(T) * m * w);
   c.seq += (unsigned)w;
@@ -928,7 +962,7 @@ static int trsm(LuCtx<T>& c, int64_t r0, int W, const T* X, int64_t ldx, int64_t
 template <typename T>
 static int panel(LuCtx<T>& c, int64_t r0, int W, T* X, int64_t ldx) {
   int st;
-  if (W <= c.wmax) {
+  if (W <= kPanelWMax) {
     if ((st = leaf(c, r0, W))) return st;
     return leaf_inverse(c, r0, W, X, ldx);
   }
@@ -1048,12 +1082,6 @@ static int getrf(int64_t n, T* A, int64_t lda, int64_t* ipiv, void* work, size_t
   RBFX_REQUIRE(ok && c.cand && c.cand_row, "workspace carve failed");
   c.seq = 0;
   c.G_max = G;
-  // A leaf keeps its rows in LDS.  96 KB leaves room for two trailing-update
-  // GEMM workgroups (32 KB each) on the same CU, so the look-ahead panel runs
-  // beside the update.
-  c.lds_budget = std::min<size_t>(di.lds_per_block, 96 * 1024);
-  c.wmax = panel_width_max(n, sizeof(T), di, c.lds_budget);
-  RBFX_REQUIRE(c.wmax > 0, "matrix too large for the LDS-resident panel (n=%lld)", (long long)n);
   hipLaunchKernelGGL(zero_u32_kernel, dim3(1), dim3(64), 0, s, ctl, 16);
   RBFX_CHECK_LAUNCH();
   // pivot records start with tag 0, which no column uses
@@ -1146,6 +1174,12 @@ int gemm_f32(int64_t m, int64_t n, int64_t k, float alpha, const float* A, int64
 }  // namespace rbfx
 
 using namespace rbfx;
+
+extern "C" int rbfx_set_panel_groups(int max_groups) {
+  RBFX_REQUIRE(max_groups >= 0, "max_groups must be >= 0");
+  g_panel_groups.store(max_groups, std::memory_order_relaxed);
+  return RBFX_OK;
+}
 
 extern "C" size_t rbfx_getrf_workspace(int64_t n, int dtype_bytes) {
   if (n <= 0 || (dtype_bytes != 4 && dtype_bytes != 8)) return 0;
