@@ -16,17 +16,21 @@ def _lib():
 
 
 @pytest.mark.parametrize("dtype", [torch.float64, torch.float32])
-@pytest.mark.parametrize("m,n,k", [(1, 1, 1), (130, 70, 17), (300, 257, 64), (512, 384, 256)])
-def test_gemm_matches_torch(dtype, m, n, k):
+@pytest.mark.parametrize("m,n,k,beta", [(1, 1, 1, 1.0), (130, 70, 17, 1.0), (300, 257, 64, 1.0),
+                                         (512, 384, 256, 1.0), (300, 258, 64, 1.0), (130, 72, 16, 0.0),
+                                         (1000, 520, 512, 1.0), (257, 8, 8, 0.0), (129, 136, 32, 1.0)])
+def test_gemm_matches_torch(dtype, m, n, k, beta):
+    """Both GEMM paths: LDS-DMA (k a multiple of the k-tile, n of 16 bytes)
+    and register-staged (the rest), with partial tiles on every edge."""
     L = _lib()
     g = torch.Generator().manual_seed(m * 7 + n + k)
     A = torch.randn(m, k, generator=g, dtype=torch.float64)
     B = torch.randn(k, n, generator=g, dtype=torch.float64)
     C = torch.randn(m, n, generator=g, dtype=torch.float64)
-    ref = C - A @ B
+    ref = beta * C - A @ B
     dA, dB, dC = (t.to(dtype).cuda().contiguous() for t in (A, B, C))
     fn = L.lib().rbfx_gemm_f64 if dtype == torch.float64 else L.lib().rbfx_gemm_f32
-    L.check(fn(m, n, k, -1.0, L.ptr(dA), k, L.ptr(dB), n, 1.0, L.ptr(dC), n, L.stream()))
+    L.check(fn(m, n, k, -1.0, L.ptr(dA), k, L.ptr(dB), n, beta, L.ptr(dC), n, L.stream()))
     torch.cuda.synchronize()
     tol = 1e-12 if dtype == torch.float64 else 2e-4
     err = (dC.double().cpu() - ref).abs().max().item() / max(1.0, ref.abs().max().item())

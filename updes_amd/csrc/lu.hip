@@ -207,6 +207,158 @@ __global__ __launch_bounds__(GTHREADS) __attribute__((amdgpu_waves_per_eu(2, 2))
   }
 }
 
+// GEMM with LDS-DMA operand loads (global_load_lds_dwordx4): the k-tiles go
+// from global memory straight into a ring of kGemmStages LDS stages, so no
+// registers stage them and a tile is requested kGemmStages-1 tiles before it
+// is multiplied.  One barrier per k-tile.
+//
+// Layouts (E = 16 / sizeof(T) elements per 16-byte chunk):
+//   A stage: 128 rows x BK, chunk (m, g) = A[m][E g .. E g + E) at position
+//     4 m + (g ^ ((m >> 2) & 3))  (the xor keeps the 16-lane groups of the
+//     operand reads on distinct banks);
+//   B stage: BK rows x 128, chunk (k, q) = B[k][E q .. E q + E) at k*(128/E) + q.
+// MFMA k-assignment: lane group g = lane/16 of MFMA s (0 <= s < E) holds
+// k = E g + s, so one 16-byte read gives a lane its A values of all E MFMAs.
+// For B one 16-byte read gives E adjacent columns, which feed E adjacent
+// n-tiles: tile j's column for lane column c is 16E (j/E) + E c + j%E.
+// Requirements (checked by the host): k % BK == 0, n % E == 0, lda, ldb and
+// ldc multiples of E, 16-byte aligned A, B and C.
+constexpr int kGemmStages = 4;
+
+template <typename T, int USE>
+__global__ __launch_bounds__(GTHREADS) __attribute__((amdgpu_waves_per_eu(2, 2))) void gemm_dma_kernel(
+    int64_t m, int64_t n, int64_t k, T alpha, const T* __restrict__ A, int64_t lda, const T* __restrict__ B,
+    int64_t ldb, T beta, T* __restrict__ C, int64_t ldc, int64_t tiles_m, int64_t tiles_n) {
+  constexpr int BK = Mma<T>::BK;
+  constexpr int E = 16 / (int)sizeof(T);
+  constexpr int NG = GBN / E;  // chunks per B row
+  static_assert(4 * E == BK, "one 16-byte chunk per lane group and k-tile");
+  static_assert(GBM * (BK / E) == 512 && BK * NG == 512, "512 chunks per operand and stage");
+  typedef T vec_t __attribute__((ext_vector_type(E)));
+  __shared__ __align__(16) T sm[kGemmStages][2][512 * E];
+
+  // tile order as in gemm_kernel (groups of 8 tile rows, XCD-contiguous)
+  const int64_t T_ = tiles_m * tiles_n;
+  const int64_t L = blockIdx.x;
+  const int64_t per = (T_ + 7) / 8;
+  const int64_t tile = (L % 8) * per + L / 8;
+  if (tile >= T_) return;
+  constexpr int64_t kGroupM = 8;
+  const int64_t gwidth = kGroupM * tiles_n;
+  const int64_t first_m = (tile / gwidth) * kGroupM;
+  const int64_t gm = tiles_m - first_m < kGroupM ? tiles_m - first_m : kGroupM;
+  const int64_t tm = first_m + (tile % gwidth) % gm;
+  const int64_t tn = (tile % gwidth) / gm;
+  const int64_t row0 = tm * GBM, col0 = tn * GBN;
+
+  const int tid = threadIdx.x;
+  const int lane = tid & 63, wid = tid >> 6;
+  const int wm = wid >> 1, wn = wid & 1;
+  const int lg = lane >> 4, lc = lane & 15;
+
+  // DMA sources of this thread (two A chunks, two B chunks per k-tile); rows
+  // past m are clamped (their results are never stored)
+  const T* srcA[2];
+  const T* srcB[2];
+#pragma unroll
+  for (int r = 0; r < 2; ++r) {
+    const int p = (r * 4 + wid) * 64 + lane;
+    const int mm = p >> 2, g = (p & 3) ^ ((mm >> 2) & 3);
+    const int64_t row = row0 + mm < m ? row0 + mm : m - 1;
+    srcA[r] = A + row * lda + g * E;
+    const int kk = p / NG, q = p % NG;
+    const int64_t col = col0 + (int64_t)q * E < n ? col0 + (int64_t)q * E : n - E;
+    srcB[r] = B + (int64_t)kk * ldb + col;
+  }
+  auto issue = [&](int64_t kt, int st) {
+#pragma unroll
+    for (int r = 0; r < 2; ++r) {
+      const int base = (r * 4 + wid) * 64 * E;
+      __builtin_amdgcn_global_load_lds((const void*)(srcA[r] + kt),
+                                       (__attribute__((address_space(3))) void*)(&sm[st][0][base]), 16, 0, 0);
+      __builtin_amdgcn_global_load_lds((const void*)(srcB[r] + kt * ldb),
+                                       (__attribute__((address_space(3))) void*)(&sm[st][1][base]), 16, 0, 0);
+    }
+  };
+
+  typename Mma<T>::acc_t acc[4][4];
+  const bool add = beta != T(0);
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      const int64_t r = row0 + wm * 64 + i * 16 + Mma<T>::row(lane, v);
+#pragma unroll
+      for (int jg = 0; jg < 4 / E; ++jg) {
+        const int64_t c = col0 + wn * 64 + 16 * E * jg + E * lc;
+        vec_t x = {};
+        if (add && r < m && c < n) x = *reinterpret_cast<const vec_t*>(C + r * ldc + c);
+#pragma unroll
+        for (int e = 0; e < E; ++e) acc[i][jg * E + e][v] = x[e];
+      }
+    }
+
+  const int64_t nt = k / BK;
+#pragma unroll
+  for (int t = 0; t < kGemmStages - 1; ++t)
+    if (t < nt) issue((int64_t)t * BK, t);
+
+  for (int64_t t = 0; t < nt; ++t) {
+    // tile t has landed once at most (tiles issued after it) x 4 loads of
+    // this thread are outstanding; the barrier makes every thread's part
+    // visible and ends all reads of the stage refilled below
+    const int64_t ahead = nt - 1 - t < kGemmStages - 2 ? nt - 1 - t : kGemmStages - 2;
+    if (ahead >= 2)
+      __builtin_amdgcn_s_waitcnt(0xF70 | 8);
+    else if (ahead == 1)
+      __builtin_amdgcn_s_waitcnt(0xF70 | 4);
+    else
+      __builtin_amdgcn_s_waitcnt(0xF70);
+    // A plain barrier: __syncthreads() would also wait for the loads of the
+    // tiles still in flight (its release fence covers the LDS-DMA).  Every
+    // LDS read of this wave has completed (the MFMAs consumed them).
+    __asm__ volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
+    if (t + kGemmStages - 1 < nt) issue((t + kGemmStages - 1) * BK, (int)((t + kGemmStages - 1) % kGemmStages));
+    const int st = (int)(t % kGemmStages);
+    const T* As = sm[st][0];
+    const T* Bs = sm[st][1];
+    vec_t a[4], b[4 / E][E];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const int mm = wm * 64 + i * 16 + lc;
+      a[i] = *reinterpret_cast<const vec_t*>(As + (mm * 4 + (lg ^ ((mm >> 2) & 3))) * E);
+    }
+#pragma unroll
+    for (int jg = 0; jg < 4 / E; ++jg)
+#pragma unroll
+      for (int s2 = 0; s2 < E; ++s2)
+        b[jg][s2] = *reinterpret_cast<const vec_t*>(Bs + ((E * lg + s2) * NG + wn * 64 / E + 16 * jg + lc) * E);
+#pragma unroll
+    for (int s2 = 0; s2 < E; ++s2)
+#pragma unroll
+      for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) acc[i][j] = Mma<T>::mma(alpha * a[i][s2], b[j / E][s2][j % E], acc[i][j]);
+  }
+
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      const int64_t r = row0 + wm * 64 + i * 16 + Mma<T>::row(lane, v);
+      if (r >= m) continue;
+#pragma unroll
+      for (int jg = 0; jg < 4 / E; ++jg) {
+        const int64_t c = col0 + wn * 64 + 16 * E * jg + E * lc;
+        if (c >= n) continue;
+        vec_t x;
+#pragma unroll
+        for (int e = 0; e < E; ++e) x[e] = acc[i][jg * E + e][v];
+        *reinterpret_cast<vec_t*>(C + r * ldc + c) = x;
+      }
+    }
+}
+
 template <typename T, int USE = 0>
 static int gemm_launch(int64_t m, int64_t n, int64_t k, T alpha, const T* A, int64_t lda, const T* B,
                        int64_t ldb, T beta, T* C, int64_t ldc, hipStream_t s) {
@@ -214,8 +366,16 @@ static int gemm_launch(int64_t m, int64_t n, int64_t k, T alpha, const T* A, int
   const int64_t tm = ceil_div(m, GBM), tn = ceil_div(n, GBN);
   const int64_t tiles = tm * tn;
   const int64_t grid = ((tiles + 7) / 8) * 8;
-  hipLaunchKernelGGL((gemm_kernel<T, USE>), dim3((unsigned)grid), dim3(GTHREADS), 0, s, m, n, k, alpha, A,
-                     lda, B, ldb, beta, C, ldc, tm, tn);
+  constexpr int E = 16 / (int)sizeof(T);
+  auto al16 = [](const void* p) { return (reinterpret_cast<uintptr_t>(p) & 15) == 0; };
+  const bool dma = k > 0 && k % Mma<T>::BK == 0 && n >= E && n % E == 0 && lda % E == 0 && ldb % E == 0 &&
+                   ldc % E == 0 && al16(A) && al16(B) && al16(C);
+  if (dma)
+    hipLaunchKernelGGL((gemm_dma_kernel<T, USE>), dim3((unsigned)grid), dim3(GTHREADS), 0, s, m, n, k, alpha, A,
+                       lda, B, ldb, beta, C, ldc, tm, tn);
+  else
+    hipLaunchKernelGGL((gemm_kernel<T, USE>), dim3((unsigned)grid), dim3(GTHREADS), 0, s, m, n, k, alpha, A,
+                       lda, B, ldb, beta, C, ldc, tm, tn);
   RBFX_CHECK_LAUNCH();
   return RBFX_OK;
 }
