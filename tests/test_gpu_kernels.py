@@ -37,6 +37,20 @@ def test_gemm_matches_torch(dtype, m, n, k, beta):
     assert err < tol, err
 
 
+@pytest.mark.parametrize("alpha", [0.5, 1.0])
+def test_gemm_alpha(alpha):
+    """alpha = +-1 takes the LDS-DMA kernel, other values the register-staged one."""
+    L = _lib()
+    m, n, k = 260, 136, 64
+    g = torch.Generator().manual_seed(5)
+    A, B, C = (torch.randn(*sh, generator=g, dtype=torch.float64) for sh in ((m, k), (k, n), (m, n)))
+    ref = C + alpha * (A @ B)
+    dA, dB, dC = (t.cuda().contiguous() for t in (A, B, C))
+    L.check(L.lib().rbfx_gemm_f64(m, n, k, alpha, L.ptr(dA), k, L.ptr(dB), n, 1.0, L.ptr(dC), n, L.stream()))
+    torch.cuda.synchronize()
+    assert (dC.cpu() - ref).abs().max().item() < 1e-12 * ref.abs().max().item()
+
+
 def _random_well(n, seed):
     rng = np.random.default_rng(seed)
     a = rng.standard_normal((n, n))

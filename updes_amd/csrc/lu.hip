@@ -283,6 +283,8 @@ __global__ __launch_bounds__(GTHREADS) __attribute__((amdgpu_waves_per_eu(2, 2))
 
   typename Mma<T>::acc_t acc[4][4];
   const bool add = beta != T(0);
+  // alpha is +-1 here: the sum runs on alpha*C and alpha is applied at the end
+  const bool neg = alpha < T(0);
 #pragma unroll
   for (int i = 0; i < 4; ++i)
 #pragma unroll
@@ -294,7 +296,7 @@ __global__ __launch_bounds__(GTHREADS) __attribute__((amdgpu_waves_per_eu(2, 2))
         vec_t x = {};
         if (add && r < m && c < n) x = *reinterpret_cast<const vec_t*>(C + r * ldc + c);
 #pragma unroll
-        for (int e = 0; e < E; ++e) acc[i][jg * E + e][v] = x[e];
+        for (int e = 0; e < E; ++e) acc[i][jg * E + e][v] = neg ? -x[e] : x[e];
       }
     }
 
@@ -338,7 +340,7 @@ __global__ __launch_bounds__(GTHREADS) __attribute__((amdgpu_waves_per_eu(2, 2))
 #pragma unroll
       for (int i = 0; i < 4; ++i)
 #pragma unroll
-        for (int j = 0; j < 4; ++j) acc[i][j] = Mma<T>::mma(alpha * a[i][s2], b[j / E][s2][j % E], acc[i][j]);
+        for (int j = 0; j < 4; ++j) acc[i][j] = Mma<T>::mma(a[i][s2], b[j / E][s2][j % E], acc[i][j]);
   }
 
 #pragma unroll
@@ -353,7 +355,7 @@ __global__ __launch_bounds__(GTHREADS) __attribute__((amdgpu_waves_per_eu(2, 2))
         if (c >= n) continue;
         vec_t x;
 #pragma unroll
-        for (int e = 0; e < E; ++e) x[e] = acc[i][jg * E + e][v];
+        for (int e = 0; e < E; ++e) x[e] = neg ? -acc[i][jg * E + e][v] : acc[i][jg * E + e][v];
         *reinterpret_cast<vec_t*>(C + r * ldc + c) = x;
       }
     }
@@ -368,7 +370,7 @@ static int gemm_launch(int64_t m, int64_t n, int64_t k, T alpha, const T* A, int
   const int64_t grid = ((tiles + 7) / 8) * 8;
   constexpr int E = 16 / (int)sizeof(T);
   auto al16 = [](const void* p) { return (reinterpret_cast<uintptr_t>(p) & 15) == 0; };
-  const bool dma = k > 0 && k % Mma<T>::BK == 0 && n >= E && n % E == 0 && lda % E == 0 && ldb % E == 0 &&
+  const bool dma = (alpha == T(1) || alpha == T(-1)) && k > 0 && k % Mma<T>::BK == 0 && n >= E && n % E == 0 && lda % E == 0 && ldb % E == 0 &&
                    ldc % E == 0 && al16(A) && al16(B) && al16(C);
   if (dma)
     hipLaunchKernelGGL((gemm_dma_kernel<T, USE>), dim3((unsigned)grid), dim3(GTHREADS), 0, s, m, n, k, alpha, A,
