@@ -12,13 +12,15 @@
 //      the leaf in LDS and agree on every pivot through one exchange per
 //      column (every workgroup publishes its best row and a tagged record,
 //      all of them poll the records and pick the same winner);
-//   2. row interchanges of the other columns through a gather/scatter plan
-//      (row-major rows are contiguous, so a swap is a coalesced copy);
-//   3. U12 = inv(L11) A12: the unit-lower diagonal block is inverted (64 x 64
-//      diagonal inverses + block recurrence) and applied with the MFMA GEMM;
+//      the leaves apply their interchanges to the panel's other columns and
+//      invert their diagonal blocks; the splits join those inverses;
+//   2. row interchanges of the columns outside the panel through a
+//      gather/scatter plan (row-major rows are contiguous, so a swap is a
+//      coalesced copy);
+//   3. U12 = inv(L11) A12 with the MFMA GEMM;
 //   4. trailing update A22 -= L21 U12 with an MFMA GEMM
 //      (v_mfma_f64_16x16x4f64 / v_mfma_f32_16x16x4f32, 128x128 tiles,
-//      XCD-aware tile order).
+//      k-tiles loaded global -> LDS by DMA, XCD-aware tile order).
 #include <algorithm>
 #include <atomic>
 #include <mutex>
@@ -62,10 +64,11 @@ struct Mma<float> {
 // Block tile 128x128, 4 waves (2x2), wave tile 64x64 = 4x4 MFMA 16x16 tiles.
 constexpr int GBM = 128, GBN = 128, GTHREADS = 256;
 
+// Register-staged version: the k-tiles of A and B are loaded two tiles
+// ahead into two register slots and stored to an LDS double buffer.  It
+// serves the shapes and alpha values the LDS-DMA kernel below does not take.
 // Two waves per SIMD (<= 256 registers each) so one wave's barrier or load
-// wait overlaps the other's MFMAs.  The k-tile (BK) keeps the LDS of one
-// workgroup at 32 KB, so two GEMM workgroups leave 96 KB of a CU's LDS to
-// the panel kernels that run beside the trailing update (look-ahead).
+// wait overlaps the other's MFMAs.
 // USE only separates the kernel symbols of the trailing update (0), of the
 // triangular solve (1) and of the diagonal-block inverse (2) in profiles;
 // the code is the same.
