@@ -7,9 +7,9 @@
 // once: P K = L U.
 //
 // Structure per outer panel of NB columns (k0 .. k0+NB):
-//   1. recursive panel factorisation: halves down to WMAX-wide leaves; each
-//      leaf runs as ONE launch of co-resident workgroups that keep their rows of
-//      the leaf in LDS and agree on every pivot through one exchange per
+//   1. recursive panel factorisation: halves down to 64-wide leaves; each
+//      leaf runs as ONE launch of co-resident workgroups that keep their rows
+//      of the leaf in registers and agree on every pivot through one exchange per
 //      column (every workgroup publishes its best row and a tagged record,
 //      all of them poll the records and pick the same winner);
 //      the leaves apply their interchanges to the panel's other columns and
