@@ -18,10 +18,13 @@ def _lib():
 @pytest.mark.parametrize("dtype", [torch.float64, torch.float32])
 @pytest.mark.parametrize("m,n,k,beta", [(1, 1, 1, 1.0), (130, 70, 17, 1.0), (300, 257, 64, 1.0),
                                          (512, 384, 256, 1.0), (300, 258, 64, 1.0), (130, 72, 16, 0.0),
-                                         (1000, 520, 512, 1.0), (257, 8, 8, 0.0), (129, 136, 32, 1.0)])
+                                         (1000, 520, 512, 1.0), (257, 8, 8, 0.0), (129, 136, 32, 1.0),
+                                         (2944, 2952, 64, 1.0)])
 def test_gemm_matches_torch(dtype, m, n, k, beta):
-    """Both GEMM paths: LDS-DMA (k a multiple of the k-tile, n of 16 bytes)
-    and register-staged (the rest), with partial tiles on every edge."""
+    """All GEMM paths: LDS-DMA with 128 x 128 tiles (products with at least
+    2 x CU-count such tiles, the last shape) or 64 x 64 tiles (smaller FP64
+    products), both needing k a multiple of the k-tile and n of 16 bytes, and
+    register-staged (the rest), with partial tiles on every edge."""
     L = _lib()
     g = torch.Generator().manual_seed(m * 7 + n + k)
     A = torch.randn(m, k, generator=g, dtype=torch.float64)

@@ -228,17 +228,25 @@ __global__ __launch_bounds__(GTHREADS) __attribute__((amdgpu_waves_per_eu(2, 2))
 // ldc multiples of E, 16-byte aligned A, B and C.
 constexpr int kGemmStages = 4;
 
-template <typename T, int USE>
-__global__ __launch_bounds__(GTHREADS) __attribute__((amdgpu_waves_per_eu(2, 2))) void gemm_dma_kernel(
-    int64_t m, int64_t n, int64_t k, T alpha, const T* __restrict__ A, int64_t lda, const T* __restrict__ B,
-    int64_t ldb, T beta, T* __restrict__ C, int64_t ldc, int64_t tiles_m, int64_t tiles_n) {
+// BT x BT block tile (BT = 128, or 64 for small FP64 products), 2 x 2 waves
+// of (BT/2) x (BT/2), each TW x TW MFMA 16x16 tiles.
+template <typename T, int USE, int BT>
+__device__ __forceinline__ void gemm_dma_body(int64_t m, int64_t n, int64_t k, T alpha, const T* __restrict__ A,
+                                              int64_t lda, const T* __restrict__ B, int64_t ldb, T beta,
+                                              T* __restrict__ C, int64_t ldc, int64_t tiles_m, int64_t tiles_n) {
   constexpr int BK = Mma<T>::BK;
   constexpr int E = 16 / (int)sizeof(T);
-  constexpr int NG = GBN / E;  // chunks per B row
+  constexpr int TW = BT / 32;      // MFMA tiles per wave and dimension
+  constexpr int HB = BT / 2;       // wave tile extent
+  constexpr int NG = BT / E;       // chunks per B row
+  constexpr int NCH = 4 * BT;      // chunks per operand and stage
+  constexpr int RC = NCH / GTHREADS;  // chunks per thread and operand
   static_assert(4 * E == BK, "one 16-byte chunk per lane group and k-tile");
-  static_assert(GBM * (BK / E) == 512 && BK * NG == 512, "512 chunks per operand and stage");
+  static_assert(BT * (BK / E) == NCH && BK * NG == NCH, "chunk counts");
+  static_assert(TW % E == 0 || E == 2, "E adjacent n-tiles per 16-byte B read");
+  static_assert(TW >= E, "E adjacent n-tiles per 16-byte B read");
   typedef T vec_t __attribute__((ext_vector_type(E)));
-  __shared__ __align__(16) T sm[kGemmStages][2][512 * E];
+  __shared__ __align__(16) T sm[kGemmStages][2][NCH * E];
 
   // tile order as in gemm_kernel (groups of 8 tile rows, XCD-contiguous)
   const int64_t T_ = tiles_m * tiles_n;
@@ -252,19 +260,19 @@ __global__ __launch_bounds__(GTHREADS) __attribute__((amdgpu_waves_per_eu(2, 2))
   const int64_t gm = tiles_m - first_m < kGroupM ? tiles_m - first_m : kGroupM;
   const int64_t tm = first_m + (tile % gwidth) % gm;
   const int64_t tn = (tile % gwidth) / gm;
-  const int64_t row0 = tm * GBM, col0 = tn * GBN;
+  const int64_t row0 = tm * BT, col0 = tn * BT;
 
   const int tid = threadIdx.x;
   const int lane = tid & 63, wid = tid >> 6;
   const int wm = wid >> 1, wn = wid & 1;
   const int lg = lane >> 4, lc = lane & 15;
 
-  // DMA sources of this thread (two A chunks, two B chunks per k-tile); rows
+  // DMA sources of this thread (RC A chunks, RC B chunks per k-tile); rows
   // past m are clamped (their results are never stored)
-  const T* srcA[2];
-  const T* srcB[2];
+  const T* srcA[RC];
+  const T* srcB[RC];
 #pragma unroll
-  for (int r = 0; r < 2; ++r) {
+  for (int r = 0; r < RC; ++r) {
     const int p = (r * 4 + wid) * 64 + lane;
     const int mm = p >> 2, g = (p & 3) ^ ((mm >> 2) & 3);
     const int64_t row = row0 + mm < m ? row0 + mm : m - 1;
@@ -275,7 +283,7 @@ __global__ __launch_bounds__(GTHREADS) __attribute__((amdgpu_waves_per_eu(2, 2))
   }
   auto issue = [&](int64_t kt, int st) {
 #pragma unroll
-    for (int r = 0; r < 2; ++r) {
+    for (int r = 0; r < RC; ++r) {
       const int base = (r * 4 + wid) * 64 * E;
       __builtin_amdgcn_global_load_lds((const void*)(srcA[r] + kt),
                                        (__attribute__((address_space(3))) void*)(&sm[st][0][base]), 16, 0, 0);
@@ -284,18 +292,18 @@ __global__ __launch_bounds__(GTHREADS) __attribute__((amdgpu_waves_per_eu(2, 2))
     }
   };
 
-  typename Mma<T>::acc_t acc[4][4];
+  typename Mma<T>::acc_t acc[TW][TW];
   const bool add = beta != T(0);
   // alpha is +-1 here: the sum runs on alpha*C and alpha is applied at the end
   const bool neg = alpha < T(0);
 #pragma unroll
-  for (int i = 0; i < 4; ++i)
+  for (int i = 0; i < TW; ++i)
 #pragma unroll
     for (int v = 0; v < 4; ++v) {
-      const int64_t r = row0 + wm * 64 + i * 16 + Mma<T>::row(lane, v);
+      const int64_t r = row0 + wm * HB + i * 16 + Mma<T>::row(lane, v);
 #pragma unroll
-      for (int jg = 0; jg < 4 / E; ++jg) {
-        const int64_t c = col0 + wn * 64 + 16 * E * jg + E * lc;
+      for (int jg = 0; jg < TW / E; ++jg) {
+        const int64_t c = col0 + wn * HB + 16 * E * jg + E * lc;
         vec_t x = {};
         if (add && r < m && c < n) x = *reinterpret_cast<const vec_t*>(C + r * ldc + c);
 #pragma unroll
@@ -309,14 +317,14 @@ __global__ __launch_bounds__(GTHREADS) __attribute__((amdgpu_waves_per_eu(2, 2))
     if (t < nt) issue((int64_t)t * BK, t);
 
   for (int64_t t = 0; t < nt; ++t) {
-    // tile t has landed once at most (tiles issued after it) x 4 loads of
+    // tile t has landed once at most (tiles issued after it) x 2 RC loads of
     // this thread are outstanding; the barrier makes every thread's part
     // visible and ends all reads of the stage refilled below
     const int64_t ahead = nt - 1 - t < kGemmStages - 2 ? nt - 1 - t : kGemmStages - 2;
     if (ahead >= 2)
-      __builtin_amdgcn_s_waitcnt(0xF70 | 8);
+      __builtin_amdgcn_s_waitcnt(0xF70 | (4 * RC));
     else if (ahead == 1)
-      __builtin_amdgcn_s_waitcnt(0xF70 | 4);
+      __builtin_amdgcn_s_waitcnt(0xF70 | (2 * RC));
     else
       __builtin_amdgcn_s_waitcnt(0xF70);
     // A plain barrier: __syncthreads() would also wait for the loads of the
@@ -327,34 +335,34 @@ __global__ __launch_bounds__(GTHREADS) __attribute__((amdgpu_waves_per_eu(2, 2))
     const int st = (int)(t % kGemmStages);
     const T* As = sm[st][0];
     const T* Bs = sm[st][1];
-    vec_t a[4], b[4 / E][E];
+    vec_t a[TW], b[TW / E][E];
 #pragma unroll
-    for (int i = 0; i < 4; ++i) {
-      const int mm = wm * 64 + i * 16 + lc;
+    for (int i = 0; i < TW; ++i) {
+      const int mm = wm * HB + i * 16 + lc;
       a[i] = *reinterpret_cast<const vec_t*>(As + (mm * 4 + (lg ^ ((mm >> 2) & 3))) * E);
     }
 #pragma unroll
-    for (int jg = 0; jg < 4 / E; ++jg)
+    for (int jg = 0; jg < TW / E; ++jg)
 #pragma unroll
       for (int s2 = 0; s2 < E; ++s2)
-        b[jg][s2] = *reinterpret_cast<const vec_t*>(Bs + ((E * lg + s2) * NG + wn * 64 / E + 16 * jg + lc) * E);
+        b[jg][s2] = *reinterpret_cast<const vec_t*>(Bs + ((E * lg + s2) * NG + wn * HB / E + 16 * jg + lc) * E);
 #pragma unroll
     for (int s2 = 0; s2 < E; ++s2)
 #pragma unroll
-      for (int i = 0; i < 4; ++i)
+      for (int i = 0; i < TW; ++i)
 #pragma unroll
-        for (int j = 0; j < 4; ++j) acc[i][j] = Mma<T>::mma(a[i][s2], b[j / E][s2][j % E], acc[i][j]);
+        for (int j = 0; j < TW; ++j) acc[i][j] = Mma<T>::mma(a[i][s2], b[j / E][s2][j % E], acc[i][j]);
   }
 
 #pragma unroll
-  for (int i = 0; i < 4; ++i)
+  for (int i = 0; i < TW; ++i)
 #pragma unroll
     for (int v = 0; v < 4; ++v) {
-      const int64_t r = row0 + wm * 64 + i * 16 + Mma<T>::row(lane, v);
+      const int64_t r = row0 + wm * HB + i * 16 + Mma<T>::row(lane, v);
       if (r >= m) continue;
 #pragma unroll
-      for (int jg = 0; jg < 4 / E; ++jg) {
-        const int64_t c = col0 + wn * 64 + 16 * E * jg + E * lc;
+      for (int jg = 0; jg < TW / E; ++jg) {
+        const int64_t c = col0 + wn * HB + 16 * E * jg + E * lc;
         if (c >= n) continue;
         vec_t x;
 #pragma unroll
@@ -362,6 +370,25 @@ __global__ __launch_bounds__(GTHREADS) __attribute__((amdgpu_waves_per_eu(2, 2))
         *reinterpret_cast<vec_t*>(C + r * ldc + c) = x;
       }
     }
+}
+
+template <typename T, int USE>
+__global__ __launch_bounds__(GTHREADS) __attribute__((amdgpu_waves_per_eu(2, 2))) void gemm_dma_kernel(
+    int64_t m, int64_t n, int64_t k, T alpha, const T* __restrict__ A, int64_t lda, const T* __restrict__ B,
+    int64_t ldb, T beta, T* __restrict__ C, int64_t ldc, int64_t tiles_m, int64_t tiles_n) {
+  gemm_dma_body<T, USE, 128>(m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, tiles_m, tiles_n);
+}
+
+// 64 x 64 tiles for FP64 products too small to fill the GPU with 128 x 128
+// tiles (the in-panel updates, inverses and triangular solves of late
+// panels): four times the workgroups, a quarter of the k-loop work each.
+template <int USE>
+__global__ __launch_bounds__(GTHREADS) void gemm_dma64_kernel(int64_t m, int64_t n, int64_t k, double alpha,
+                                                             const double* __restrict__ A, int64_t lda,
+                                                             const double* __restrict__ B, int64_t ldb,
+                                                             double beta, double* __restrict__ C, int64_t ldc,
+                                                             int64_t tiles_m, int64_t tiles_n) {
+  gemm_dma_body<double, USE, 64>(m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, tiles_m, tiles_n);
 }
 
 template <typename T, int USE = 0>
@@ -375,7 +402,13 @@ static int gemm_launch(int64_t m, int64_t n, int64_t k, T alpha, const T* A, int
   auto al16 = [](const void* p) { return (reinterpret_cast<uintptr_t>(p) & 15) == 0; };
   const bool dma = (alpha == T(1) || alpha == T(-1)) && k > 0 && k % Mma<T>::BK == 0 && n >= E && n % E == 0 && lda % E == 0 && ldb % E == 0 &&
                    ldc % E == 0 && al16(A) && al16(B) && al16(C);
-  if (dma)
+  if (dma && sizeof(T) == 8 && tiles < 2 * (int64_t)device_info().num_cus) {
+    const int64_t tm64 = ceil_div(m, (int64_t)64), tn64 = ceil_div(n, (int64_t)64);
+    const int64_t grid64 = ((tm64 * tn64 + 7) / 8) * 8;
+    hipLaunchKernelGGL((gemm_dma64_kernel<USE>), dim3((unsigned)grid64), dim3(GTHREADS), 0, s, m, n, k,
+                       (double)alpha, reinterpret_cast<const double*>(A), lda, reinterpret_cast<const double*>(B),
+                       ldb, (double)beta, reinterpret_cast<double*>(C), ldc, tm64, tn64);
+  } else if (dma)
     hipLaunchKernelGGL((gemm_dma_kernel<T, USE>), dim3((unsigned)grid), dim3(GTHREADS), 0, s, m, n, k, alpha, A,
                        lda, B, ldb, beta, C, ldc, tm, tn);
   else
