@@ -95,10 +95,13 @@ def test_getrf_getrs(dtype, n):
 
 
 @pytest.mark.parametrize("dtype", [torch.float64, torch.float32])
-@pytest.mark.parametrize("n,groups", [(500, 1), (700, 1), (1301, 2), (1301, 3)])
-def test_getrf_multirow_leaves(dtype, n, groups):
+@pytest.mark.parametrize("n,groups,narrow", [(500, 1, 0), (700, 1, 0), (1301, 2, 0), (1301, 3, 0),
+                                             (1301, 0, 1), (700, 1, 1), (1301, 3, 600)])
+def test_getrf_multirow_leaves(dtype, n, groups, narrow):
     """Leaves with two or three rows per thread (panel launches capped to few
-    workgroups): same pivots as LAPACK and a small P A - L U residual."""
+    workgroups) and 16-wide leaves (narrow: panels of at least that many rows;
+    0 = default, none at these sizes): same pivots as LAPACK and a small
+    P A - L U residual."""
     from updes_amd import engine as E
     L = _lib()
     a = _random_well(n, 11)
@@ -106,11 +109,13 @@ def test_getrf_multirow_leaves(dtype, n, groups):
     m = torch.zeros((n, ld), dtype=dtype)
     m[:, :n] = torch.as_tensor(a).to(dtype)
     L.check(L.lib().rbfx_set_panel_groups(groups))
+    L.check(L.lib().rbfx_set_narrow_leaf_rows(narrow))
     try:
         lu = E.LU(m.cuda(), n)
         torch.cuda.synchronize()
     finally:
         L.check(L.lib().rbfx_set_panel_groups(0))
+        L.check(L.lib().rbfx_set_narrow_leaf_rows(0))
     assert lu.singular == 0
     ipiv = lu.ipiv.cpu().numpy()
     if dtype == torch.float64:

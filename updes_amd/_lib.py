@@ -86,6 +86,7 @@ _SIGS = {
     "rbfx_gemm_f64": (ctypes.c_int, [_i64, _i64, _i64, _d, _p, _i64, _p, _i64, _d, _p, _i64, _p]),
     "rbfx_gemm_f32": (ctypes.c_int, [_i64, _i64, _i64, _f, _p, _i64, _p, _i64, _f, _p, _i64, _p]),
     "rbfx_set_panel_groups": (ctypes.c_int, [ctypes.c_int]),
+    "rbfx_set_narrow_leaf_rows": (ctypes.c_int, [_i64]),
 }
 
 EXPORTED = sorted(_SIGS)

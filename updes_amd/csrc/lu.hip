@@ -448,6 +448,15 @@ struct PanelArgs {
 
 constexpr int kPanelThreads = 256;
 constexpr int kPanelWMax = 64;
+// Narrow leaves for the panels that run beside a long trailing update: a
+// 16-wide FP64 leaf (67 VGPRs) shares a CU with two trailing-update
+// workgroups (2 x 216 VGPRs per SIMD), where a 64-wide one (164) displaces
+// one of them.  More leaves and splits per panel, hidden under the update
+// while it is long: panels of at least kNarrowLeafRows rows (measured best
+// at n = 40,006 among 20k..36k).
+constexpr int kPanelWNarrow = 16;
+constexpr int64_t kNarrowLeafRows = 28000;
+static std::atomic<int64_t> g_narrow_leaf_rows{0};
 // A pivot record (two 64-bit words) owns a 128-byte line: every workgroup
 // polls every record, and records sharing a line would serialise the polls.
 constexpr int kRecWords = 16;
@@ -547,9 +556,9 @@ constexpr int kSwapStage = 2048;
 // registers left by one after each column, so the current column is always
 // register 0 (no run-time register indexing) and after the loop the rows
 // are back in column order.  Columns >= w are zero and only rotated.
-template <typename T, int ROWS>
+template <typename T, int ROWS, int W>
 __global__ __launch_bounds__(kPanelThreads) void panel_kernel(PanelArgs<T> a) {
-  constexpr int W = kPanelWMax;
+  static_assert(W <= kPanelWMax, "leaf wider than the candidate rows");
   __shared__ T urow[2 * W];  // winner row of the current column; zeros from w on
   __shared__ unsigned long long s_k1[kPanelThreads / 64], s_k2[kPanelThreads / 64];
   __shared__ int64_t s_win[W];
@@ -598,110 +607,108 @@ __global__ __launch_bounds__(kPanelThreads) void panel_kernel(PanelArgs<T> a) {
     }
   };
   for (int j = 0; j < w; ++j) {
-    {
-      const int par = j & 1;
-      const unsigned long long tag = (unsigned long long)((a.seq_base + (unsigned)(j + 1)) & 0xFFFFu);
-      // 1. best active row of this workgroup for column j
-      if (tid == 0) s_bad = 0;
-      const unsigned long long k1 = block_max_key(thread_key(), s_k1);
-      const int blocal = kKeyMaxRows - 1 - (int)(k1 & (kKeyMaxRows - 1));
-      // 2. publish it.  The owner of that row stores the row (absolute column
-      //    order), waits until the stores have completed, then stores the
-      //    tagged record
-      //      w0 = bits of |value| with the 16 low mantissa bits set to the tag,
-      //      w1 = tag << 48 | row.
-      //    A reader takes a record only when both words carry the tag of
-      //    column j, so the record needs no separate flag and its two stores
-      //    no order.  With no candidate, thread 0 publishes value -1.
-      const bool owner = k1 != 0 && (blocal % kPanelThreads) == tid;
-      if (owner) {
-        T* dst = a.cand_row + ((int64_t)par * G + g) * w;
-        const int qo = blocal / kPanelThreads;
-#pragma unroll
-        for (int q = 0; q < ROWS; ++q)
-          if (q == qo) {
-#pragma unroll
-            for (int c = 0; c < W; ++c)
-              if (j + c < w) st_coh(dst + j + c, P[q][c]);
-          }
-        __builtin_amdgcn_s_waitcnt(0);
-      }
-      if (owner || (k1 == 0 && tid == 0)) {
-        const double bv = k1 ? __longlong_as_double((long long)(k1 & kKeyValMask)) : -1.0;
-        const int64_t bi = k1 ? rb + blocal : INT64_MAX;
-        unsigned long long* rec = recs + kRecWords * ((int64_t)par * G + g);
-        st_coh(rec, (unsigned long long)(__double_as_longlong(bv) & ~0xFFFFll) | tag);
-        st_coh(rec + 1, (tag << 48) | ((unsigned long long)bi & 0xFFFFFFFFFFFFull));
-      }
-      // 3.-4. every workgroup's record of column j (one thread each) and the
-      //    winner across workgroups
-      unsigned long long k2 = 0;
-      for (int q = tid; q < G; q += kPanelThreads) {
-        const unsigned long long* o = recs + kRecWords * ((int64_t)par * G + q);
-        unsigned long long w0 = ld_coh(o), w1 = ld_coh(o + 1);
-        uint32_t spins = 0;
-        while ((w0 & 0xFFFFull) != tag || (w1 >> 48) != tag) {
-          __builtin_amdgcn_s_sleep(1);
-          ++spins;
-          if ((spins & 255u) == 0 &&
-              __hip_atomic_load(a.status, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) {
-            s_bad = 1;
-            break;
-          }
-          if (spins > kSpinLimit) {
-            __hip_atomic_store(a.status, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            s_bad = 1;
-            break;
-          }
-          w0 = ld_coh(o);
-          w1 = ld_coh(o + 1);
-        }
-        const double ov = __longlong_as_double((long long)(w0 & ~0xFFFFull));
-        if (ov >= 0.0)
-          k2 = umax64(k2, pivot_key(ov, q, (int)((int64_t)(w1 & 0xFFFFFFFFFFFFull) - (int64_t)q * a.R)));
-      }
-      k2 = block_max_key(k2, s_k2);
-      if (s_bad) return;
-      const int wq = kKeyMaxGroups - 1 - (int)((k2 >> 11) & (kKeyMaxGroups - 1));
-      const int64_t wi = (int64_t)wq * a.R + (kKeyMaxRows - 1 - (int)(k2 & (kKeyMaxRows - 1)));
-      if (k2 != 0 && (wq >= G || wi >= a.m)) {
-        // a key that names no row cannot come from the records: stop
-        if (tid == 0) __hip_atomic_store(a.status, 3, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        return;
-      }
-      if (k2 == 0) {
-        // no finite candidate anywhere (NaN column): every workgroup sees the
-        // same key, so all of them leave here together
-        if (g == 0 && tid == 0) {
-          *a.info = -1;
-          __hip_atomic_store(a.status, 2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        }
-        return;
-      }
-      if (tid < w) urow[tid] = ld_coh(a.cand_row + ((int64_t)par * G + wq) * w + tid);
-      if (tid == 0) s_win[j] = wi;
+    const int par = j & 1;
+    const unsigned long long tag = (unsigned long long)((a.seq_base + (unsigned)(j + 1)) & 0xFFFFu);
+    // 1. best active row of this workgroup for column j
+    if (tid == 0) s_bad = 0;
+    const unsigned long long k1 = block_max_key(thread_key(), s_k1);
+    const int blocal = kKeyMaxRows - 1 - (int)(k1 & (kKeyMaxRows - 1));
+    // 2. publish it.  The owner of that row stores the row (absolute column
+    //    order), waits until the stores have completed, then stores the
+    //    tagged record
+    //      w0 = bits of |value| with the 16 low mantissa bits set to the tag,
+    //      w1 = tag << 48 | row.
+    //    A reader takes a record only when both words carry the tag of
+    //    column j, so the record needs no separate flag and its two stores
+    //    no order.  With no candidate, thread 0 publishes value -1.
+    const bool owner = k1 != 0 && (blocal % kPanelThreads) == tid;
+    if (owner) {
+      T* dst = a.cand_row + ((int64_t)par * G + g) * w;
+      const int qo = blocal / kPanelThreads;
 #pragma unroll
       for (int q = 0; q < ROWS; ++q)
-        if (rb + tid + q * kPanelThreads == wi) act[q] = false;
-      __syncthreads();
-      // 5. eliminate column j from the active rows: register c holds column
-      //    j + c (urow beyond w is zero, which also leaves the finished
-      //    columns rotated to the top untouched)
-      const T piv = urow[j];
-      const T rpiv = piv != T(0) ? T(1) / piv : T(0);
-      if (tid == 0) s_zero[j] = piv == T(0);
-      // Branch-free: an inactive row takes multiplier 0 (a divergent branch
-      // here makes the compiler keep a second copy of the row registers).
-      // Eliminated and rotated in one pass: register c takes column j+1+c,
-      // the top register the multiplier of column j.
+        if (q == qo) {
 #pragma unroll
-      for (int q = 0; q < ROWS; ++q) {
-        const T p0 = P[q][0];
-        const T l = act[q] ? p0 * rpiv : T(0);
-#pragma unroll
-        for (int c = 0; c + 1 < W; ++c) P[q][c] = P[q][c + 1] - l * urow[j + 1 + c];
-        P[q][W - 1] = act[q] ? l : p0;
+          for (int c = 0; c < W; ++c)
+            if (j + c < w) st_coh(dst + j + c, P[q][c]);
+        }
+      __builtin_amdgcn_s_waitcnt(0);
+    }
+    if (owner || (k1 == 0 && tid == 0)) {
+      const double bv = k1 ? __longlong_as_double((long long)(k1 & kKeyValMask)) : -1.0;
+      const int64_t bi = k1 ? rb + blocal : INT64_MAX;
+      unsigned long long* rec = recs + kRecWords * ((int64_t)par * G + g);
+      st_coh(rec, (unsigned long long)(__double_as_longlong(bv) & ~0xFFFFll) | tag);
+      st_coh(rec + 1, (tag << 48) | ((unsigned long long)bi & 0xFFFFFFFFFFFFull));
+    }
+    // 3.-4. every workgroup's record of column j (one thread each) and the
+    //    winner across workgroups
+    unsigned long long k2 = 0;
+    for (int q = tid; q < G; q += kPanelThreads) {
+      const unsigned long long* o = recs + kRecWords * ((int64_t)par * G + q);
+      unsigned long long w0 = ld_coh(o), w1 = ld_coh(o + 1);
+      uint32_t spins = 0;
+      while ((w0 & 0xFFFFull) != tag || (w1 >> 48) != tag) {
+        __builtin_amdgcn_s_sleep(1);
+        ++spins;
+        if ((spins & 255u) == 0 &&
+            __hip_atomic_load(a.status, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) {
+          s_bad = 1;
+          break;
+        }
+        if (spins > kSpinLimit) {
+          __hip_atomic_store(a.status, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          s_bad = 1;
+          break;
+        }
+        w0 = ld_coh(o);
+        w1 = ld_coh(o + 1);
       }
+      const double ov = __longlong_as_double((long long)(w0 & ~0xFFFFull));
+      if (ov >= 0.0)
+        k2 = umax64(k2, pivot_key(ov, q, (int)((int64_t)(w1 & 0xFFFFFFFFFFFFull) - (int64_t)q * a.R)));
+    }
+    k2 = block_max_key(k2, s_k2);
+    if (s_bad) return;
+    const int wq = kKeyMaxGroups - 1 - (int)((k2 >> 11) & (kKeyMaxGroups - 1));
+    const int64_t wi = (int64_t)wq * a.R + (kKeyMaxRows - 1 - (int)(k2 & (kKeyMaxRows - 1)));
+    if (k2 != 0 && (wq >= G || wi >= a.m)) {
+      // a key that names no row cannot come from the records: stop
+      if (tid == 0) __hip_atomic_store(a.status, 3, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      return;
+    }
+    if (k2 == 0) {
+      // no finite candidate anywhere (NaN column): every workgroup sees the
+      // same key, so all of them leave here together
+      if (g == 0 && tid == 0) {
+        *a.info = -1;
+        __hip_atomic_store(a.status, 2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      }
+      return;
+    }
+    if (tid < w) urow[tid] = ld_coh(a.cand_row + ((int64_t)par * G + wq) * w + tid);
+    if (tid == 0) s_win[j] = wi;
+#pragma unroll
+    for (int q = 0; q < ROWS; ++q)
+      if (rb + tid + q * kPanelThreads == wi) act[q] = false;
+    __syncthreads();
+    // 5. eliminate column j from the active rows: register c holds column
+    //    j + c (urow beyond w is zero, which also leaves the finished
+    //    columns rotated to the top untouched)
+    const T piv = urow[j];
+    const T rpiv = piv != T(0) ? T(1) / piv : T(0);
+    if (tid == 0) s_zero[j] = piv == T(0);
+    // Branch-free: an inactive row takes multiplier 0 (a divergent branch
+    // here makes the compiler keep a second copy of the row registers).
+    // Eliminated and rotated in one pass: register c takes column j+1+c,
+    // the top register the multiplier of column j.
+#pragma unroll
+    for (int q = 0; q < ROWS; ++q) {
+      const T p0 = P[q][0];
+      const T l = act[q] ? p0 * rpiv : T(0);
+#pragma unroll
+      for (int c = 0; c + 1 < W; ++c) P[q][c] = P[q][c + 1] - l * urow[j + 1 + c];
+      P[q][W - 1] = act[q] ? l : p0;
     }
   }
   // the remaining rotations bring the columns back into register order
@@ -1019,6 +1026,7 @@ struct LuCtx {
   unsigned seq;     // steps taken by the panel launches so far
   LuScratch<T> sc;
   int G_max;
+  int wleaf = kPanelWMax;  // leaf width of the recursion
   int64_t pc0 = 0;  // outer panel being factored: columns [pc0, pc0 + pw)
   int pw = 0;
 };
@@ -1069,12 +1077,21 @@ static int leaf(LuCtx<T>& c, int64_t r0, int w) {
   // every wait is bounded in any case.  (A cooperative launch guarantees
   // the same but goes through a separate device queue, which added ~60 us
   // of dispatch latency per leaf, measured.)
-  if (rows == 1)
-    hipLaunchKernelGGL((panel_kernel<T, 1>), dim3(G), dim3(kPanelThreads), 0, c.s, a);
-  else if (rows == 2)
-    hipLaunchKernelGGL((panel_kernel<T, 2>), dim3(G), dim3(kPanelThreads), 0, c.s, a);
-  else
-    hipLaunchKernelGGL((panel_kernel<T, 3>), dim3(G), dim3(kPanelThreads), 0, c.s, a);
+  if (w <= kPanelWNarrow) {
+    if (rows == 1)
+      hipLaunchKernelGGL((panel_kernel<T, 1, kPanelWNarrow>), dim3(G), dim3(kPanelThreads), 0, c.s, a);
+    else if (rows == 2)
+      hipLaunchKernelGGL((panel_kernel<T, 2, kPanelWNarrow>), dim3(G), dim3(kPanelThreads), 0, c.s, a);
+    else
+      hipLaunchKernelGGL((panel_kernel<T, 3, kPanelWNarrow>), dim3(G), dim3(kPanelThreads), 0, c.s, a);
+  } else {
+    if (rows == 1)
+      hipLaunchKernelGGL((panel_kernel<T, 1, kPanelWMax>), dim3(G), dim3(kPanelThreads), 0, c.s, a);
+    else if (rows == 2)
+      hipLaunchKernelGGL((panel_kernel<T, 2, kPanelWMax>), dim3(G), dim3(kPanelThreads), 0, c.s, a);
+    else
+      hipLaunchKernelGGL((panel_kernel<T, 3, kPanelWMax>), dim3(G), dim3(kPanelThreads), 0, c.s, a);
+  }
   RBFX_CHECK_LAUNCH();
   prof_stop(pid, c.s, (double)m * w * w, 2.0 * sizeof(T) * m * w);
   c.seq += (unsigned)w;
@@ -1160,7 +1177,7 @@ static int trsm(LuCtx<T>& c, int64_t r0, int W, const T* X, int64_t ldx, int64_t
 template <typename T>
 static int panel(LuCtx<T>& c, int64_t r0, int W, T* X, int64_t ldx) {
   int st;
-  if (W <= kPanelWMax) {
+  if (W <= c.wleaf) {
     if ((st = leaf(c, r0, W))) return st;
     return leaf_inverse(c, r0, W, X, ldx);
   }
@@ -1309,6 +1326,9 @@ static int getrf(int64_t n, T* A, int64_t lda, int64_t* ipiv, void* work, size_t
     const int pid = prof_start(sizeof(T) == 8 ? "panel_step_f64" : "panel_step_f32", cp.s);
     cp.pc0 = r0;
     cp.pw = W;
+    const int64_t nl = g_narrow_leaf_rows.load(std::memory_order_relaxed);
+    const int64_t narrow_rows = nl == 0 ? kNarrowLeafRows : nl;
+    cp.wleaf = narrow_rows > 0 && n - r0 >= narrow_rows ? kPanelWNarrow : kPanelWMax;
     if ((e = panel(cp, r0, W, out.tri_inv, (int64_t)kNB))) return e;
     prof_stop(pid, cp.s, (double)(n - r0) * W * W, 0.0);
     return make_plan(cp, out, r0, r0 + W);
@@ -1376,6 +1396,11 @@ using namespace rbfx;
 extern "C" int rbfx_set_panel_groups(int max_groups) {
   RBFX_REQUIRE(max_groups >= 0, "max_groups must be >= 0");
   g_panel_groups.store(max_groups, std::memory_order_relaxed);
+  return RBFX_OK;
+}
+
+extern "C" int rbfx_set_narrow_leaf_rows(int64_t rows) {
+  g_narrow_leaf_rows.store(rows, std::memory_order_relaxed);
   return RBFX_OK;
 }
 
