@@ -144,11 +144,11 @@ def measured_traffic(kernel, side):
     return None
 
 
-def gemm_replay(n, nb=512, leaf=64):
-    """The GEMM launches of the blocked LU of an n x n matrix (panel width nb,
-    recursive panels halved down to `leaf` columns): trailing update of the
-    next panel's columns, of the rest, and the in-panel updates, each on the
-    same sub-blocks of one row-major matrix as in the factorisation."""
+def gemm_replay(n, nb=512):
+    """The trailing-update GEMM launches of the blocked LU of an n x n matrix
+    (panel width nb): for every panel step, the update of the next panel's
+    columns and of the rest, on the same sub-blocks of one row-major matrix
+    as in the factorisation (the panels themselves do not run)."""
     from updes_amd import _lib as L, engine as E
     lib = L.lib()
     ld = E._ld(n)
@@ -161,19 +161,9 @@ def gemm_replay(n, nb=512, leaf=64):
                                   ctypes.c_void_p(base + el * (b_r * ld + b_c)), ld, 1.0,
                                   ctypes.c_void_p(base + el * (c_r * ld + c_c)), ld, L.stream()))
 
-    def panel(r0, W):
-        if W <= leaf:
-            return
-        W1 = max(8, (W // 2 + 7) // 8 * 8)
-        W2 = W - W1
-        panel(r0, W1)
-        gemm(n - r0 - W1, W2, W1, r0 + W1, r0, r0, r0 + W1, r0 + W1, r0 + W1)
-        panel(r0 + W1, W2)
-
     count = 0
     for k0 in range(0, n, nb):
         W = min(nb, n - k0)
-        panel(k0, W)
         k1 = k0 + W
         rest = n - k1
         if rest <= 0:

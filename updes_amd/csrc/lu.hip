@@ -70,8 +70,8 @@ constexpr int GBM = 128, GBN = 128, GTHREADS = 256;
 // Two waves per SIMD (<= 256 registers each) so one wave's barrier or load
 // wait overlaps the other's MFMAs.
 // USE only separates the kernel symbols of the trailing update (0), of the
-// triangular solve (1) and of the diagonal-block inverse (2) in profiles;
-// the code is the same.
+// triangular solve (1), of the diagonal-block inverse (2) and of the updates
+// inside a recursive panel (3) in profiles; the code is the same.
 template <typename T, int USE>
 __global__ __launch_bounds__(GTHREADS) __attribute__((amdgpu_waves_per_eu(2, 2))) void gemm_kernel(
                                                         int64_t m, int64_t n, int64_t k, T alpha,
@@ -420,10 +420,15 @@ static int gemm_launch(int64_t m, int64_t n, int64_t k, T alpha, const T* A, int
 
 template <typename T>
 static int gemm(int64_t m, int64_t n, int64_t k, T alpha, const T* A, int64_t lda, const T* B,
-                int64_t ldb, T beta, T* C, int64_t ldc, hipStream_t s) {
+                int64_t ldb, T beta, T* C, int64_t ldc, hipStream_t s, bool in_panel = false) {
   if (m <= 0 || n <= 0) return RBFX_OK;
-  const int pid = prof_start(sizeof(T) == 8 ? "gemm_f64" : "gemm_f32", s);
-  const int st = gemm_launch<T>(m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, s);
+  // profile names: the trailing update of the factorisation (and direct
+  // calls) vs the updates inside a recursive panel
+  const int pid = prof_start(in_panel ? (sizeof(T) == 8 ? "gemm_panel_f64" : "gemm_panel_f32")
+                                      : (sizeof(T) == 8 ? "gemm_f64" : "gemm_f32"),
+                             s);
+  const int st = in_panel ? gemm_launch<T, 3>(m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, s)
+                          : gemm_launch<T, 0>(m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, s);
   prof_stop(pid, s, 2.0 * m * n * k, (double)sizeof(T) * (2.0 * m * n + (double)m * k + (double)k * n));
   return st;
 }
@@ -1190,7 +1195,7 @@ static int panel(LuCtx<T>& c, int64_t r0, int W, T* X, int64_t ldx) {
   const int64_t m2 = c.n - r0 - W1;
   const T* L21 = c.A + (r0 + W1) * c.lda + r0;
   if ((st = gemm<T>(m2, W2, W1, T(-1), L21, c.lda, c.A + r0 * c.lda + r0 + W1, c.lda, T(1),
-                    c.A + (r0 + W1) * c.lda + r0 + W1, c.lda, c.s)))
+                    c.A + (r0 + W1) * c.lda + r0 + W1, c.lda, c.s, true)))
     return st;
   T* X22 = X + (int64_t)W1 * ldx + W1;
   if ((st = panel(c, r0 + W1, W2, X22, ldx))) return st;
