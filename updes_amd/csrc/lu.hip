@@ -1326,14 +1326,16 @@ static int getrf(int64_t n, T* A, int64_t lda, int64_t* ipiv, void* work, size_t
   // Panel stream, per step: factor the panel (which also leaves the inverse
   // of its diagonal block in the main stream's slot), then the
   // row-interchange plan for that slot.
-  auto panel_step = [&](int64_t r0, int W, LuScratch<T>& out) -> int {
+  // Narrow leaves only for panels that run beside a trailing update (not
+  // for panel 0, which runs alone).
+  auto panel_step = [&](int64_t r0, int W, LuScratch<T>& out, bool beside_update) -> int {
     int e;
     const int pid = prof_start(sizeof(T) == 8 ? "panel_step_f64" : "panel_step_f32", cp.s);
     cp.pc0 = r0;
     cp.pw = W;
     const int64_t nl = g_narrow_leaf_rows.load(std::memory_order_relaxed);
     const int64_t narrow_rows = nl == 0 ? kNarrowLeafRows : nl;
-    cp.wleaf = narrow_rows > 0 && n - r0 >= narrow_rows ? kPanelWNarrow : kPanelWMax;
+    cp.wleaf = beside_update && narrow_rows > 0 && n - r0 >= narrow_rows ? kPanelWNarrow : kPanelWMax;
     if ((e = panel(cp, r0, W, out.tri_inv, (int64_t)kNB))) return e;
     prof_stop(pid, cp.s, (double)(n - r0) * W * W, 0.0);
     return make_plan(cp, out, r0, r0 + W);
@@ -1341,7 +1343,7 @@ static int getrf(int64_t n, T* A, int64_t lda, int64_t* ipiv, void* work, size_t
   // panel 0 after everything queued so far on the caller's stream
   RBFX_HIP(hipEventRecord(ready, s));
   RBFX_HIP(hipStreamWaitEvent(cp.s, ready, 0));
-  if ((st = panel_step(0, (int)std::min<int64_t>(kNB, n), slot[0]))) return st;
+  if ((st = panel_step(0, (int)std::min<int64_t>(kNB, n), slot[0], false))) return st;
   RBFX_HIP(hipEventRecord(done, cp.s));
   int par = 0;
   for (int64_t k0 = 0; k0 < n; k0 += kNB, par ^= 1) {
@@ -1362,15 +1364,14 @@ static int getrf(int64_t n, T* A, int64_t lda, int64_t* ipiv, void* work, size_t
     // while this stream does the rest of the step (look-ahead): the
     // interchanges of the columns left of the panel, which nothing in the
     // factorisation reads again, and those, U12 and the update of the
-    // remaining columns.
+    // remaining columns.  The rest of the step is enqueued before the panel,
+    // whose many small launches take the host a few ms: so the device never
+    // waits for the host to reach the long update.
     const int Wn = (int)std::min<int64_t>(kNB, rest);
     if ((st = apply_swaps(c, k0, k1, k1, Wn))) return st;
     if ((st = trsm(c, k0, W, c.sc.tri_inv, (int64_t)kNB, k1, Wn))) return st;
     if ((st = gemm<T>(rest, Wn, W, T(-1), L21, lda, U12, lda, T(1), A22, lda, s))) return st;
     RBFX_HIP(hipEventRecord(ready, s));
-    RBFX_HIP(hipStreamWaitEvent(cp.s, ready, 0));
-    if ((st = panel_step(k1, Wn, slot[par ^ 1]))) return st;
-    RBFX_HIP(hipEventRecord(done, cp.s));
     if ((st = apply_swaps(c, k0, k1, 0, k0))) return st;
     if (rest > Wn) {
       if ((st = apply_swaps(c, k0, k1, k1 + Wn, rest - Wn))) return st;
@@ -1378,6 +1379,9 @@ static int getrf(int64_t n, T* A, int64_t lda, int64_t* ipiv, void* work, size_t
       if ((st = gemm<T>(rest, rest - Wn, W, T(-1), L21, lda, U12 + Wn, lda, T(1), A22 + Wn, lda, s)))
         return st;
     }
+    RBFX_HIP(hipStreamWaitEvent(cp.s, ready, 0));
+    if ((st = panel_step(k1, Wn, slot[par ^ 1], rest > Wn))) return st;
+    RBFX_HIP(hipEventRecord(done, cp.s));
   }
   hipLaunchKernelGGL(status_to_info_kernel, dim3(1), dim3(64), 0, s, c.status, info);
   RBFX_CHECK_LAUNCH();
