@@ -17,8 +17,13 @@ and the error of the mixed path are reported beside it.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--side 200]
                        [--precisions fp64,fp32_ir] [--copy-probe-mib M]
+                       [--gemm-replay] [--no-cpu-baseline] [--no-kernel-profile]
 --copy-probe-mib runs one plain device copy of M MiB before the workload, so a
 counter run (FETCH_SIZE / WRITE_SIZE) can be scaled by a copy of known size.
+--gemm-replay only replays the trailing-update GEMM launches of the FP64 LU
+(for counter runs on that kernel alone).  The `roofline` object is that
+trailing-update GEMM (the dominant kernel), timed with HIP events on its
+launch stream during one extra profiled step after the timed ones.
 """
 import argparse
 import ctypes
