@@ -93,6 +93,7 @@ __global__ __launch_bounds__(kTsThreads) void sweep_kernel(const TF* __restrict_
   __shared__ T acc[TB];
   __shared__ T part[4][TB];
   __shared__ T Ds[TB][TB + 1];
+  __shared__ T red[TRANS ? 1 : TB][TB + 1];  // no-transpose partial sums, row x column
   __shared__ int s_I;
   __shared__ unsigned s_avail;
   const int tid = threadIdx.x;
@@ -112,11 +113,14 @@ __global__ __launch_bounds__(kTsThreads) void sweep_kernel(const TF* __restrict_
   const int64_t oI = I * TB;
   const int szI = (int)imin64(TB, n - oI);
 
-  // Thread ownership of a tile, kept for every off-diagonal tile:
-  //   no-transpose: row rr = tid/4 of block I, columns q*16 .. q*16+15 of block J
+  // Thread ownership of a tile, kept for every off-diagonal tile: lanes run
+  // along the stored rows (coalesced), a wave owns 16 of the other index.
+  //   no-transpose: column rr = tid%64 of block J, rows q*16 .. q*16+15 of
+  //                 block I (16 partial row sums per thread, summed over the
+  //                 lanes at the end)
   //   transpose:    column rr = tid%64 of block I, rows q*16 .. q*16+15 of block J
-  const int rr = TRANS ? (tid & 63) : (tid >> 2);
-  const int q = TRANS ? (tid >> 6) : (tid & 3);
+  const int rr = tid & 63;
+  const int q = tid >> 6;
 
   // prefetch the diagonal inverse and the tile of the last dependency (the
   // one on the critical path) before waiting
@@ -130,15 +134,19 @@ __global__ __launch_bounds__(kTsThreads) void sweep_kernel(const TF* __restrict_
 #pragma unroll
     for (int k = 0; k < 16; ++k) {
       const int kk = q * 16 + k;
-      pre[k] = (rr < szI && kk < szJ)
-                   ? (TRANS ? (T)F[(oJ + kk) * lda + oI + rr] : (T)F[(oI + rr) * lda + oJ + kk])
-                   : T(0);
+      if (TRANS)
+        pre[k] = (rr < szI && kk < szJ) ? (T)F[(oJ + kk) * lda + oI + rr] : T(0);
+      else
+        pre[k] = (kk < szI && rr < szJ) ? (T)F[(oI + kk) * lda + oJ + rr] : T(0);
     }
   }
 
   // fold in the dependencies in completion order d = 0 .. seq-1 as they
-  // become available (one private partial sum per thread, no barrier per tile)
+  // become available (private partial sums per thread, no barrier per tile)
   T s = 0;
+  T ps[TRANS ? 1 : 16];
+#pragma unroll
+  for (int k = 0; k < (TRANS ? 1 : 16); ++k) ps[k] = 0;
   int d = 0;
   while (d < seq) {
     if ((unsigned)d >= s_avail) {
@@ -169,6 +177,20 @@ __global__ __launch_bounds__(kTsThreads) void sweep_kernel(const TF* __restrict_
       const int64_t J = FWD ? d : nb - 1 - d;
       const int64_t oJ = J * TB;
       const int szJ = (int)imin64(TB, n - oJ);
+      if (!TRANS) {
+        // one y entry per thread (its column), 16 coalesced row segments
+        if (rr >= szJ) continue;
+        const T yk = ld_coh(y + oJ + rr);
+        if (d == seq - 1) {
+#pragma unroll
+          for (int k = 0; k < 16; ++k) ps[k] += pre[k] * yk;
+        } else {
+#pragma unroll
+          for (int k = 0; k < 16; ++k)
+            if (q * 16 + k < szI) ps[k] += (T)F[(oI + q * 16 + k) * lda + oJ + rr] * yk;
+        }
+        continue;
+      }
       if (rr >= szI) continue;
       if (d == seq - 1) {
 #pragma unroll
@@ -176,11 +198,6 @@ __global__ __launch_bounds__(kTsThreads) void sweep_kernel(const TF* __restrict_
           const int kk = q * 16 + k;
           if (kk < szJ) s += pre[k] * ld_coh(y + oJ + kk);
         }
-      } else if (!TRANS) {
-        const TF* row = F + (oI + rr) * lda + oJ + q * 16;
-#pragma unroll
-        for (int k = 0; k < 16; ++k)
-          if (q * 16 + k < szJ) s += (T)row[k] * ld_coh(y + oJ + q * 16 + k);
       } else {
 #pragma unroll
         for (int k = 0; k < 16; ++k) {
@@ -193,9 +210,17 @@ __global__ __launch_bounds__(kTsThreads) void sweep_kernel(const TF* __restrict_
 
   // acc = b_I - (sum of the folded tiles)
   if (!TRANS) {
-    s += __shfl_xor(s, 1, 64);
-    s += __shfl_xor(s, 2, 64);
-    if (q == 0) acc[rr] = rr < szI ? y[oI + rr] - s : T(0);
+    // row sums over the 64 column lanes: through LDS, 4 threads per row
+#pragma unroll
+    for (int k = 0; k < 16; ++k) red[q * 16 + k][rr] = ps[k];
+    __syncthreads();
+    const int r = tid >> 2, qq = tid & 3;
+    T t = 0;
+#pragma unroll
+    for (int c = 0; c < 16; ++c) t += red[r][qq * 16 + c];
+    t += __shfl_xor(t, 1, 64);
+    t += __shfl_xor(t, 2, 64);
+    if (qq == 0) acc[r] = r < szI ? y[oI + r] - t : T(0);
   } else {
     part[q][rr] = s;
     __syncthreads();
