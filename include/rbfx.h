@@ -25,7 +25,7 @@
 extern "C" {
 #endif
 
-#define RBFX_ABI_VERSION 1
+#define RBFX_ABI_VERSION 2
 
 enum rbfx_status {
   RBFX_OK = 0,
@@ -66,12 +66,22 @@ enum rbfx_matrix {
  *   [.., +Npd)              periodic flux rows  grad phi(x_i1,.).n1 + grad phi(x_i2,.).n2
  *   [N, N+M)                P^T rows
  * The descriptor coefficients are op_*_const[t] + (op_*_row[t] ? op_*_row[t][i] : 0).
+ *
+ * Local supports (updes/cloud.py:83-112 with support_size < N; the scatters
+ * of updes/assembly.py:31-32, 127-128, 171-258): with RBFX_FLAG_LOCAL_SUPPORT
+ * in flags, the RBF columns of node row i are restricted to the S =
+ * support_cols node ids in d_support[i*S .. i*S+S) (ascending, without i),
+ * and periodic rows to the union of the supports of both nodes of the pair.
+ * Monomial columns and the P^T rows stay full.  Without the flag every node
+ * but the row's own is a column (support_size = "max").
  */
+enum rbfx_flags { RBFX_FLAG_LOCAL_SUPPORT = 1 };
+
 typedef struct rbfx_problem {
   int64_t N;            /* number of nodes                                   */
   int64_t M;            /* number of monomials, 0..15                        */
   int32_t kernel;       /* enum rbfx_kernel                                  */
-  int32_t flags;        /* reserved, 0                                       */
+  int32_t flags;        /* enum rbfx_flags bits, or 0                        */
   double param;         /* a (polyharmonic / thin plate) or eps              */
   const double* d_x;    /* (N) node x, sorted order                          */
   const double* d_y;    /* (N) node y                                        */
@@ -87,6 +97,8 @@ typedef struct rbfx_problem {
   double op_mon_const[RBFX_NOPS];
   const double* d_op_rbf_row[RBFX_NOPS]; /* (Ni) per-row coefficients or NULL */
   const double* d_op_mon_row[RBFX_NOPS];
+  int64_t support_cols;      /* S: support entries per node (flag set)      */
+  const int32_t* d_support;  /* (N*S) ascending node ids per node, or NULL  */
 } rbfx_problem;
 
 /* ---- library ---------------------------------------------------------- */

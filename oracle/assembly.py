@@ -1,9 +1,11 @@
 """Dense collocation assembly (oracle, test infrastructure only).
 
-Restates ``updes/assembly.py`` for clouds with ``support_size="max"``:
-every node's support is every other node (``cloud.py:110-111``), so Phi, op(Phi)
-and the d/n/r rows of bd(Phi) have a zero diagonal, while the periodic rows use
-the union of two supports, i.e. all N columns (``assembly.py:219-258``).
+Restates ``updes/assembly.py``.  Every RBF block is written through the
+support mask of ``support_mask``: row i of Phi, op(Phi) and the d/n/r rows of
+bd(Phi) keep only the columns of node i's support, the periodic rows the union
+of the supports of both nodes of the pair (``assembly.py:31-32, 127-128,
+171-258``).  With ``support_size="max"`` a support is every other node
+(``cloud.py:110-111``): a zero diagonal, and all N columns in periodic rows.
 
 Operators are given as descriptors instead of Python callables: a descriptor is
 ``{"rbf": [c_VAL, c_DX, c_DY, c_LAP, c_DXX, c_DYY, c_DXY],
@@ -36,12 +38,22 @@ def _combine(ops, coeffs):
     return acc
 
 
+def support_mask(cloud):
+    """(N, N) bool: column j is in the support of node i (``cloud.py:83-112``)."""
+    N = cloud.N
+    sup = getattr(cloud, "supports", None)
+    if sup is None:
+        return ~np.eye(N, dtype=bool)
+    m = np.zeros((N, N), dtype=bool)
+    m[np.repeat(np.arange(N), sup.shape[1]), np.asarray(sup).ravel()] = True
+    return m
+
+
 def assemble_Phi(cloud, kind, param):
-    """``assembly.py:10-36``: Phi[i, supp(i)] = rbf(x_i, x_supp); diagonal stays 0."""
+    """``assembly.py:10-36``: Phi[i, supp(i)] = rbf(x_i, x_supp); other entries 0."""
     x, y = cloud.sorted_nodes[:, 0], cloud.sorted_nodes[:, 1]
     Phi = _pairs(kind, param, x, y, x, y)[R.VAL]
-    np.fill_diagonal(Phi, 0.0)
-    return Phi
+    return np.where(support_mask(cloud), Phi, 0.0)
 
 
 def assemble_P(cloud, M):
@@ -67,8 +79,7 @@ def assemble_op_Phi_P(desc, cloud, kind, param, M):
     Ni = cloud.Ni
     x, y = cloud.sorted_nodes[:, 0], cloud.sorted_nodes[:, 1]
     ops = _pairs(kind, param, x[:Ni], y[:Ni], x, y)
-    opPhi = _combine(ops, desc["rbf"])
-    opPhi[np.arange(Ni), np.arange(Ni)] = 0.0       # self excluded from the support
+    opPhi = np.where(support_mask(cloud)[:Ni], _combine(ops, desc["rbf"]), 0.0)
     opP = np.zeros((Ni, M))
     for k in range(M):
         mops = R.monomial_ops(k, x[:Ni], y[:Ni])
@@ -92,7 +103,7 @@ def assemble_bd_Phi_P(cloud, kind, param, M, robin_betas=None):
     nb = Nd + Nn + Nr + sum(Np)
     bdPhi = np.zeros((nb, N))
     bdP = np.zeros((nb, M))
-    cols = np.arange(N)
+    mask = support_mask(cloud)
 
     def ops_at(i_rows):
         return _pairs(kind, param, x[i_rows], y[i_rows], x, y)
@@ -103,9 +114,7 @@ def assemble_bd_Phi_P(cloud, kind, param, M, robin_betas=None):
     # Dirichlet (169-176, 283-287)
     rows = np.arange(Ni, Ni + Nd)
     if Nd:
-        v = ops_at(rows)[R.VAL]
-        v[np.arange(Nd), rows] = 0.0
-        bdPhi[rows - Ni] = v
+        bdPhi[rows - Ni] = np.where(mask[rows], ops_at(rows)[R.VAL], 0.0)
         bdP[rows - Ni] = mon_at(rows)[R.VAL]
     # Neumann (178-190, 290-299)
     rows = np.arange(Ni + Nd, Ni + Nd + Nn)
@@ -113,8 +122,7 @@ def assemble_bd_Phi_P(cloud, kind, param, M, robin_betas=None):
         o = ops_at(rows)
         n = nrm[rows - Ni - Nd]
         v = o[R.DX] * n[:, 0:1] + o[R.DY] * n[:, 1:2]
-        v[np.arange(Nn), rows] = 0.0
-        bdPhi[rows - Ni] = v
+        bdPhi[rows - Ni] = np.where(mask[rows], v, 0.0)
         m = mon_at(rows)
         nn = np.array([cloud.outward_normals[i] for i in rows])
         bdP[rows - Ni] = m[R.DX] * nn[:, 0:1] + m[R.DY] * nn[:, 1:2]
@@ -125,8 +133,7 @@ def assemble_bd_Phi_P(cloud, kind, param, M, robin_betas=None):
         o = ops_at(rows)
         n_bug = nrm[rows - Ni - Nd - Nn]
         v = betas[:, None] * o[R.VAL] + o[R.DX] * n_bug[:, 0:1] + o[R.DY] * n_bug[:, 1:2]
-        v[np.arange(Nr), rows] = 0.0
-        bdPhi[rows - Ni] = v
+        bdPhi[rows - Ni] = np.where(mask[rows], v, 0.0)
         m = mon_at(rows)
         nn = np.array([cloud.outward_normals[i] for i in rows])
         bdP[rows - Ni] = betas[:, None] * m[R.VAL] + m[R.DX] * nn[:, 0:1] + m[R.DY] * nn[:, 1:2]
@@ -140,14 +147,15 @@ def assemble_bd_Phi_P(cloud, kind, param, M, robin_betas=None):
         o1, o2 = ops_at(i1), ops_at(i2)
         m1, m2 = mon_at(i1), mon_at(i2)
         rpd = i1 - Ni - jump
-        bdPhi[rpd] = o1[R.VAL] - o2[R.VAL]
+        pmask = mask[i1] | mask[i2]        # concatenated supports (219-221, 242-244)
+        bdPhi[rpd] = np.where(pmask, o1[R.VAL] - o2[R.VAL], 0.0)
         bdP[rpd] = m1[R.VAL] - m2[R.VAL]
         n1 = nrm[i1 - Ni - Nd]
         n2 = nrm[i1 - Ni - Nd + nc]
         g1 = o1[R.DX] * n1[:, 0:1] + o1[R.DY] * n1[:, 1:2]
         g2 = o2[R.DX] * (-n2[:, 0:1]) + o2[R.DY] * (-n2[:, 1:2])
         rpn = rpd + half
-        bdPhi[rpn] = g1 - g2
+        bdPhi[rpn] = np.where(pmask, g1 - g2, 0.0)
         nn1 = np.array([cloud.outward_normals[i] for i in i1])
         nn2 = np.array([cloud.outward_normals[i] for i in i2])
         gm1 = m1[R.DX] * nn1[:, 0:1] + m1[R.DY] * nn1[:, 1:2]
@@ -155,7 +163,6 @@ def assemble_bd_Phi_P(cloud, kind, param, M, robin_betas=None):
         bdP[rpn] = gm1 - gm2
         start += nbp
         jump += nc
-    del cols
     return bdPhi, bdP
 
 

@@ -14,15 +14,19 @@ Follows ``updes/cloud.py``:
 The reference draws its jitter from ``jax.random``; here the jitter is an
 explicit (N, 2) array indexed by the original id, which is what the workloads
 use (parity for ``noise_key`` itself is not claimed: parity unpinned).
-The support table of ``define_local_supports`` (cloud.py:83-112) is not built:
-with ``support_size="max"`` it only encodes "every node but itself", which the
-assembly restates directly.
+With ``support_size="max"`` the support table of ``define_local_supports``
+(cloud.py:83-112) is not built: it only encodes "every node but itself".  A
+smaller ``support_size`` builds ``supports`` (new ids) from a brute-force
+nearest-neighbour query (``knn_supports``), or from an explicit table of
+neighbours by original id (``supports=``), e.g. one taken from the same
+BallTree query the reference makes, for grids whose equidistant neighbours
+tie at the cut.
 """
 import numpy as np
 
 
 class SquareCloud:
-    def __init__(self, Nx=7, Ny=5, facet_types=None, jitter=None):
+    def __init__(self, Nx=7, Ny=5, facet_types=None, jitter=None, support_size="max", supports=None):
         if facet_types is None:
             raise ValueError("facet_types is required")
         self.Nx, self.Ny = int(Nx), int(Ny)
@@ -127,6 +131,43 @@ class SquareCloud:
         else:
             self.sorted_outward_normals = np.zeros((0, 2))
         self.original_ids = np.array(order)   # new id -> original id
+
+        # local supports (cloud.py:83-112, 160-163)
+        if support_size in ("max", None):
+            support_size = N
+        if not (0 < int(support_size) <= N):
+            raise AssertionError("support size must be in [1, N]")
+        self.support_size = int(support_size)
+        self.supports = None                  # None: every other node
+        if self.support_size < N:
+            if supports is None:
+                supports = knn_supports(coords, self.support_size)
+            supports = np.asarray(supports, dtype=np.int64)
+            tab = np.empty_like(supports)
+            for g in range(N):
+                tab[new_numb[g]] = [new_numb[int(j)] for j in supports[g]]
+            self.supports = tab               # (N, support_size-1), new ids, rows by new id
+
+
+def knn_supports(coords, k):
+    """The k-1 nearest other nodes of each node, nearest first (original ids).
+
+    ``cloud.py:108-111`` asks scikit-learn's BallTree for the k nearest
+    points of each node and drops the first hit (the node itself).  This is
+    the brute-force statement of the same query: squared distances
+    dx*dx + dy*dy (the reduced distance BallTree compares), the node itself
+    removed, ties between equidistant neighbours broken by lowest original
+    id.  BallTree breaks exact ties by its traversal order instead, so the
+    two agree wherever the k-1-th and k-th distances differ.
+    """
+    coords = np.asarray(coords, dtype=np.float64)
+    N = coords.shape[0]
+    out = np.empty((N, k - 1), dtype=np.int64)
+    for i in range(N):
+        d2 = (coords[:, 0] - coords[i, 0]) ** 2 + (coords[:, 1] - coords[i, 1]) ** 2
+        order = [int(j) for j in np.argsort(d2, kind="stable") if j != i]
+        out[i] = order[:k - 1]
+    return out
 
 
 def interpolate_field(field, cloud1, cloud2):

@@ -30,7 +30,7 @@ def test_library_exports_every_declared_symbol():
     lib = L.load()
     for name in _declared():
         assert hasattr(lib, name), name
-    assert lib.rbfx_abi_version() == 1
+    assert lib.rbfx_abi_version() == 2
     out = subprocess.run(["nm", "-D", "--defined-only", L.LIB_PATH], capture_output=True, text=True).stdout
     exported = set(re.findall(r" T (rbfx_\w+)", out))
     assert set(_declared()) <= exported
@@ -46,7 +46,7 @@ int main(void){
   printf("size %zu\n", sizeof(rbfx_problem));
   F(N) F(M) F(kernel) F(flags) F(param) F(d_x) F(d_y) F(Ni) F(Nd) F(Nn) F(Nr) F(Npd) F(d_nx) F(d_ny)
   F(d_robin_normal_x) F(d_robin_normal_y) F(d_beta) F(d_pd_i1) F(d_pd_i2) F(op_rbf_const) F(op_mon_const)
-  F(d_op_rbf_row) F(d_op_mon_row)
+  F(d_op_rbf_row) F(d_op_mon_row) F(support_cols) F(d_support)
   printf("irsize %zu\n", sizeof(rbfx_ir_stats));
   return 0;
 }

@@ -19,6 +19,7 @@ POLYHARMONIC, GAUSSIAN, THIN_PLATE, MULTIQUADRIC, INV_MULTIQUADRIC = 0, 1, 2, 3,
 VAL, DX, DY, LAP, DXX, DYY, DXY = range(7)
 NOPS = 7
 MAT_K, MAT_A = 0, 1
+FLAG_LOCAL_SUPPORT = 1
 
 _p = ctypes.c_void_p
 _i64 = ctypes.c_int64
@@ -44,6 +45,7 @@ class Problem(ctypes.Structure):
         ("d_beta", _p), ("d_pd_i1", _p), ("d_pd_i2", _p),
         ("op_rbf_const", _d * NOPS), ("op_mon_const", _d * NOPS),
         ("d_op_rbf_row", _p * NOPS), ("d_op_mon_row", _p * NOPS),
+        ("support_cols", _i64), ("d_support", _p),
     ]
 
 

@@ -7,7 +7,9 @@ index arithmetic instead of per-node Python loops and BallTree queries, so a
 ``sorted_local_supports`` table (``cloud.py:83-112, 405``) is not stored:
 with ``support_size="max"`` it only says "every node but itself", which the
 assembly kernels encode directly; ``local_supports`` is produced on demand
-for small clouds.
+for small clouds.  A ``support_size`` below N (RBF-FD style local supports)
+is honoured: the neighbour table is built with the reference's own BallTree
+query and handed to the kernels, which zero every RBF column outside it.
 
 Jitter: the reference draws it from ``jax.random`` with ``noise_key``
 (``cloud.py:431-444``); here ``noise_key`` may be an integer seed (NumPy
@@ -151,6 +153,49 @@ class SquareCloud(Cloud):
         self.facet_nodes = {f: [int(v) for v in new_id[orig_facet_nodes[f]]] for f in self.facet_types}
         self.sorted_nodes = torch.as_tensor(self._nodes_np, device=self._device)
         self.sorted_outward_normals = torch.as_tensor(self._normals_np.copy(), device=self._device)
+        self._define_local_supports(coords)
+
+    def _define_local_supports(self, coords):
+        """The support_size - 1 nearest other nodes of each node (``cloud.py:83-112``).
+
+        "max" / None mean every node (``support_size`` becomes N) and store
+        nothing: the kernels exclude the node itself by index.  A smaller size
+        queries scikit-learn's BallTree exactly as the reference does (built
+        on the coordinates in original id order, ``leaf_size=1``, k =
+        support_size, first hit, the node itself, dropped), so ties between
+        equidistant neighbours of a regular grid resolve the same way; the
+        ids are then renumbered.  ``_support_knn`` keeps the neighbours
+        nearest first (new ids, rows in new-id order), ``_support_sorted``
+        the same sets ascending, as the device kernels search them.
+        """
+        if self.support_size == "max" or self.support_size is None:
+            self.support_size = self.N
+        k = int(self.support_size)
+        if k != self.support_size or k <= 0:
+            raise AssertionError("Support size must be strictly greater than 0")
+        if k > self.N:
+            raise AssertionError("Support size must be strictly less than or equal the number of nodes")
+        self.support_size = k
+        self._support_knn = None
+        self._support_sorted = None
+        if k == self.N:
+            return
+        from sklearn.neighbors import BallTree
+        tree = BallTree(coords, leaf_size=1, metric="euclidean")
+        _, nb = tree.query(coords, k=k)
+        nb = nb[:, 1:]
+        knn = np.empty_like(nb)
+        knn[self._new_id] = self._new_id[nb]
+        self._support_knn = knn.astype(np.int64)
+        self._support_sorted = np.sort(knn, axis=1).astype(np.int32)
+
+    @property
+    def sorted_local_supports(self):
+        """(N, support_size-1) neighbour ids, nearest first, in new-id row order (``cloud.py:405``)."""
+        if self._support_knn is None:
+            raise AttributeError("sorted_local_supports is only stored for support_size < N; "
+                                 "with every node as support it is every other node")
+        return torch.as_tensor(self._support_knn)
 
     # -- dict-valued attributes of the reference, built on demand -------------
     @property
@@ -185,7 +230,10 @@ class SquareCloud(Cloud):
 
     @property
     def local_supports(self):
-        """Every other node, nearest first (``cloud.py:83-112``); O(N^2), small clouds only."""
+        """Neighbours of each node, nearest first (``cloud.py:83-112``).  With
+        every node as support this is O(N^2): small clouds only."""
+        if self._support_knn is not None:
+            return {i: [int(j) for j in self._support_knn[i]] for i in range(self.N)}
         if self.N > 4096:
             raise MemoryError("local_supports is an N x (N-1) table; not built for N > 4096")
         x = self._nodes_np

@@ -9,6 +9,12 @@
 // adjacent columns (16-byte non-temporal stores), loops over a tile of rows,
 // and keeps its two centres in registers; the row context (node, normal,
 // operator coefficients) is wave-uniform.  The kernel is bound by HBM writes.
+//
+// Local supports (support_size < N, updes/cloud.py:83-112): the same kernels,
+// instantiated with LOC, zero every RBF column outside the row's support.  A
+// row's support is an ascending id list; a column outside its [first, last]
+// range is rejected with two compares (almost all of a row), the rest by a
+// binary search in the list, so the dense write stays the bound.
 #include <stdarg.h>
 
 #include "common.h"
@@ -35,6 +41,9 @@ struct ProbDev {
   const double* rrow[RBFX_NOPS];
   const double* mrow[RBFX_NOPS];
   int matrix;
+  bool local;          // RBFX_FLAG_LOCAL_SUPPORT: RBF columns restricted to sup
+  int64_t S;           // support entries per node
+  const int32_t* sup;  // (N, S) ascending node ids
 };
 
 enum RowClass { RC_INT = 0, RC_D, RC_N, RC_R, RC_PD, RC_PN, RC_PT, RC_VAL };
@@ -50,6 +59,10 @@ struct RowCtx {
   double n2x, n2y;  // pn second normal
   double beta;
   OpCoef rc, mc;
+  int64_t i2;               // periodic partner node
+  const int32_t* s1;        // local supports: support row of node i ...
+  const int32_t* s2;        // ... and of the partner (periodic rows)
+  int64_t lo1, hi1, lo2, hi2;  // first / last id of each support row
 };
 
 __device__ inline OpCoef make_coef(const double* c, const double* const* rows, int64_t i) {
@@ -68,8 +81,9 @@ __device__ inline OpCoef make_coef(const double* c, const double* const* rows, i
   return o;
 }
 
-__device__ inline RowCtx row_ctx(const ProbDev& P, int64_t r) {
+__device__ inline RowCtx row_ctx_base(const ProbDev& P, int64_t r) {
   RowCtx c;
+  c.i2 = -1;
   c.mono = 0;
   c.beta = 0.0;
   c.nx = c.ny = c.mnx = c.mny = c.n2x = c.n2y = 0.0;
@@ -124,6 +138,7 @@ __device__ inline RowCtx row_ctx(const ProbDev& P, int64_t r) {
   const int64_t i1 = P.pd1[t], i2 = P.pd2[t];
   c.cls = flux ? RC_PN : RC_PD;
   c.i = i1;
+  c.i2 = i2;
   c.xi = P.x[i1];
   c.yi = P.y[i1];
   c.x2 = P.x[i2];
@@ -137,9 +152,55 @@ __device__ inline RowCtx row_ctx(const ProbDev& P, int64_t r) {
   return c;
 }
 
-// Entry (row context, centre column j < N).
+// Local supports (updes/cloud.py:108-112): the support row of a node and its
+// id range; an empty range when S = 0.  The ids are only compared, never used
+// as addresses, so a malformed table gives wrong zeros but no stray access.
+__device__ inline void support_row(const ProbDev& P, int64_t node, const int32_t*& s, int64_t& lo,
+                                   int64_t& hi) {
+  s = P.sup + node * P.S;
+  if (P.S > 0) {
+    lo = s[0];
+    hi = s[P.S - 1];
+  } else {
+    lo = 1;
+    hi = 0;
+  }
+}
+
+__device__ inline bool in_support(const int32_t* s, int64_t S, int64_t lo, int64_t hi, int64_t j) {
+  if (j < lo || j > hi) return false;  // most columns of a row end here
+  int64_t a = 0, b = S - 1;
+  while (a < b) {
+    const int64_t m = (a + b) >> 1;
+    if (s[m] < j)
+      a = m + 1;
+    else
+      b = m;
+  }
+  return s[a] == j;
+}
+
+template <bool LOC>
+__device__ inline RowCtx row_ctx(const ProbDev& P, int64_t r) {
+  RowCtx c = row_ctx_base(P, r);
+  if (LOC && c.cls != RC_PT) {
+    support_row(P, c.i, c.s1, c.lo1, c.hi1);
+    if (c.cls == RC_PD || c.cls == RC_PN) support_row(P, c.i2, c.s2, c.lo2, c.hi2);
+  }
+  return c;
+}
+
+// Entry (row context, centre column j < N).  With local supports the column
+// must be in the row node's support, or (periodic rows, the concatenated
+// supports of updes/assembly.py:219-221, 242-244) in its partner's.
+template <bool LOC>
 __device__ inline double elem_rbf(const ProbDev& P, const RowCtx& c, int64_t j, double xj,
                                   double yj) {
+  if (LOC && c.cls != RC_PT) {
+    bool in = in_support(c.s1, P.S, c.lo1, c.hi1, j);
+    if (!in && (c.cls == RC_PD || c.cls == RC_PN)) in = in_support(c.s2, P.S, c.lo2, c.hi2, j);
+    if (!in) return 0.0;
+  }
   switch (c.cls) {
     case RC_INT:
       return j == c.i ? 0.0 : apply_op(P.k, c.rc, c.xi - xj, c.yi - yj);
@@ -216,11 +277,6 @@ __device__ inline double elem_mono(const RowCtx& c, int kid) {
   }
 }
 
-__device__ inline double elem(const ProbDev& P, const RowCtx& c, int64_t j) {
-  if (j < P.N) return elem_rbf(P, c, j, P.x[j], P.y[j]);
-  return elem_mono(c, (int)(j - P.N));
-}
-
 typedef double vd2 __attribute__((ext_vector_type(2)));
 typedef float vf2 __attribute__((ext_vector_type(2)));
 
@@ -228,7 +284,7 @@ constexpr int kAsmThreads = 256;
 constexpr int kAsmCols = 2 * kAsmThreads;  // columns per block
 constexpr int kAsmRows = 16;               // rows per block
 
-template <int KIND, bool FAST, bool W64, bool W32>
+template <int KIND, bool FAST, bool LOC, bool W64, bool W32>
 __global__ __launch_bounds__(kAsmThreads) void assemble_kernel(ProbDev P, int64_t row0,
                                                                int64_t nrows, double* out64,
                                                                int64_t ld64, float* out32,
@@ -244,10 +300,10 @@ __global__ __launch_bounds__(kAsmThreads) void assemble_kernel(ProbDev P, int64_
   const int64_t rend = rb + kAsmRows < nrows ? rb + kAsmRows : nrows;
   for (int64_t rr = rb; rr < rend; ++rr) {
     const int64_t r = row0 + rr;
-    const RowCtx c = row_ctx(P, r);
-    const double v0 = c0rbf ? elem_rbf(P, c, j0, xj0, yj0) : elem_mono(c, (int)(j0 - P.N));
+    const RowCtx c = row_ctx<LOC>(P, r);
+    const double v0 = c0rbf ? elem_rbf<LOC>(P, c, j0, xj0, yj0) : elem_mono(c, (int)(j0 - P.N));
     double v1 = 0.0;
-    if (two) v1 = c1rbf ? elem_rbf(P, c, j0 + 1, xj1, yj1) : elem_mono(c, (int)(j0 + 1 - P.N));
+    if (two) v1 = c1rbf ? elem_rbf<LOC>(P, c, j0 + 1, xj1, yj1) : elem_mono(c, (int)(j0 + 1 - P.N));
     if (W64) {
       double* dst = out64 + rr * ld64 + j0;
       if (two && (((uintptr_t)dst & 15) == 0)) {
@@ -275,7 +331,7 @@ __global__ __launch_bounds__(kAsmThreads) void assemble_kernel(ProbDev P, int64_
 constexpr int kMvThreads = 256;
 constexpr int kMvRows = 4;
 
-template <int KIND, bool FAST>
+template <int KIND, bool FAST, bool LOC>
 __global__ __launch_bounds__(kMvThreads) void matvec_kernel(ProbDev P, const double* __restrict__ xv,
                                                             double* __restrict__ yv) {
   pin_kind<KIND, FAST>(P.k);
@@ -285,7 +341,7 @@ __global__ __launch_bounds__(kMvThreads) void matvec_kernel(ProbDev P, const dou
 #pragma unroll
   for (int q = 0; q < kMvRows; ++q) {
     const int64_t r = r0 + q < P.n ? r0 + q : P.n - 1;
-    c[q] = row_ctx(P, r);
+    c[q] = row_ctx<LOC>(P, r);
     acc[q] = 0.0;
   }
   for (int64_t j = threadIdx.x; j < P.n; j += kMvThreads) {
@@ -293,7 +349,7 @@ __global__ __launch_bounds__(kMvThreads) void matvec_kernel(ProbDev P, const dou
     if (j < P.N) {
       const double xj = P.x[j], yj = P.y[j];
 #pragma unroll
-      for (int q = 0; q < kMvRows; ++q) acc[q] += elem_rbf(P, c[q], j, xj, yj) * w;
+      for (int q = 0; q < kMvRows; ++q) acc[q] += elem_rbf<LOC>(P, c[q], j, xj, yj) * w;
     } else {
 #pragma unroll
       for (int q = 0; q < kMvRows; ++q) acc[q] += elem_mono(c[q], (int)(j - P.N)) * w;
@@ -319,7 +375,7 @@ __global__ __launch_bounds__(kMvThreads) void matvec_kernel(ProbDev P, const dou
 // Partial y^T = x^T K over a chunk of rows: part[chunk][j].
 constexpr int kMtThreads = 256;
 
-template <int KIND, bool FAST>
+template <int KIND, bool FAST, bool LOC>
 __global__ __launch_bounds__(kMtThreads) void matvec_t_kernel(ProbDev P, const double* __restrict__ xv,
                                                               int64_t rows_per_chunk,
                                                               double* __restrict__ part) {
@@ -333,8 +389,8 @@ __global__ __launch_bounds__(kMtThreads) void matvec_t_kernel(ProbDev P, const d
   const double xj = isrbf ? P.x[jj] : 0.0, yj = isrbf ? P.y[jj] : 0.0;
   double acc = 0.0;
   for (int64_t r = rs; r < re; ++r) {
-    const RowCtx c = row_ctx(P, r);
-    const double e = isrbf ? elem_rbf(P, c, jj, xj, yj) : elem_mono(c, (int)(jj - P.N));
+    const RowCtx c = row_ctx<LOC>(P, r);
+    const double e = isrbf ? elem_rbf<LOC>(P, c, jj, xj, yj) : elem_mono(c, (int)(jj - P.N));
     acc += e * xv[r];
   }
   if (valid) part[blockIdx.y * P.n + j] = acc;
@@ -381,6 +437,18 @@ static int make_prob(const rbfx_problem* p, int matrix, ProbDev& P) {
     P.mrow[t] = p->d_op_mon_row[t];
   }
   P.matrix = matrix;
+  RBFX_REQUIRE((p->flags & ~(int32_t)RBFX_FLAG_LOCAL_SUPPORT) == 0, "unknown flags 0x%x", p->flags);
+  P.local = (p->flags & RBFX_FLAG_LOCAL_SUPPORT) != 0;
+  P.S = 0;
+  P.sup = nullptr;
+  if (P.local) {
+    RBFX_REQUIRE(p->support_cols >= 0 && p->support_cols < p->N,
+                 "support_cols=%lld must be in [0, N-1=%lld]", (long long)p->support_cols,
+                 (long long)(p->N - 1));
+    RBFX_REQUIRE(p->support_cols == 0 || p->d_support, "support table missing");
+    P.S = p->support_cols;
+    P.sup = p->d_support;
+  }
   if (matrix == RBFX_MAT_K) {
     RBFX_REQUIRE(p->Ni + p->Nd + p->Nn + p->Nr + 2 * p->Npd == p->N,
                  "row classes do not add up: Ni+Nd+Nn+Nr+2Npd=%lld != N=%lld",
@@ -399,6 +467,20 @@ static int make_prob(const rbfx_problem* p, int matrix, ProbDev& P) {
 
 using namespace rbfx;
 
+template <int KIND, bool FAST, bool LOC>
+static void launch_assemble(dim3 grid, hipStream_t s, const ProbDev& P, int64_t row0, int64_t nrows,
+                            double* out64, int64_t ld64, float* out32, int64_t ld32) {
+  if (out64 && out32)
+    hipLaunchKernelGGL((assemble_kernel<KIND, FAST, LOC, true, true>), grid, dim3(kAsmThreads), 0, s, P, row0,
+                       nrows, out64, ld64, out32, ld32);
+  else if (out64)
+    hipLaunchKernelGGL((assemble_kernel<KIND, FAST, LOC, true, false>), grid, dim3(kAsmThreads), 0, s, P, row0,
+                       nrows, out64, ld64, out32, ld32);
+  else
+    hipLaunchKernelGGL((assemble_kernel<KIND, FAST, LOC, false, true>), grid, dim3(kAsmThreads), 0, s, P, row0,
+                       nrows, out64, ld64, out32, ld32);
+}
+
 extern "C" int rbfx_assemble(const rbfx_problem* p, int matrix, int64_t row0, int64_t nrows,
                              double* d_out64, int64_t ld64, float* d_out32, int64_t ld32,
                              void* stream) {
@@ -415,15 +497,10 @@ extern "C" int rbfx_assemble(const rbfx_problem* p, int matrix, int64_t row0, in
   hipStream_t s = as_stream(stream);
   const int pid = prof_start(d_out64 ? (d_out32 ? "assemble_f64_f32" : "assemble_f64") : "assemble_f32", s);
   RBFX_KIND_SWITCH(P.k, {
-    if (d_out64 && d_out32)
-      hipLaunchKernelGGL((assemble_kernel<KIND_, FAST_, true, true>), grid, dim3(kAsmThreads), 0, s, P, row0,
-                         nrows, d_out64, ld64, d_out32, ld32);
-    else if (d_out64)
-      hipLaunchKernelGGL((assemble_kernel<KIND_, FAST_, true, false>), grid, dim3(kAsmThreads), 0, s, P, row0,
-                         nrows, d_out64, ld64, d_out32, ld32);
+    if (P.local)
+      launch_assemble<KIND_, FAST_, true>(grid, s, P, row0, nrows, d_out64, ld64, d_out32, ld32);
     else
-      hipLaunchKernelGGL((assemble_kernel<KIND_, FAST_, false, true>), grid, dim3(kAsmThreads), 0, s, P, row0,
-                         nrows, d_out64, ld64, d_out32, ld32);
+      launch_assemble<KIND_, FAST_, false>(grid, s, P, row0, nrows, d_out64, ld64, d_out32, ld32);
   });
   prof_stop(pid, s, 0.0, (double)nrows * P.n * ((d_out64 ? 8.0 : 0.0) + (d_out32 ? 4.0 : 0.0)));
   RBFX_CHECK_LAUNCH();
@@ -457,9 +534,12 @@ extern "C" int rbfx_matvec(const rbfx_problem* p, int matrix, int trans, const d
   hipStream_t s = as_stream(stream);
   if (!trans) {
     const int pid = prof_start("matvec", s);
+    const dim3 grid((unsigned)ceil_div(P.n, kMvRows));
     RBFX_KIND_SWITCH(P.k, {
-      hipLaunchKernelGGL((matvec_kernel<KIND_, FAST_>), dim3((unsigned)ceil_div(P.n, kMvRows)), dim3(kMvThreads),
-                         0, s, P, d_x, d_y);
+      if (P.local)
+        hipLaunchKernelGGL((matvec_kernel<KIND_, FAST_, true>), grid, dim3(kMvThreads), 0, s, P, d_x, d_y);
+      else
+        hipLaunchKernelGGL((matvec_kernel<KIND_, FAST_, false>), grid, dim3(kMvThreads), 0, s, P, d_x, d_y);
     });
     prof_stop(pid, s, 0.0, 16.0 * P.n);
     RBFX_CHECK_LAUNCH();
@@ -472,9 +552,12 @@ extern "C" int rbfx_matvec(const rbfx_problem* p, int matrix, int trans, const d
   const int64_t rpc = ceil_div(P.n, chunks);
   const int64_t used_chunks = ceil_div(P.n, rpc);
   double* part = static_cast<double*>(d_work);
+  const dim3 grid((unsigned)ceil_div(P.n, kMtThreads), (unsigned)used_chunks);
   RBFX_KIND_SWITCH(P.k, {
-    hipLaunchKernelGGL((matvec_t_kernel<KIND_, FAST_>), dim3((unsigned)ceil_div(P.n, kMtThreads),
-                       (unsigned)used_chunks), dim3(kMtThreads), 0, s, P, d_x, rpc, part);
+    if (P.local)
+      hipLaunchKernelGGL((matvec_t_kernel<KIND_, FAST_, true>), grid, dim3(kMtThreads), 0, s, P, d_x, rpc, part);
+    else
+      hipLaunchKernelGGL((matvec_t_kernel<KIND_, FAST_, false>), grid, dim3(kMtThreads), 0, s, P, d_x, rpc, part);
   });
   RBFX_CHECK_LAUNCH();
   hipLaunchKernelGGL(sum_chunks_kernel, dim3((unsigned)ceil_div(P.n, 256)), dim3(256), 0, s, part,

@@ -61,6 +61,28 @@ class DeviceCloud:
             self.pd1 = self.pd2 = None
         if self.Ni + self.Nd + self.Nn + self.Nr + 2 * self.Npd != self.N:
             raise ValueError("periodic facets must pair up node for node (odd periodic group size)")
+        # local supports (support_size < N): (N, S) ascending neighbour ids
+        sup = support_table(cloud)
+        self.local = sup is not None
+        self.S = 0 if sup is None else int(sup.shape[1])
+        self.sup = None
+        if self.local and self.S > 0:
+            self.sup = torch.as_tensor(sup, dtype=torch.int32).to(device).contiguous()
+
+
+def support_table(cloud):
+    """Ascending (N, support_size-1) int32 neighbour table, or None when every
+    node is in every support (``updes/cloud.py:97-112``)."""
+    sup = cloud.__dict__.get("_support_sorted", False)
+    if sup is not False:
+        return sup
+    k = getattr(cloud, "support_size", "max")
+    if k in ("max", None) or int(k) >= cloud.N:
+        return None
+    sls = np.asarray(torch.as_tensor(cloud.sorted_local_supports).cpu().numpy(), dtype=np.int64)
+    if sls.shape != (cloud.N, int(k) - 1) or sls.min(initial=0) < 0 or sls.max(initial=0) >= cloud.N:
+        raise ValueError("sorted_local_supports must be (N, support_size-1) node ids")
+    return np.sort(sls, axis=1).astype(np.int32)
 
 
 def device_cloud(cloud):
@@ -127,6 +149,10 @@ class Problem:
             p.d_beta = self.beta.data_ptr()
         p.d_pd_i1 = dc.pd1.data_ptr() if dc.pd1 is not None else None
         p.d_pd_i2 = dc.pd2.data_ptr() if dc.pd2 is not None else None
+        if dc.local:
+            p.flags |= L.FLAG_LOCAL_SUPPORT
+            p.support_cols = dc.S
+            p.d_support = dc.sup.data_ptr() if dc.sup is not None else None
         self.desc = desc.to(dc.x.device) if desc is not None else OpDesc()
         for t in range(L.NOPS):
             p.op_rbf_const[t] = float(self.desc.rbf_const[t])
