@@ -18,12 +18,16 @@ and the error of the mixed path are reported beside it.
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--side 200]
                        [--precisions fp64,fp32_ir] [--copy-probe-mib M]
                        [--gemm-replay] [--no-cpu-baseline] [--no-kernel-profile]
+                       [--support-size K]
 --copy-probe-mib runs one plain device copy of M MiB before the workload, so a
 counter run (FETCH_SIZE / WRITE_SIZE) can be scaled by a copy of known size.
 --gemm-replay only replays the trailing-update GEMM launches of the FP64 LU
 (for counter runs on that kernel alone).  The `roofline` object is that
 trailing-update GEMM (the dominant kernel), timed with HIP events on its
 launch stream during one extra profiled step after the timed ones.
+--support-size K runs the same workload on local supports of K nodes (each
+RBF row restricted to the node's K-1 nearest neighbours, the reference's
+support_size < N); the default is every node ("max").
 """
 import argparse
 import ctypes
@@ -44,11 +48,11 @@ PEAK_FP32_MFMA = 157.3e12
 PEAK_HBM = 8.0e12
 
 
-def fourier_problem(side, seed=0):
+def fourier_problem(side, seed=0, support_size="max"):
     """Cloud, boundary data and source basis of the large-Poisson workload."""
     import updes_amd as u
     ft = {"North": "d", "South": "n", "East": "d", "West": "d"}
-    cloud = u.SquareCloud(Nx=side, Ny=side, facet_types=ft)
+    cloud = u.SquareCloud(Nx=side, Ny=side, facet_types=ft, support_size=support_size)
     rng = np.random.default_rng(seed)
     k = np.arange(1, 9)
     bcs = {}
@@ -192,6 +196,8 @@ def main():
     ap.add_argument("--precisions", default="fp64,fp32_ir")
     ap.add_argument("--copy-probe-mib", type=int, default=0)
     ap.add_argument("--no-kernel-profile", action="store_true")
+    ap.add_argument("--support-size", default="max",
+                    help="local support size K (< N) or 'max' (every node, the default)")
     ap.add_argument("--gemm-replay", action="store_true",
                     help="run only the FP64 factorisation's GEMM launches (same shapes and operand blocks) "
                          "on one matrix, for counter runs of the GEMM alone")
@@ -219,7 +225,10 @@ def main():
     if args.gemm_replay:
         gemm_replay(args.side * args.side + 6)
         return
-    cloud, bcs, b, basis = fourier_problem(args.side)
+    support = args.support_size if args.support_size == "max" else int(args.support_size)
+    t_cloud = time.perf_counter()
+    cloud, bcs, b, basis = fourier_problem(args.side, support_size=support)
+    t_cloud = time.perf_counter() - t_cloud
     n = cloud.N + 6
     print("[bench] N=%d n=%d, precisions %s" % (cloud.N, n, args.precisions), file=sys.stderr, flush=True)
 
@@ -269,6 +278,7 @@ def main():
     out["config"] = {"workload": "large Poisson %dx%d (N=%d, n=%d), r^3, max_degree 2, Neumann South, Fourier "
                                  "boundary data and 64-coefficient source; step = 1 solve + 1 adjoint gradient"
                                  % (args.side, args.side, cloud.N, n), "N": cloud.N, "n": n,
+                     "support_size": cloud.support_size, "cloud_build_s": t_cloud,
                      "precision": head, "factorisation_used": used, "grads_per_s": 1e3 / r["ms"]}
     for prec, rr in results.items():
         d = {"solves_per_s": 1e3 / rr["ms"], "grads_per_s": 1e3 / rr["ms"], "ms_per_solve_plus_grad": rr["ms"],
