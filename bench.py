@@ -203,9 +203,8 @@ def main():
                          "on one matrix, for counter runs of the GEMM alone")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
-        print("bench.py measures one GPU: the dense solve of one collocation system runs on a single MI355X; "
-              "multi-GPU runs are not part of this build (see DESIGN.md for how the path would shard).",
-              file=sys.stderr)
+        print("bench.py measures one GPU: the dense solve of one collocation system runs on a single MI355X, "
+              "and there is no multi-GPU path to measure.", file=sys.stderr)
         sys.exit(2)
 
     import updes_amd as u

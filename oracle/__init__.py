@@ -3,8 +3,8 @@
 This package is the oracle: a plain NumPy/SciPy restatement of the reference
 algorithm (ddrous/Updes v1.0.3, ``updes/{utils,cloud,assembly,operators}.py``)
 used to check the HIP path.  Only ``tests/``, ``__graft_entry__.smoke()`` and the
-``cpu_baseline`` leg of ``bench.py`` may import it, and only as the checker or
-as the timed CPU baseline.  The product package ``updes_amd`` never imports it
+``cpu_baseline`` leg of ``bench.py`` may import it, and only to check results
+or as the timed CPU baseline.  The product package ``updes_amd`` never imports it
 and has no CPU fallback.
 
 Modules

@@ -1,7 +1,7 @@
 """Writes the golden vectors under tests/golden from the pinned oracle.
 
-The reference's own tests hold no vectors for pde_solver (SURVEY: their
-tightest solve tolerance is 1e-2), and the reference (JAX) cannot run here,
+The reference's own tests hold no vectors for pde_solver (their tightest
+solve tolerance is 1e-2), and the reference (JAX) cannot run here,
 so these fixtures are produced by the oracle after it has passed its pins
 (tests/test_oracle_pins.py: SymPy/mpmath closed forms, an independent
 mpmath brute force of the reference loops, invariants).  Run:

@@ -241,7 +241,8 @@ def test_value_at_nodes_includes_self_term():
 
 
 def test_gaussian_zero_diagonal_conditioning():
-    """The zero self-interaction keeps the Gaussian system well posed (SURVEY key finding 3)."""
+    """The zero self-interaction (every support omits the node itself,
+    ``cloud.py:110-111``) keeps the Gaussian system well posed."""
     oc = OC.SquareCloud(20, 20, {"North": "d", "South": "d", "East": "p1", "West": "p1"})
     A = OA.assemble_A(oc, R.GAUSSIAN, 5.0, 3)
     assert np.linalg.cond(A) < 1e8

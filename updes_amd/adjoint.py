@@ -1,7 +1,7 @@
 """Differentiation through the collocation solve (torch.autograd).
 
 The reference differentiates ``pde_solver`` with ``jax.grad`` through
-lineax's implicit linear-solve rule and XLA's ``inv`` (SURVEY call stack
+lineax's implicit linear-solve rule and XLA's ``inv`` (as in
 ``demos/Laplace/10_laplace_with_diff_phys.py:89-104``): for B u = q the
 cotangent of q is B^-T dL/du.  Here the solve is the augmented system
 K c = [q; 0] and the adjoint reuses the LU factors of K through the
