@@ -89,12 +89,15 @@ def test_local_assembly_A_matches_oracle(kind, param):
     assert not np.any(A[:oc.N, :oc.N][~OA.support_mask(oc)])
 
 
+@pytest.mark.parametrize("k", [17, "max"])
 @pytest.mark.parametrize("fs", [1, 3, 4, 5])
-def test_local_matvecs_match_assembled(fs):
-    """y = K x, K^T x, A x, A^T x recomputed matrix-free with the support test."""
+def test_local_matvecs_match_assembled(fs, k):
+    """y = K x, K^T x, A x, A^T x recomputed matrix-free with the support test
+    (and, with every node as support, A^T x through the symmetric A)."""
     import updes_amd as u
     from updes_amd import _lib as L, engine as E
-    uc, _ = _pair(33, 31, C.FACET_SETS[fs], 17, seed=fs)
+    uc = u.SquareCloud(Nx=33, Ny=31, facet_types=C.FACET_SETS[fs], jitter=C.jitter(33, 31, 0.2, fs),
+                       support_size=k)
     dc = E.device_cloud(uc)
     beta = torch.linspace(0.5, 1.5, dc.Nr, dtype=torch.float64) if dc.Nr else None
     prob = E.Problem(dc, L.POLYHARMONIC, 1.0, 6, E.OpDesc.builtin(L.LAP), beta)

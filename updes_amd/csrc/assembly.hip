@@ -532,7 +532,10 @@ extern "C" int rbfx_matvec(const rbfx_problem* p, int matrix, int trans, const d
   if (st) return st;
   RBFX_REQUIRE(d_x && d_y && d_x != d_y, "matvec needs distinct x and y");
   hipStream_t s = as_stream(stream);
-  if (!trans) {
+  // A = [[Phi P];[P^T 0]] is symmetric entry for entry when every node is in
+  // every support (phi depends on |x_i - x_j| only): A^T x takes the row kernel
+  const bool symmetric = matrix == RBFX_MAT_A && !P.local;
+  if (!trans || symmetric) {
     const int pid = prof_start("matvec", s);
     const dim3 grid((unsigned)ceil_div(P.n, kMvRows));
     RBFX_KIND_SWITCH(P.k, {
