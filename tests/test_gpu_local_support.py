@@ -229,3 +229,51 @@ def test_large_poisson_40k_local_supports():
     print("N=40k, support 20: backward error %.3e, loss %.6e" % (eta, loss.item()), flush=True)
     assert np.isfinite(loss.item())
     assert eta <= n * np.finfo(float).eps * 100
+
+
+def test_local_time_step_and_gradient():
+    """One implicit advection-diffusion step with rhs value(x, coeffs(u^n))
+    on local supports: the coefficients come from A with the local Phi
+    (``assembly.py:404-410``), the rhs evaluation sums every centre
+    (``operators.py:118-150``).  Values and d(w . u^{n+1})/du^n against the
+    oracle's B-path with the same supports."""
+    import functools
+    import updes_amd as u
+    nx, k, M = 16, 20, 3
+    ft = {"North": "d", "South": "d", "West": "p1", "East": "p1"}
+    uc, oc = _pair(nx, nx, ft, k, seed=7)
+    dt, nu, vel = 1e-2, 0.05, (1.0, 0.5)
+    V = torch.tensor(vel, dtype=torch.float64)
+
+    def diff(x, center=None, rbf=None, monomial=None, fields=None):
+        return (u.nodal_value(x, center, rbf, monomial) + dt * torch.dot(V, u.nodal_gradient(x, center, rbf, monomial))
+                - dt * nu * u.nodal_laplacian(x, center, rbf, monomial))
+
+    def rhs(x, centers=None, rbf=None, fields=None):
+        return u.value(x, fields[:, 0], centers, rbf)
+
+    bcs = {f: 0.0 for f in ft}
+    xy = uc.sorted_nodes.cuda()
+    u0 = torch.exp(-((xy[:, 0] - 0.3) ** 2 + (xy[:, 1] - 0.5) ** 2) / (2 * 0.15 ** 2))
+    w = torch.as_tensor(np.random.default_rng(3).standard_normal(uc.N), device="cuda")
+    u.clear_cache()
+    a = u0.clone().requires_grad_(True)
+    u1 = u.pde_solver(diff, rhs, uc, bcs, u.polyharmonic, 1, rhs_args=[a]).vals
+    (u1 * w).sum().backward()
+    # oracle: u1 = B^-1 q, q_int = Eval(x_int) A^-1 [u0; 0], boundary rows 0
+    N, Ni = oc.N, oc.Ni
+    desc = C.adv_diff_desc(dt, nu, vel)
+    B = OA.assemble_B(desc, oc, R.POLYHARMONIC, 1.0, M)
+    invA = np.linalg.inv(OA.assemble_A(oc, R.POLYHARMONIC, 1.0, M))
+    Ev = np.stack([OS.value(oc.sorted_nodes[:Ni], e, oc.sorted_nodes, R.POLYHARMONIC, 1.0)
+                   for e in np.eye(N + M)], axis=1)                # (Ni, N+M)
+    Q = np.zeros((N, N))
+    Q[:Ni] = Ev @ invA[:, :N]                                      # dq/du0
+    u0n = u0.cpu().numpy()
+    u_ref = OS.qr_solve(B, Q @ u0n)
+    g_ref = Q.T @ np.linalg.solve(B.T, w.cpu().numpy())
+    tol = max(1e-9, 10 * np.linalg.cond(B) * np.finfo(float).eps)
+    e_u = np.linalg.norm(u1.detach().cpu().numpy() - u_ref) / np.linalg.norm(u_ref)
+    e_g = np.linalg.norm(a.grad.cpu().numpy() - g_ref) / np.linalg.norm(g_ref)
+    print("local time step: vals %.2e grad %.2e (tol %.1e)" % (e_u, e_g, tol), flush=True)
+    assert e_u <= tol and e_g <= tol
