@@ -877,6 +877,23 @@ __global__ __launch_bounds__(64) void swap_plan_kernel(const int64_t* __restrict
   }
 }
 
+// This workgroup's share (blockIdx.x of gridDim.x) of a copy of nc
+// entries, with 16-byte accesses when both ends are 16-byte aligned.
+template <typename T>
+__device__ inline void copy_row_part(const T* __restrict__ s, T* __restrict__ d, int64_t nc) {
+  constexpr int E = 16 / sizeof(T);
+  typedef T V __attribute__((ext_vector_type(E)));
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  const int64_t t0 = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (((reinterpret_cast<uintptr_t>(s) | reinterpret_cast<uintptr_t>(d)) & 15) == 0) {
+    const int64_t nv = nc / E;
+    for (int64_t v = t0; v < nv; v += stride) reinterpret_cast<V*>(d)[v] = reinterpret_cast<const V*>(s)[v];
+    for (int64_t c = nv * E + t0; c < nc; c += stride) d[c] = s[c];
+  } else {
+    for (int64_t c = t0; c < nc; c += stride) d[c] = s[c];
+  }
+}
+
 template <typename T>
 __global__ __launch_bounds__(256) void swap_gather_kernel(const T* __restrict__ A, int64_t lda,
                                                           int64_t c0, int64_t nc,
@@ -885,9 +902,7 @@ __global__ __launch_bounds__(256) void swap_gather_kernel(const T* __restrict__ 
                                                           T* __restrict__ tmp) {
   const int q = blockIdx.y;
   if (q >= *count) return;
-  const int64_t row = src[q];
-  for (int64_t c = (int64_t)blockIdx.x * 256 + threadIdx.x; c < nc; c += (int64_t)gridDim.x * 256)
-    tmp[(int64_t)q * nc + c] = A[row * lda + c0 + c];
+  copy_row_part<T>(A + src[q] * lda + c0, tmp + (int64_t)q * nc, nc);
 }
 
 template <typename T>
@@ -898,9 +913,7 @@ __global__ __launch_bounds__(256) void swap_scatter_kernel(T* __restrict__ A, in
                                                            const T* __restrict__ tmp) {
   const int q = blockIdx.y;
   if (q >= *count) return;
-  const int64_t row = dst[q];
-  for (int64_t c = (int64_t)blockIdx.x * 256 + threadIdx.x; c < nc; c += (int64_t)gridDim.x * 256)
-    A[row * lda + c0 + c] = tmp[(int64_t)q * nc + c];
+  copy_row_part<T>(tmp + (int64_t)q * nc, A + dst[q] * lda + c0, nc);
 }
 
 // ---------------------------------------------------------------------------
@@ -1141,7 +1154,8 @@ static int apply_swaps(LuCtx<T>& c, int64_t s, int64_t e, int64_t c0, int64_t nc
   const int64_t slice = std::max<int64_t>(256, c.sc.swap_tmp_elems / maxcnt);
   for (int64_t off = 0; off < nc; off += slice) {
     const int64_t w = std::min<int64_t>(slice, nc - off);
-    dim3 grid((unsigned)std::min<int64_t>(ceil_div(w, 256), 64), (unsigned)maxcnt);
+    // a workgroup moves at least 4 x 16 bytes per thread of a row
+    dim3 grid((unsigned)std::min<int64_t>(ceil_div(w, 256 * 4 * (16 / (int64_t)sizeof(T))), 64), (unsigned)maxcnt);
     hipLaunchKernelGGL((swap_gather_kernel<T>), grid, dim3(256), 0, c.s, c.A, c.lda, c0 + off, w,
                        c.sc.plan_src, c.sc.plan_count, c.sc.swap_tmp);
     RBFX_CHECK_LAUNCH();
