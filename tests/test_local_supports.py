@@ -75,8 +75,12 @@ def test_support_size_arguments():
     for s in ("max", None, 30):
         c = u.SquareCloud(Nx=6, Ny=5, facet_types=ft, support_size=s)
         assert c.support_size == 30
-        with pytest.raises(AttributeError):
-            c.sorted_local_supports
+        # every other node, nearest first, built on demand; no table for the kernels
+        tab = c.sorted_local_supports.numpy()
+        assert tab.shape == (30, 29)
+        assert np.array_equal(np.sort(tab, axis=1), np.array([np.delete(np.arange(30), i) for i in range(30)]))
+        from updes_amd import engine as E
+        assert E.support_table(c) is None
     with pytest.raises(AssertionError):
         u.SquareCloud(Nx=6, Ny=5, facet_types=ft, support_size=0)
     with pytest.raises(AssertionError):

@@ -191,10 +191,12 @@ class SquareCloud(Cloud):
 
     @property
     def sorted_local_supports(self):
-        """(N, support_size-1) neighbour ids, nearest first, in new-id row order (``cloud.py:405``)."""
+        """(N, support_size-1) neighbour ids, nearest first, in new-id row order
+        (``cloud.py:405``).  With every node as support the N x (N-1) table is
+        built on demand, for small clouds only (as ``local_supports``)."""
         if self._support_knn is None:
-            raise AttributeError("sorted_local_supports is only stored for support_size < N; "
-                                 "with every node as support it is every other node")
+            ls = self.local_supports
+            return torch.as_tensor(np.array([ls[i] for i in range(self.N)], dtype=np.int64).reshape(self.N, -1))
         return torch.as_tensor(self._support_knn)
 
     # -- dict-valued attributes of the reference, built on demand -------------
