@@ -932,29 +932,31 @@ constexpr int kNB = 512;  // outer panel width of the blocked LU
 //   2. block rows I = 1..3 in turn: S = L[I, 0:I] X[0:I, 0:I], then
 //      X[I, 0:I] = -X_II S (two barriers per block row).
 // Rows and columns from sz to 64 are padded with the identity.
-template <typename T>
+// NT = 16 for the 16-wide leaves: the 16 x 16 substitution alone, without
+// the three empty block-row joins of the 64 x 64 case.
+template <typename T, int NT>
 __global__ __launch_bounds__(256) void trtri_diag_kernel(const T* __restrict__ L, int64_t ldl, int sz,
                                                         T* __restrict__ X, int64_t ldx) {
-  __shared__ double Ls[kTri * (kTri - 1) / 2];  // strictly lower part, packed by rows
-  __shared__ double Xs[kTri][kTri + 1];
-  __shared__ double Ss[16][kTri + 1];
+  __shared__ double Ls[NT * (NT - 1) / 2];  // strictly lower part, packed by rows
+  __shared__ double Xs[NT][NT + 1];
+  __shared__ double Ss[16][NT + 1];
   const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
   auto lidx = [](int i, int k) { return i * (i - 1) / 2 + k; };
   // whole rows per pass (coalesced), all 16 loads in flight
-  T lv[kTri * kTri / 256];
+  T lv[NT * NT / 256];
 #pragma unroll
-  for (int q = 0; q < kTri * kTri / 256; ++q) {
-    const int e = q * 256 + t, i = e / kTri, k = e % kTri;
+  for (int q = 0; q < NT * NT / 256; ++q) {
+    const int e = q * 256 + t, i = e / NT, k = e % NT;
     lv[q] = (k < i && i < sz) ? L[(int64_t)i * ldl + k] : T(0);
   }
 #pragma unroll
-  for (int q = 0; q < kTri * kTri / 256; ++q) {
-    const int e = q * 256 + t, i = e / kTri, k = e % kTri;
+  for (int q = 0; q < NT * NT / 256; ++q) {
+    const int e = q * 256 + t, i = e / NT, k = e % NT;
     if (k < i) Ls[lidx(i, k)] = (double)lv[q];
     Xs[i][k] = 0.0;
   }
   __syncthreads();
-  {
+  if (16 * wv < NT) {
     const int c = lane & 15, r4 = lane >> 4, o = 16 * wv;
     double y[4];
 #pragma unroll
@@ -972,7 +974,7 @@ __global__ __launch_bounds__(256) void trtri_diag_kernel(const T* __restrict__ L
     for (int q = 0; q < 4; ++q) Xs[o + 4 * q + r4][o + c] = y[q];
   }
   __syncthreads();
-  for (int I = 1; I < 4; ++I) {
+  for (int I = 1; I < NT / 16; ++I) {
     const int nI = 16 * I, oI = 16 * I;
     for (int e = t; e < 16 * nI; e += 256) {
       const int i = e / nI, j = e % nI;
@@ -1135,8 +1137,12 @@ template <typename T>
 static int leaf_inverse(LuCtx<T>& c, int64_t r0, int w, T* X, int64_t ldx) {
   RBFX_REQUIRE(w <= kTri, "leaf wider than the diagonal inverse kernel");
   const int pid = prof_start(sizeof(T) == 8 ? "trtri_f64" : "trtri_f32", c.s);
-  hipLaunchKernelGGL((trtri_diag_kernel<T>), dim3(1), dim3(256), 0, c.s, c.A + r0 * c.lda + r0, c.lda, w, X,
-                     ldx);
+  if (w <= 16)
+    hipLaunchKernelGGL((trtri_diag_kernel<T, 16>), dim3(1), dim3(256), 0, c.s, c.A + r0 * c.lda + r0, c.lda, w,
+                       X, ldx);
+  else
+    hipLaunchKernelGGL((trtri_diag_kernel<T, kTri>), dim3(1), dim3(256), 0, c.s, c.A + r0 * c.lda + r0, c.lda, w,
+                       X, ldx);
   RBFX_CHECK_LAUNCH();
   prof_stop(pid, c.s, (double)w * w * w / 3.0, (double)sizeof(T) * w * w);
   return RBFX_OK;
