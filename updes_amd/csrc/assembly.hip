@@ -153,8 +153,9 @@ __device__ inline RowCtx row_ctx_base(const ProbDev& P, int64_t r) {
 }
 
 // Local supports (updes/cloud.py:108-112): the support row of a node and its
-// id range; an empty range when S = 0.  The ids are only compared, never used
-// as addresses, so a malformed table gives wrong zeros but no stray access.
+// id range; an empty range when S = 0.  The dense kernels only compare the
+// ids; the support-only product reads the nodes they name and skips ids
+// outside [0, N), so a malformed table gives wrong values but no stray access.
 __device__ inline void support_row(const ProbDev& P, int64_t node, const int32_t*& s, int64_t& lo,
                                    int64_t& hi) {
   s = P.sup + node * P.S;
@@ -372,6 +373,54 @@ __global__ __launch_bounds__(kMvThreads) void matvec_kernel(ProbDev P, const dou
   }
 }
 
+// y = K x (or A x) on local supports, visiting only the support entries:
+// one thread per node row (its support columns, for a periodic row also the
+// partner's columns that are not in its own support, then the monomial
+// columns), and one workgroup per P^T row (a reduction over the N nodes).
+constexpr int kLocThreads = 256;
+
+template <int KIND, bool FAST>
+__global__ __launch_bounds__(kLocThreads) void matvec_local_kernel(ProbDev P, const double* __restrict__ xv,
+                                                                   double* __restrict__ yv) {
+  pin_kind<KIND, FAST>(P.k);
+  const int64_t nbn = (P.N + kLocThreads - 1) / kLocThreads;
+  if ((int64_t)blockIdx.x < nbn) {
+    const int64_t r = (int64_t)blockIdx.x * kLocThreads + threadIdx.x;
+    if (r >= P.N) return;
+    const RowCtx c = row_ctx<true>(P, r);
+    double acc = 0.0;
+    // ids outside [0, N) (a malformed table) are skipped, never dereferenced
+    for (int64_t q = 0; q < P.S; ++q) {
+      const int64_t j = c.s1[q];
+      if ((uint64_t)j < (uint64_t)P.N) acc += elem_rbf<false>(P, c, j, P.x[j], P.y[j]) * xv[j];
+    }
+    if (c.cls == RC_PD || c.cls == RC_PN) {
+      for (int64_t q = 0; q < P.S; ++q) {
+        const int64_t j = c.s2[q];
+        if ((uint64_t)j < (uint64_t)P.N && !in_support(c.s1, P.S, c.lo1, c.hi1, j))
+          acc += elem_rbf<false>(P, c, j, P.x[j], P.y[j]) * xv[j];
+      }
+    }
+    for (int k = 0; k < (int)P.M; ++k) acc += elem_mono(c, k) * xv[P.N + k];
+    yv[r] = acc;
+    return;
+  }
+  // P^T row N + k: sum_j p_k(x_j) x_j over the nodes
+  const int64_t r = P.N + ((int64_t)blockIdx.x - nbn);
+  const RowCtx c = row_ctx<false>(P, r);
+  double acc = 0.0;
+  for (int64_t j = threadIdx.x; j < P.N; j += kLocThreads) acc += elem_rbf<false>(P, c, j, P.x[j], P.y[j]) * xv[j];
+  __shared__ double red[kLocThreads / 64];
+  for (int off = 32; off > 0; off >>= 1) acc += __shfl_down(acc, off, 64);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = acc;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double t = 0.0;
+    for (int w = 0; w < kLocThreads / 64; ++w) t += red[w];
+    yv[r] = t;
+  }
+}
+
 // Partial y^T = x^T K over a chunk of rows: part[chunk][j].
 constexpr int kMtThreads = 256;
 
@@ -538,8 +587,11 @@ extern "C" int rbfx_matvec(const rbfx_problem* p, int matrix, int trans, const d
   if (!trans || symmetric) {
     const int pid = prof_start("matvec", s);
     const dim3 grid((unsigned)ceil_div(P.n, kMvRows));
+    const dim3 grid_loc((unsigned)(ceil_div(P.N, kLocThreads) + P.M));
     RBFX_KIND_SWITCH(P.k, {
-      if (P.local)
+      if (P.local && !trans)
+        hipLaunchKernelGGL((matvec_local_kernel<KIND_, FAST_>), grid_loc, dim3(kLocThreads), 0, s, P, d_x, d_y);
+      else if (P.local)
         hipLaunchKernelGGL((matvec_kernel<KIND_, FAST_, true>), grid, dim3(kMvThreads), 0, s, P, d_x, d_y);
       else
         hipLaunchKernelGGL((matvec_kernel<KIND_, FAST_, false>), grid, dim3(kMvThreads), 0, s, P, d_x, d_y);
