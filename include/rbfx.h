@@ -118,9 +118,11 @@ int rbfx_assemble(const rbfx_problem* p, int matrix, int64_t row0, int64_t nrows
                   void* stream);
 
 /* Matrix-free product y = K x (trans=0) or y = K^T x (trans=1) over all
- * n = N+M rows, recomputing every entry from the node coordinates in FP64
- * (with local supports, y = K x visits only the support entries).  The
- * workspace is needed for trans=1 only.
+ * n = N+M rows, recomputing every entry from the node coordinates in FP64.
+ * With local supports only the support entries are visited: always for
+ * trans=0, and for trans=1 when the workspace holds the reverse support
+ * table (the size rbfx_matvec_workspace returns); a workspace of the dense
+ * size takes the column kernel instead.  No workspace is needed for trans=0.
  * Replaces the products with B / A implied by updes/assembly.py:399,410,430. */
 size_t rbfx_matvec_workspace(const rbfx_problem* p, int trans);
 int rbfx_matvec(const rbfx_problem* p, int matrix, int trans, const double* d_x,

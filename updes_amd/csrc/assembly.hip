@@ -17,6 +17,8 @@
 // binary search in the list, so the dense write stays the bound.
 #include <stdarg.h>
 
+#include <algorithm>
+
 #include "common.h"
 #include "rbf_device.h"
 
@@ -421,6 +423,123 @@ __global__ __launch_bounds__(kLocThreads) void matvec_local_kernel(ProbDev P, co
   }
 }
 
+// y = K^T x (or A^T x) on local supports over the support entries only.
+// The rows that name each column are gathered first into a reverse table
+// (count, scan, fill; each column's rows then sorted, so the sums run in a
+// fixed order), then one thread per column sums over its rows and the P^T
+// rows, and one workgroup per monomial column sums over the node rows.
+// Column set of node-matrix row r: its node's support, for a periodic row
+// also the partner's ids that are not in it.  Ids outside [0, N) are skipped.
+template <typename F>
+__device__ inline void for_row_cols(const ProbDev& P, const RowCtx& c, F f) {
+  for (int64_t q = 0; q < P.S; ++q) {
+    const int64_t j = c.s1[q];
+    if ((uint64_t)j < (uint64_t)P.N) f(j);
+  }
+  if (c.cls == RC_PD || c.cls == RC_PN)
+    for (int64_t q = 0; q < P.S; ++q) {
+      const int64_t j = c.s2[q];
+      if ((uint64_t)j < (uint64_t)P.N && !in_support(c.s1, P.S, c.lo1, c.hi1, j)) f(j);
+    }
+}
+
+__global__ void zero_i32_kernel(int* __restrict__ p, int64_t n) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0;
+}
+
+__global__ __launch_bounds__(kLocThreads) void rev_count_kernel(ProbDev P, int* __restrict__ cnt) {
+  const int64_t r = (int64_t)blockIdx.x * kLocThreads + threadIdx.x;
+  if (r >= P.N) return;
+  const RowCtx c = row_ctx<true>(P, r);
+  for_row_cols(P, c, [&](int64_t j) { atomicAdd(cnt + j, 1); });
+}
+
+// Exclusive scan of cnt[0..N) into off[0..N] by one workgroup of 1024.
+constexpr int kScanThreads = 1024;
+__global__ __launch_bounds__(kScanThreads) void rev_scan_kernel(const int* __restrict__ cnt, int64_t N,
+                                                                int* __restrict__ off) {
+  __shared__ int part[kScanThreads];
+  const int t = threadIdx.x;
+  const int64_t per = (N + kScanThreads - 1) / kScanThreads;
+  const int64_t b = t * per, e = b + per < N ? b + per : N;
+  int sum = 0;
+  for (int64_t i = b; i < e; ++i) sum += cnt[i];
+  part[t] = sum;
+  __syncthreads();
+  for (int d = 1; d < kScanThreads; d <<= 1) {  // inclusive Hillis-Steele scan
+    const int v = t >= d ? part[t - d] : 0;
+    __syncthreads();
+    part[t] += v;
+    __syncthreads();
+  }
+  int run = t > 0 ? part[t - 1] : 0;
+  for (int64_t i = b; i < e; ++i) {
+    off[i] = run;
+    run += cnt[i];
+  }
+  if (t == kScanThreads - 1) off[N] = part[t];
+}
+
+__global__ __launch_bounds__(kLocThreads) void rev_fill_kernel(ProbDev P, const int* __restrict__ off,
+                                                               int* __restrict__ fill, int* __restrict__ rows) {
+  const int64_t r = (int64_t)blockIdx.x * kLocThreads + threadIdx.x;
+  if (r >= P.N) return;
+  const RowCtx c = row_ctx<true>(P, r);
+  for_row_cols(P, c, [&](int64_t j) { rows[off[j] + atomicAdd(fill + j, 1)] = (int)r; });
+}
+
+template <int KIND, bool FAST>
+__global__ __launch_bounds__(kLocThreads) void matvec_local_t_kernel(ProbDev P, const int* __restrict__ off,
+                                                                     int* __restrict__ rows,
+                                                                     const double* __restrict__ xv,
+                                                                     double* __restrict__ yv) {
+  pin_kind<KIND, FAST>(P.k);
+  const int64_t nbn = (P.N + kLocThreads - 1) / kLocThreads;
+  if ((int64_t)blockIdx.x < nbn) {
+    const int64_t j = (int64_t)blockIdx.x * kLocThreads + threadIdx.x;
+    if (j >= P.N) return;
+    const int b = off[j], e = off[j + 1];
+    for (int a = b + 1; a < e; ++a) {  // insertion sort of this column's rows
+      const int v = rows[a];
+      int q = a - 1;
+      while (q >= b && rows[q] > v) {
+        rows[q + 1] = rows[q];
+        --q;
+      }
+      rows[q + 1] = v;
+    }
+    const double xj = P.x[j], yj = P.y[j];
+    double acc = 0.0;
+    for (int a = b; a < e; ++a) {
+      const int64_t r = rows[a];
+      acc += elem_rbf<false>(P, row_ctx<false>(P, r), j, xj, yj) * xv[r];
+    }
+    for (int k = 0; k < (int)P.M; ++k) acc += elem_rbf<false>(P, row_ctx<false>(P, P.N + k), j, xj, yj) * xv[P.N + k];
+    yv[j] = acc;
+    return;
+  }
+  // monomial column N + k: sum over the node rows
+  const int k = (int)((int64_t)blockIdx.x - nbn);
+  double acc = 0.0;
+  for (int64_t r = threadIdx.x; r < P.N; r += kLocThreads) acc += elem_mono(row_ctx<false>(P, r), k) * xv[r];
+  __shared__ double red[kLocThreads / 64];
+  for (int o = 32; o > 0; o >>= 1) acc += __shfl_down(acc, o, 64);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = acc;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double t = 0.0;
+    for (int w = 0; w < kLocThreads / 64; ++w) t += red[w];
+    yv[P.N + k] = t;
+  }
+}
+
+// Workspace of the reverse table: cnt, fill (N each), off (N+1), rows (N*2S).
+static size_t local_t_ws_bytes(int64_t N, int64_t S) {
+  return 2 * ws_round(sizeof(int) * (size_t)N) + ws_round(sizeof(int) * (size_t)(N + 1)) +
+         ws_round(sizeof(int) * (size_t)N * (size_t)(2 * S > 0 ? 2 * S : 1));
+}
+
 // Partial y^T = x^T K over a chunk of rows: part[chunk][j].
 constexpr int kMtThreads = 256;
 
@@ -571,7 +690,10 @@ size_t matvec_t_ws_bytes(int64_t n) { return ws_round(sizeof(double) * (size_t)m
 
 extern "C" size_t rbfx_matvec_workspace(const rbfx_problem* p, int trans) {
   if (!p || !trans) return 0;
-  return matvec_t_ws_bytes(p->N + p->M);
+  const size_t dense = matvec_t_ws_bytes(p->N + p->M);
+  if ((p->flags & RBFX_FLAG_LOCAL_SUPPORT) && p->support_cols >= 0 && p->N > 0)
+    return std::max(dense, local_t_ws_bytes(p->N, p->support_cols));
+  return dense;
 }
 
 extern "C" int rbfx_matvec(const rbfx_problem* p, int matrix, int trans, const double* d_x,
@@ -595,6 +717,31 @@ extern "C" int rbfx_matvec(const rbfx_problem* p, int matrix, int trans, const d
         hipLaunchKernelGGL((matvec_kernel<KIND_, FAST_, true>), grid, dim3(kMvThreads), 0, s, P, d_x, d_y);
       else
         hipLaunchKernelGGL((matvec_kernel<KIND_, FAST_, false>), grid, dim3(kMvThreads), 0, s, P, d_x, d_y);
+    });
+    prof_stop(pid, s, 0.0, 16.0 * P.n);
+    RBFX_CHECK_LAUNCH();
+    return RBFX_OK;
+  }
+  if (P.local && d_work && work_bytes >= local_t_ws_bytes(P.N, P.S)) {
+    // reverse table in the workspace, then the support-only transposed product
+    Workspace ws(d_work, work_bytes);
+    int* cnt = ws.take<int>(P.N);
+    int* fill = ws.take<int>(P.N);
+    int* off = ws.take<int>(P.N + 1);
+    int* rows = ws.take<int>((size_t)P.N * (size_t)(2 * P.S > 0 ? 2 * P.S : 1));
+    RBFX_REQUIRE(cnt && fill && off && rows, "matvec workspace carve failed");
+    const unsigned gz = (unsigned)ceil_div(P.N, 256), gr = (unsigned)ceil_div(P.N, kLocThreads);
+    hipLaunchKernelGGL(zero_i32_kernel, dim3(gz), dim3(256), 0, s, cnt, P.N);
+    hipLaunchKernelGGL(zero_i32_kernel, dim3(gz), dim3(256), 0, s, fill, P.N);
+    hipLaunchKernelGGL(rev_count_kernel, dim3(gr), dim3(kLocThreads), 0, s, P, cnt);
+    hipLaunchKernelGGL(rev_scan_kernel, dim3(1), dim3(kScanThreads), 0, s, cnt, P.N, off);
+    hipLaunchKernelGGL(rev_fill_kernel, dim3(gr), dim3(kLocThreads), 0, s, P, off, fill, rows);
+    RBFX_CHECK_LAUNCH();
+    const int pid = prof_start("matvec_t", s);
+    const dim3 grid_loc((unsigned)(ceil_div(P.N, kLocThreads) + P.M));
+    RBFX_KIND_SWITCH(P.k, {
+      hipLaunchKernelGGL((matvec_local_t_kernel<KIND_, FAST_>), grid_loc, dim3(kLocThreads), 0, s, P, off, rows,
+                         d_x, d_y);
     });
     prof_stop(pid, s, 0.0, 16.0 * P.n);
     RBFX_CHECK_LAUNCH();

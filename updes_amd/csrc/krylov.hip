@@ -263,7 +263,9 @@ extern "C" int rbfx_gmres_ir(int trans, int64_t n, const rbfx_problem* p, const 
   RBFX_REQUIRE(part != nullptr, "gmres_ir workspace carve failed");
   IrOps ops{trans, n, p, d_K64, ld64, d_LU32, ld32, d_perm32, d_dinv32, tsw, tsb, part, gw, s};
   if (!d_K64) {
-    RBFX_REQUIRE(gw >= rbfx_matvec_workspace(p, trans), "matvec workspace too small");
+    // at least the column kernel's workspace (the support-only transposed
+    // product on local supports needs more and is then not used here)
+    RBFX_REQUIRE(gw >= (trans ? matvec_t_ws_bytes(n) : 0), "matvec workspace too small");
   }
   const unsigned g = (unsigned)ceil_div(n, 256);
   int st;
