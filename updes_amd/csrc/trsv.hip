@@ -173,6 +173,48 @@ __global__ __launch_bounds__(kTsThreads) void sweep_kernel(const TF* __restrict_
       if (s_avail == 0xffffffffu) return;
     }
     const int lim = (unsigned)seq < s_avail ? seq : (int)s_avail;
+    // two tiles at a time while neither is the last dependency: all
+    // their loads are issued before any is used (folding one tile at a time
+    // is bound by the load latency once the factor no longer fits in the
+    // Infinity Cache).  Every product is an explicit FMA in the same order as
+    // tile by tile, so the result does not depend on how the tiles group
+    constexpr int FB = 2;
+    for (; d + FB <= lim && d + FB <= seq - 1; d += FB) {
+      T f[FB][16];
+      if constexpr (!TRANS) {
+        T yk[FB];
+#pragma unroll
+        for (int u = 0; u < FB; ++u) {
+          const int64_t oJ = (FWD ? d + u : nb - 1 - (d + u)) * TB;
+          const bool ok = rr < (int)imin64(TB, n - oJ);
+          yk[u] = ok ? ld_coh(y + oJ + rr) : T(0);
+#pragma unroll
+          for (int k = 0; k < 16; ++k)
+            f[u][k] = (ok && q * 16 + k < szI) ? (T)F[(oI + q * 16 + k) * lda + oJ + rr] : T(0);
+        }
+#pragma unroll
+        for (int u = 0; u < FB; ++u)
+#pragma unroll
+          for (int k = 0; k < 16; ++k) ps[k] = __builtin_fma(f[u][k], yk[u], ps[k]);
+      } else {
+        T yk[FB][16];
+#pragma unroll
+        for (int u = 0; u < FB; ++u) {
+          const int64_t oJ = (FWD ? d + u : nb - 1 - (d + u)) * TB;
+          const int szJ = (int)imin64(TB, n - oJ);
+#pragma unroll
+          for (int k = 0; k < 16; ++k) {
+            const bool ok = rr < szI && q * 16 + k < szJ;
+            f[u][k] = ok ? (T)F[(oJ + q * 16 + k) * lda + oI + rr] : T(0);
+            yk[u][k] = ok ? ld_coh(y + oJ + q * 16 + k) : T(0);
+          }
+        }
+#pragma unroll
+        for (int u = 0; u < FB; ++u)
+#pragma unroll
+          for (int k = 0; k < 16; ++k) s = __builtin_fma(f[u][k], yk[u][k], s);
+      }
+    }
     for (; d < lim; ++d) {
       const int64_t J = FWD ? d : nb - 1 - d;
       const int64_t oJ = J * TB;
@@ -183,11 +225,11 @@ __global__ __launch_bounds__(kTsThreads) void sweep_kernel(const TF* __restrict_
         const T yk = ld_coh(y + oJ + rr);
         if (d == seq - 1) {
 #pragma unroll
-          for (int k = 0; k < 16; ++k) ps[k] += pre[k] * yk;
+          for (int k = 0; k < 16; ++k) ps[k] = __builtin_fma(pre[k], yk, ps[k]);
         } else {
 #pragma unroll
           for (int k = 0; k < 16; ++k)
-            if (q * 16 + k < szI) ps[k] += (T)F[(oI + q * 16 + k) * lda + oJ + rr] * yk;
+            if (q * 16 + k < szI) ps[k] = __builtin_fma((T)F[(oI + q * 16 + k) * lda + oJ + rr], yk, ps[k]);
         }
         continue;
       }
@@ -196,13 +238,13 @@ __global__ __launch_bounds__(kTsThreads) void sweep_kernel(const TF* __restrict_
 #pragma unroll
         for (int k = 0; k < 16; ++k) {
           const int kk = q * 16 + k;
-          if (kk < szJ) s += pre[k] * ld_coh(y + oJ + kk);
+          if (kk < szJ) s = __builtin_fma(pre[k], ld_coh(y + oJ + kk), s);
         }
       } else {
 #pragma unroll
         for (int k = 0; k < 16; ++k) {
           const int kk = q * 16 + k;
-          if (kk < szJ) s += (T)F[(oJ + kk) * lda + oI + rr] * ld_coh(y + oJ + kk);
+          if (kk < szJ) s = __builtin_fma((T)F[(oJ + kk) * lda + oI + rr], ld_coh(y + oJ + kk), s);
         }
       }
     }
