@@ -722,7 +722,8 @@ extern "C" int rbfx_matvec(const rbfx_problem* p, int matrix, int trans, const d
     RBFX_CHECK_LAUNCH();
     return RBFX_OK;
   }
-  if (P.local && d_work && work_bytes >= local_t_ws_bytes(P.N, P.S)) {
+  // (the reverse table indexes rows and entries with 32-bit ints)
+  if (P.local && d_work && work_bytes >= local_t_ws_bytes(P.N, P.S) && P.N * (2 * P.S + 1) < INT32_MAX) {
     // reverse table in the workspace, then the support-only transposed product
     Workspace ws(d_work, work_bytes);
     int* cnt = ws.take<int>(P.N);
