@@ -25,7 +25,7 @@
 extern "C" {
 #endif
 
-#define RBFX_ABI_VERSION 2
+#define RBFX_ABI_VERSION 3
 
 enum rbfx_status {
   RBFX_OK = 0,
@@ -106,6 +106,11 @@ int rbfx_abi_version(void);
 const char* rbfx_last_error(void);
 /* Queries the current device (CU count, LDS size).  Called implicitly. */
 int rbfx_init(void);
+/* Releases the streams and events the library created (the LU's look-ahead
+ * stream and the timing events), after waiting for their work.  Call it
+ * before the HIP runtime shuts down (the Python binding does so at exit);
+ * the library re-creates them on its next use.  No reference counterpart. */
+int rbfx_shutdown(void);
 
 /* ---- K1 assembly ---------------------------------------------------------
  * Writes rows [row0, row0+nrows) of K or A (n = N+M columns) in FP64 and/or
@@ -185,6 +190,11 @@ int rbfx_getrs_f64(int trans, int64_t n, const double* d_LU, int64_t lda,
 int rbfx_getrs_f32(int trans, int64_t n, const float* d_LU, int64_t lda,
                    const int64_t* d_perm, const double* d_dinv, double* d_b,
                    void* d_work, size_t work_bytes, void* stream);
+/* Status of the last rbfx_getrs_* queued with this workspace: synchronises
+ * the stream and returns RBFX_ERR_TIMEOUT if one of its device-side waits
+ * expired (each gives up after about 4 s of real time; the solve then wrote
+ * NaN into d_b instead of a partial result), else RBFX_OK.               */
+int rbfx_getrs_status(int64_t n, void* d_work, size_t work_bytes, void* stream);
 
 /* ---- K5 FP64 GEMV ---------------------------------------------------------
  * y = alpha op(A) x + beta y, op = A (trans=0) or A^T (trans=1); A m x n.   */

@@ -30,7 +30,7 @@ def test_library_exports_every_declared_symbol():
     lib = L.load()
     for name in _declared():
         assert hasattr(lib, name), name
-    assert lib.rbfx_abi_version() == 2
+    assert lib.rbfx_abi_version() == 3
     out = subprocess.run(["nm", "-D", "--defined-only", L.LIB_PATH], capture_output=True, text=True).stdout
     exported = set(re.findall(r" T (rbfx_\w+)", out))
     assert set(_declared()) <= exported

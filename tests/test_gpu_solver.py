@@ -239,3 +239,26 @@ def test_boundary_conditions_end_to_end(fs):
     tol = max(1e-8, 10 * np.linalg.cond(B) * np.finfo(float).eps)
     got = sol.vals.cpu().numpy()
     assert np.linalg.norm(got - u_ref) <= tol * np.linalg.norm(u_ref), (fs, np.linalg.norm(got - u_ref) / np.linalg.norm(u_ref))
+
+
+def test_solves_on_successive_clouds_without_clearing_the_cache():
+    """Factorisations are cached per cloud object; clouds created and dropped
+    in a loop (CPython reuses their ids) must never pick up another cloud's
+    factors.  Each solve is checked against the oracle."""
+    import gc
+    import updes_amd as u
+    ft = C.README_FACETS
+    for nx, ny in [(10, 8), (14, 9), (10, 8), (12, 11), (14, 9)]:
+        cloud = u.SquareCloud(Nx=nx, Ny=ny, facet_types=ft)
+        bcs = {"South": 0.0, "West": 0.0, "North": lambda p: torch.sin(torch.pi * p[0]), "East": 0.0}
+        sol = u.pde_solver(_laplace_op, lambda x, centers, rbf, fields: 1.0, cloud, bcs, u.polyharmonic, 1)
+        oc = OC.SquareCloud(nx, ny, ft)
+        arr = {f: np.sin(np.pi * oc.sorted_nodes[oc.facet_nodes[f], 0]) if f == "North" else
+               np.zeros(len(oc.facet_nodes[f])) for f in ft}
+        q = OA.assemble_q(oc, 1.0, arr)
+        u_ref, _, _ = OS.pde_solver_kpath(C.LAPLACIAN, oc, R.POLYHARMONIC, 1.0, 3, q)
+        got = sol.vals.cpu().numpy()
+        assert got.shape == u_ref.shape
+        assert np.linalg.norm(got - u_ref) <= 1e-9 * np.linalg.norm(u_ref), (nx, ny)
+        del cloud, sol
+        gc.collect()

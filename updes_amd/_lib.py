@@ -4,6 +4,7 @@ Loading never falls back to anything: if the library is missing the import of
 the compute functions raises, and every compute call checks the returned
 status and raises ``RbfxError`` with the library's message.
 """
+import atexit
 import ctypes
 import os
 
@@ -13,6 +14,7 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "_lib", "librbfx.so")
 
 RBFX_OK = 0
+RBFX_ERR_TIMEOUT = -4
 RBFX_ERR_STALL = -5
 
 POLYHARMONIC, GAUSSIAN, THIN_PLATE, MULTIQUADRIC, INV_MULTIQUADRIC = 0, 1, 2, 3, 4
@@ -78,6 +80,8 @@ _SIGS = {
     "rbfx_getrs_workspace": (_sz, [_i64]),
     "rbfx_getrs_f64": (ctypes.c_int, [ctypes.c_int, _i64, _p, _i64, _p, _p, _p, _p, _sz, _p]),
     "rbfx_getrs_f32": (ctypes.c_int, [ctypes.c_int, _i64, _p, _i64, _p, _p, _p, _p, _sz, _p]),
+    "rbfx_getrs_status": (ctypes.c_int, [_i64, _p, _sz, _p]),
+    "rbfx_shutdown": (ctypes.c_int, []),
     "rbfx_gemv_workspace": (_sz, [ctypes.c_int, _i64, _i64]),
     "rbfx_gemv_f64": (ctypes.c_int, [ctypes.c_int, _i64, _i64, _d, _p, _i64, _p, _d, _p, _p, _sz, _p]),
     "rbfx_gmres_ir_workspace": (_sz, [_i64, _i32]),
@@ -108,7 +112,15 @@ def load():
             fn.restype = res
             fn.argtypes = args
         _lib = lib
+        atexit.register(_shutdown)
     return _lib
+
+
+def _shutdown():
+    # release the library's own streams and events while the HIP runtime is
+    # still up (only if this process ever touched the GPU)
+    if _lib is not None and torch.cuda.is_initialized():
+        _lib.rbfx_shutdown()
 
 
 def lib():

@@ -32,6 +32,10 @@ const DeviceInfo& device_info() {
       d.num_cus = 256;
       d.lds_per_block = 65536;
     }
+    int khz = 0;
+    if (hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev) != hipSuccess || khz <= 0)
+      khz = 100000;  // CDNA real-time counter: 100 MHz
+    d.wait_ticks = (uint64_t)(kWaitSeconds * 1e3 * khz);
     d.device = dev;
   }
   return d;
@@ -142,4 +146,19 @@ extern "C" int rbfx_profile_end(rbfx_prof_entry* out, int max_entries, int* n_en
   for (int i = 0; i < n && i < max_entries; ++i) out[i] = agg[i];
   if (n_entries) *n_entries = n;
   return RBFX_OK;
+}
+
+extern "C" int rbfx_shutdown(void) {
+  {
+    std::lock_guard<std::mutex> lk(rbfx::g_prof_mu);
+    for (auto& r : rbfx::g_prof) {
+      rbfx::g_pool.push_back(r.a);
+      rbfx::g_pool.push_back(r.b);
+    }
+    rbfx::g_prof.clear();
+    rbfx::g_prof_on = false;
+    for (hipEvent_t e : rbfx::g_pool) (void)hipEventDestroy(e);
+    rbfx::g_pool.clear();
+  }
+  return rbfx::release_panel_streams();
 }

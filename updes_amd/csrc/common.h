@@ -45,6 +45,10 @@ struct DeviceInfo {
   int device = -1;
   int num_cus = 0;
   size_t lds_per_block = 0;
+  // Ticks of the device's constant-rate real-time counter (wall_clock64,
+  // hipDeviceAttributeWallClockRate) in kWaitSeconds: the time bound of
+  // every device-side wait.
+  uint64_t wait_ticks = 0;
 };
 // Cached per current device.
 const DeviceInfo& device_info();
@@ -70,9 +74,24 @@ struct Workspace {
 };
 inline size_t ws_round(size_t bytes) { return (bytes + 255) & ~size_t(255); }
 
-// Spin limit of every bounded device-side wait (each iteration is one
-// device-scope load plus a short sleep, ~1 us): about 4 s in total.
-constexpr uint32_t kSpinLimit = 1u << 22;
+// Every device-side wait gives up after kWaitSeconds of real time, read from
+// the constant-rate counter (wall_clock64), whatever each poll costs; it
+// then sets a status word that the host turns into RBFX_ERR_TIMEOUT.
+constexpr double kWaitSeconds = 4.0;
+
+// Real-time deadline of one wait.  The counter is read once per
+// kWaitCheckEvery polls (a read is a scalar memory op of ~100 cycles).
+constexpr uint32_t kWaitCheckEvery = 16;
+struct WaitClock {
+  uint64_t t0, ticks;
+  uint32_t polls = 0;
+  __device__ explicit WaitClock(uint64_t budget) : t0(wall_clock64()), ticks(budget) {}
+  // true once the budget is spent
+  __device__ bool expired() {
+    if (++polls % kWaitCheckEvery) return false;
+    return wall_clock64() - t0 > ticks;
+  }
+};
 
 }  // namespace rbfx
 
@@ -113,4 +132,6 @@ int prof_start(const char* name, hipStream_t s);
 void prof_stop(int id, hipStream_t s, double flops, double bytes);
 // Workspace bytes of the transposed matrix-free product over n rows.
 size_t matvec_t_ws_bytes(int64_t n);
+// Destroys the LU's per-device panel streams and events.
+int release_panel_streams();
 }  // namespace rbfx

@@ -449,6 +449,7 @@ struct PanelArgs {
   unsigned seq_base;  // step count before this launch
   int* status;
   int32_t* info;
+  uint64_t wait_ticks;  // real-time bound of each wait (wall_clock64 ticks)
 };
 
 constexpr int kPanelThreads = 256;
@@ -653,6 +654,7 @@ __global__ __launch_bounds__(kPanelThreads) void panel_kernel(PanelArgs<T> a) {
       const unsigned long long* o = recs + kRecWords * ((int64_t)par * G + q);
       unsigned long long w0 = ld_coh(o), w1 = ld_coh(o + 1);
       uint32_t spins = 0;
+      WaitClock clk(a.wait_ticks);
       while ((w0 & 0xFFFFull) != tag || (w1 >> 48) != tag) {
         __builtin_amdgcn_s_sleep(1);
         ++spins;
@@ -661,7 +663,7 @@ __global__ __launch_bounds__(kPanelThreads) void panel_kernel(PanelArgs<T> a) {
           s_bad = 1;
           break;
         }
-        if (spins > kSpinLimit) {
+        if (clk.expired()) {
           __hip_atomic_store(a.status, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
           s_bad = 1;
           break;
@@ -1087,6 +1089,7 @@ static int leaf(LuCtx<T>& c, int64_t r0, int w) {
   a.seq_base = c.seq;
   a.status = c.status;
   a.info = c.info;
+  a.wait_ticks = di.wait_ticks;
   RBFX_REQUIRE(G <= c.G_max && G < kKeyMaxGroups && R <= (int64_t)kPanelThreads * rows && R < kKeyMaxRows,
                "too many panel workgroups or rows");
   const int pid = prof_start(sizeof(T) == 8 ? "panel_f64" : "panel_f32", c.s);
@@ -1272,6 +1275,27 @@ struct PanelStream {
 };
 static std::mutex g_ps_mu;
 static PanelStream g_ps[64];
+
+// Releases the panel streams and events of every device (rbfx_shutdown).
+int release_panel_streams() {
+  std::lock_guard<std::mutex> lock(g_ps_mu);
+  int dev0 = 0;
+  RBFX_HIP(hipGetDevice(&dev0));
+  for (int d = 0; d < 64; ++d) {
+    PanelStream& p = g_ps[d];
+    if (!p.s) continue;
+    RBFX_HIP(hipSetDevice(d));
+    RBFX_HIP(hipStreamSynchronize(p.s));
+    for (int i = 0; i < 2; ++i) {
+      if (p.ev[i]) RBFX_HIP(hipEventDestroy(p.ev[i]));
+      p.ev[i] = nullptr;
+    }
+    RBFX_HIP(hipStreamDestroy(p.s));
+    p.s = nullptr;
+  }
+  RBFX_HIP(hipSetDevice(dev0));
+  return RBFX_OK;
+}
 
 static int panel_stream(PanelStream*& out) {
   int dev = 0;

@@ -10,6 +10,7 @@
 // publishes its slice.  Row-major storage makes the no-transpose tiles
 // row-contiguous and the transposed tiles column-coalesced.
 #include <algorithm>
+#include <climits>
 #include <vector>
 
 #include "common.h"
@@ -61,21 +62,29 @@ __global__ __launch_bounds__(TB) void diag_inv_kernel(const TF* __restrict__ LU,
 }
 
 // ---- permutation staging ------------------------------------------------------
+// ctl[0] = ticket, ctl[1] = completion counter of the current sweep,
+// ctl[2] = status of the whole solve (nonzero: a wait expired).  The solve's
+// first gather clears all three; the reset before each sweep only the first
+// two, so an expired wait in the first sweep survives the second.
 __global__ void gather_perm_kernel(const double* __restrict__ b, const int64_t* __restrict__ perm,
-                                   int64_t n, double* __restrict__ y, unsigned* __restrict__ ctl) {
+                                   int64_t n, double* __restrict__ y, unsigned* __restrict__ ctl,
+                                   int clear_status) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i == 0) {
     ctl[0] = 0u;
     ctl[1] = 0u;
-    ctl[2] = 0u;
+    if (clear_status) ctl[2] = 0u;
   }
   if (i < n) y[i] = b[perm ? perm[i] : i];
 }
 
+// Writes the solution back; a solve whose status is set writes NaN instead,
+// so that a timed-out solve never reads as a result.
 __global__ void scatter_perm_kernel(const double* __restrict__ y, const int64_t* __restrict__ perm,
-                                    int64_t n, double* __restrict__ b) {
+                                    int64_t n, double* __restrict__ b, const unsigned* __restrict__ ctl) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) b[perm ? perm[i] : i] = y[i];
+  const bool failed = ctl[2] != 0u;
+  if (i < n) b[perm ? perm[i] : i] = failed ? __builtin_nan("") : y[i];
 }
 
 // ---- the sweep ---------------------------------------------------------------
@@ -88,7 +97,8 @@ __global__ void scatter_perm_kernel(const double* __restrict__ y, const int64_t*
 template <typename TF, bool FWD, bool TRANS, int WHICH>
 __global__ __launch_bounds__(kTsThreads) void sweep_kernel(const TF* __restrict__ F, int64_t lda, int64_t n,
                                                            const double* __restrict__ dinv,
-                                                           double* __restrict__ y, unsigned* __restrict__ ctl) {
+                                                           double* __restrict__ y, unsigned* __restrict__ ctl,
+                                                           uint64_t wait_ticks) {
   typedef double T;
   __shared__ T acc[TB];
   __shared__ T part[4][TB];
@@ -105,6 +115,8 @@ __global__ __launch_bounds__(kTsThreads) void sweep_kernel(const TF* __restrict_
     const unsigned t = __hip_atomic_fetch_add(ticket, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     s_I = (int)t;
     s_avail = 0;
+    // an earlier sweep of this solve gave up: nothing to compute
+    if (__hip_atomic_load(status, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) s_I = INT_MAX;
   }
   __syncthreads();
   const int seq = s_I;
@@ -154,9 +166,11 @@ __global__ __launch_bounds__(kTsThreads) void sweep_kernel(const TF* __restrict_
       if (tid == 0) {
         unsigned v;
         uint32_t spins = 0;
+        WaitClock clk(wait_ticks);
         while ((v = ld_coh(done)) <= (unsigned)d) {
           __builtin_amdgcn_s_sleep(1);
-          if (++spins > kSpinLimit) {
+          ++spins;
+          if (clk.expired()) {
             __hip_atomic_store(status, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             v = 0xffffffffu;
             break;
@@ -292,14 +306,15 @@ template <typename T>
 static int launch_sweep(bool fwd, bool trans, int which, const T* F, int64_t lda, int64_t n,
                         const double* dinv, double* y, unsigned* ctl, hipStream_t s) {
   const unsigned nb = (unsigned)ceil_div(n, TB);
-  // reset ticket/done before each sweep
+  // reset ticket/done before each sweep (the status stays)
   hipLaunchKernelGGL(gather_perm_kernel, dim3(1), dim3(1), 0, s, (const double*)nullptr,
-                     (const int64_t*)nullptr, (int64_t)0, (double*)nullptr, ctl);
+                     (const int64_t*)nullptr, (int64_t)0, (double*)nullptr, ctl, 0);
   RBFX_CHECK_LAUNCH();
+  const uint64_t ticks = device_info().wait_ticks;
   const int pid = prof_start(sizeof(T) == 8 ? "sweep_f64" : "sweep_f32", s);
 #define RBFX_SWEEP(F_, T_, W_)                                                               \
   hipLaunchKernelGGL((sweep_kernel<T, F_, T_, W_>), dim3(nb), dim3(kTsThreads), 0, s, F, lda, n, \
-                     dinv, y, ctl)
+                     dinv, y, ctl, ticks)
   if (fwd && !trans && which == 0) RBFX_SWEEP(true, false, 0);
   else if (!fwd && !trans && which == 1) RBFX_SWEEP(false, false, 1);
   else if (fwd && trans && which == 1) RBFX_SWEEP(true, true, 1);
@@ -314,31 +329,55 @@ static int launch_sweep(bool fwd, bool trans, int which, const T* F, int64_t lda
   return RBFX_OK;
 }
 
+// Workspace of one solve: the permuted vector, then the control words.
+static bool carve_getrs(void* work, size_t work_bytes, int64_t n, double*& y, unsigned*& ctl) {
+  Workspace ws(work, work_bytes);
+  y = ws.take<double>(n);
+  ctl = ws.take<unsigned>(64);
+  return y && ctl;
+}
+
 template <typename T>
 static int getrs(int trans, int64_t n, const T* LU, int64_t lda, const int64_t* perm, const double* dinv,
                  double* b, void* work, size_t work_bytes, hipStream_t s) {
   RBFX_REQUIRE(n > 0 && lda >= n && LU && perm && dinv && b, "bad getrs args");
-  Workspace ws(work, work_bytes);
-  double* y = ws.take<double>(n);
-  unsigned* ctl = ws.take<unsigned>(64);
-  RBFX_REQUIRE(y && ctl, "getrs workspace too small");
+  double* y;
+  unsigned* ctl;
+  RBFX_REQUIRE(carve_getrs(work, work_bytes, n, y, ctl), "getrs workspace too small");
   const unsigned g = (unsigned)ceil_div(n, 256);
   int st;
   if (!trans) {
-    hipLaunchKernelGGL(gather_perm_kernel, dim3(g), dim3(256), 0, s, b, perm, n, y, ctl);
+    hipLaunchKernelGGL(gather_perm_kernel, dim3(g), dim3(256), 0, s, b, perm, n, y, ctl, 1);
     RBFX_CHECK_LAUNCH();
     if ((st = launch_sweep<T>(true, false, 0, LU, lda, n, dinv, y, ctl, s))) return st;
     if ((st = launch_sweep<T>(false, false, 1, LU, lda, n, dinv, y, ctl, s))) return st;
-    hipLaunchKernelGGL(scatter_perm_kernel, dim3(g), dim3(256), 0, s, y, (const int64_t*)nullptr, n, b);
+    hipLaunchKernelGGL(scatter_perm_kernel, dim3(g), dim3(256), 0, s, y, (const int64_t*)nullptr, n, b,
+                       ctl);
     RBFX_CHECK_LAUNCH();
   } else {
     hipLaunchKernelGGL(gather_perm_kernel, dim3(g), dim3(256), 0, s, b, (const int64_t*)nullptr, n, y,
-                       ctl);
+                       ctl, 1);
     RBFX_CHECK_LAUNCH();
     if ((st = launch_sweep<T>(true, true, 1, LU, lda, n, dinv, y, ctl, s))) return st;
     if ((st = launch_sweep<T>(false, true, 0, LU, lda, n, dinv, y, ctl, s))) return st;
-    hipLaunchKernelGGL(scatter_perm_kernel, dim3(g), dim3(256), 0, s, y, perm, n, b);
+    hipLaunchKernelGGL(scatter_perm_kernel, dim3(g), dim3(256), 0, s, y, perm, n, b, ctl);
     RBFX_CHECK_LAUNCH();
+  }
+  return RBFX_OK;
+}
+
+// Status of the last solve queued with this workspace (synchronises).
+static int getrs_status(int64_t n, void* work, size_t work_bytes, hipStream_t s) {
+  RBFX_REQUIRE(n > 0 && work, "bad getrs_status args");
+  double* y;
+  unsigned* ctl;
+  RBFX_REQUIRE(carve_getrs(work, work_bytes, n, y, ctl), "getrs workspace too small");
+  unsigned st = 0;
+  RBFX_HIP(hipMemcpyAsync(&st, ctl + 2, sizeof(st), hipMemcpyDeviceToHost, s));
+  RBFX_HIP(hipStreamSynchronize(s));
+  if (st != 0u) {
+    set_error("triangular solve: a device-side wait expired after %.0f s (solution set to NaN)", kWaitSeconds);
+    return RBFX_ERR_TIMEOUT;
   }
   return RBFX_OK;
 }
@@ -393,6 +432,10 @@ extern "C" int rbfx_getrs_f64(int trans, int64_t n, const double* d_LU, int64_t 
                               const double* d_dinv, double* d_b, void* d_work, size_t work_bytes,
                               void* stream) {
   return getrs<double>(trans, n, d_LU, lda, d_perm, d_dinv, d_b, d_work, work_bytes, as_stream(stream));
+}
+
+extern "C" int rbfx_getrs_status(int64_t n, void* d_work, size_t work_bytes, void* stream) {
+  return getrs_status(n, d_work, work_bytes, as_stream(stream));
 }
 
 extern "C" int rbfx_getrs_f32(int trans, int64_t n, const float* d_LU, int64_t lda, const int64_t* d_perm,

@@ -7,6 +7,7 @@ fallback (``_lib.require_gpu`` raises when no device is present).
 import collections
 import ctypes
 import hashlib
+import itertools
 
 import numpy as np
 import torch
@@ -83,6 +84,22 @@ def support_table(cloud):
     if sls.shape != (cloud.N, int(k) - 1) or sls.min(initial=0) < 0 or sls.max(initial=0) >= cloud.N:
         raise ValueError("sorted_local_supports must be (N, support_size-1) node ids")
     return np.sort(sls, axis=1).astype(np.int32)
+
+
+_TOKENS = itertools.count(1)
+
+
+def cloud_key(cloud):
+    """Cache key of a cloud: a token owned by the cloud object (never reused,
+    unlike ``id()``) plus its row-class sizes and node data."""
+    tok = cloud.__dict__.get("_rbfx_token")
+    if tok is None:
+        tok = next(_TOKENS)
+        cloud.__dict__["_rbfx_token"] = tok
+    nodes = cloud.sorted_nodes
+    data = nodes.data_ptr() if isinstance(nodes, torch.Tensor) else id(nodes)
+    return (tok, int(cloud.N), int(cloud.Ni), int(cloud.Nd), int(cloud.Nn), int(cloud.Nr), tuple(cloud.Np),
+            data, str(getattr(cloud, "support_size", "max")))
 
 
 def device_cloud(cloud):
@@ -253,11 +270,19 @@ class LU:
         self._ws = L.workspace(lib.rbfx_getrs_workspace(n), dev)
 
     def solve(self, b, trans=False):
-        """x with A x = b (or A^T x = b), b an (n,) FP64 tensor (not modified)."""
+        """x with A x = b (or A^T x = b), b an (n,) FP64 tensor (not modified).
+
+        Synchronises: a device-side wait of the triangular sweeps that expired
+        raises RbfxError (RBFX_ERR_TIMEOUT) instead of returning NaN."""
+        if b.numel() != self.n:
+            raise ValueError("right-hand side has %d entries, the factorisation is %d x %d"
+                             % (b.numel(), self.n, self.n))
         x = b.detach().to(torch.float64).contiguous().clone()
-        fn = L.lib().rbfx_getrs_f64 if self.dtype == torch.float64 else L.lib().rbfx_getrs_f32
+        lib = L.lib()
+        fn = lib.rbfx_getrs_f64 if self.dtype == torch.float64 else lib.rbfx_getrs_f32
         L.check(fn(int(trans), self.n, L.ptr(self.a), self.ld, L.ptr(self.perm), L.ptr(self.dinv), L.ptr(x),
                    L.ptr(self._ws), self._ws.numel(), L.stream()))
+        L.check(lib.rbfx_getrs_status(self.n, L.ptr(self._ws), self._ws.numel(), L.stream()))
         return x
 
 
@@ -377,7 +402,7 @@ def clear_cache():
 
 def a_factor(cloud, kind, param, M):
     """FP64 LU of the interpolation matrix A (updes/assembly.py:62-90), cached."""
-    key = (id(cloud), kind, param, M)
+    key = (cloud_key(cloud), kind, param, M)
     f = _A_CACHE.get(key) if CACHE_ENABLED[0] else None
     if f is None:
         prob = Problem(device_cloud(cloud), kind, param, M)
@@ -391,7 +416,7 @@ def a_factor(cloud, kind, param, M):
 def k_factor(cloud, kind, param, M, desc, robin_beta, precision, **ir):
     rkey = None if robin_beta is None else hashlib.sha1(
         torch.as_tensor(robin_beta, dtype=torch.float64).cpu().numpy().tobytes()).hexdigest()
-    key = (id(cloud), kind, param, M, desc.key(), rkey, precision, tuple(sorted(ir.items())))
+    key = (cloud_key(cloud), kind, param, M, desc.key(), rkey, precision, tuple(sorted(ir.items())))
     f = _K_CACHE.get(key) if CACHE_ENABLED[0] else None
     if f is None:
         prob = Problem(device_cloud(cloud), kind, param, M, desc, robin_beta)
