@@ -77,7 +77,7 @@ def lap_op(x, center=None, rbf=None, monomial=None, fields=None):
     return u.nodal_laplacian(x, center, rbf, monomial)
 
 
-def solve_and_grad(cloud, bcs, b, basis, precision, want_grad=True):
+def solve_and_grad(cloud, bcs, b, basis, precision, want_grad=True, **ir_opts):
     import updes_amd as u
     bb = b.detach().clone().requires_grad_(want_grad)
     f_int = basis @ bb
@@ -85,7 +85,7 @@ def solve_and_grad(cloud, bcs, b, basis, precision, want_grad=True):
     def rhs(x, centers, rbf, fields):
         return -f_int
 
-    sol = u.pde_solver(lap_op, rhs, cloud, bcs, u.polyharmonic, 2, precision=precision)
+    sol = u.pde_solver(lap_op, rhs, cloud, bcs, u.polyharmonic, 2, precision=precision, **ir_opts)
     vals = sol.vals
     loss = 0.5 * (vals[:cloud.Ni] ** 2).mean()
     g = None

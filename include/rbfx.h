@@ -190,6 +190,15 @@ int rbfx_getrs_f64(int trans, int64_t n, const double* d_LU, int64_t lda,
 int rbfx_getrs_f32(int trans, int64_t n, const float* d_LU, int64_t lda,
                    const int64_t* d_perm, const double* d_dinv, double* d_b,
                    void* d_work, size_t work_bytes, void* stream);
+/* A X = B for nrhs right-hand sides at once (B: n x nrhs row-major, ldb >=
+ * nrhs, overwritten by X) with FP64 LU factors and their solve data: blocked
+ * substitution whose updates run as MFMA GEMMs.  Replaces the dense
+ * jnp.linalg.inv of updes/assembly.py:87-90 (B = identity) that assemble_B
+ * multiplies by D (updes/assembly.py:399).                                  */
+size_t rbfx_getrs_multi_workspace(int64_t n, int64_t nrhs);
+int rbfx_getrs_multi_f64(int64_t n, const double* d_LU, int64_t lda, const int64_t* d_perm,
+                         const double* d_dinv, int64_t nrhs, double* d_B, int64_t ldb, void* d_work,
+                         size_t work_bytes, void* stream);
 /* Status of the last rbfx_getrs_* queued with this workspace: synchronises
  * the stream and returns RBFX_ERR_TIMEOUT if one of its device-side waits
  * expired (each gives up after about 4 s of real time; the solve then wrote
@@ -225,6 +234,65 @@ int rbfx_gmres_ir(int trans, int64_t n, const rbfx_problem* p, const double* d_K
                   const double* d_b, double* d_x, double tol, int32_t restart,
                   int32_t max_inner, void* d_work, size_t work_bytes,
                   rbfx_ir_stats* stats, void* stream);
+
+/* ---- FP32 preconditioner in a local cardinal basis --------------------------
+ * The FP32-MFMA leg of the mixed-precision solve.  No reference counterpart
+ * (the reference solves in FP64 only, updes/operators.py:612-613).
+ * Column j of the basis C is node j's local cardinal function: the RBF
+ * interpolant on the k nearest nodes S_j (j included) with the problem's M
+ * monomials, 1 at node j and 0 at the rest of S_j, with RBF coefficients
+ * beta_j (P^T beta_j = 0 to rounding) and monomial coefficients alpha_j.
+ * K~ = diag(rs) D C (D: the first N rows of K) is N x N and well conditioned
+ * where K is not; its FP32 LU preconditions GMRES on K itself:
+ *   z = C K~^-1 diag(rs) r_D + [P (P^T P)^-1 r_P; 0]  (and its transpose).
+ * Requires every node in every support (no RBFX_FLAG_LOCAL_SUPPORT).      */
+typedef struct rbfx_cardinal {
+  int64_t N, M, k;           /* nodes, monomials (<= 15), stencil size (k <= 48, k + M <= 63) */
+  const double* d_x;         /* (N) node coordinates, sorted order                          */
+  const double* d_y;
+  const int32_t* d_stencil;  /* (N*k) stencil node ids per node, node j included            */
+  double* d_beta;            /* (N*k) out of rbfx_cardinal_coeffs                            */
+  double* d_alpha;           /* (N*M) out of rbfx_cardinal_coeffs                            */
+  double* d_rs;              /* (N) row scales, out of rbfx_cardinal_form                   */
+  const int64_t* d_rev_ptr;  /* (N+1) occurrences of node l in the stencils: entries         */
+  const int32_t* d_rev_idx;  /*   d_rev_idx[d_rev_ptr[l] .. d_rev_ptr[l+1]) = j*k + t with
+                                  d_stencil[j*k+t] == l, j ascending (rbfx_gmres_ir_cardinal) */
+  const double* d_ptp_inv;   /* (M*M) inverse of P^T P, row-major (rbfx_gmres_ir_cardinal)   */
+} rbfx_cardinal;
+
+/* Local cardinal coefficients (beta, alpha) of every node.  Synchronises;
+ * RBFX_ERR_ARG for a stencil id outside [0, N) or a node missing from its own
+ * stencil, RBFX_ERR_SINGULAR for a stencil not unisolvent for the monomials.
+ * Workspace: 256 bytes.                                                    */
+int rbfx_cardinal_coeffs(const rbfx_problem* p, const rbfx_cardinal* c, void* d_work, size_t work_bytes,
+                         void* stream);
+/* rs and K~ = diag(rs) D C in FP32 (N x N, leading dimension ld32) from the
+ * stored FP64 K (n x n, ld64).  Synchronises.  Workspace: 256 bytes.       */
+int rbfx_cardinal_form(const rbfx_cardinal* c, const double* d_K64, int64_t ld64, float* d_out32,
+                       int64_t ld32, void* d_work, size_t work_bytes, void* stream);
+/* Solves op(K) x = b (trans 0/1, n = N + M) to FP64 accuracy: flexible GMRES
+ * with FP64 residuals from the stored K, preconditioned by the FP32 LU of K~
+ * (rbfx_getrf_f32 + rbfx_getrs_prepare_f32 of the N x N matrix).  The
+ * forward system is solved row-scaled, diag(s) K x = diag(s) b with
+ * s = [rs; 1] (the scales of K~), so that GMRES sees the preconditioned
+ * operator diag(rs) D C K~^-1 without a diagonal similarity; the transposed
+ * one unscaled.  A cycle stops as converged at a relative residual <= tol,
+ * or at a normwise backward error eta = ||r|| / (k_fro ||x|| + ||b||) <=
+ * 2^-54, or once the residual no longer improves at eta <= 1e-14; k_fro is
+ * the Frobenius norm of the system's matrix (rbfx_frobenius_f64 of K with
+ * the row scales rs for trans = 0, without for trans = 1; 0 disables the
+ * eta tests).  RBFX_ERR_STALL otherwise (x = best iterate).  Synchronises. */
+size_t rbfx_gmres_ir_cardinal_workspace(int64_t N, int64_t M, int32_t restart);
+int rbfx_gmres_ir_cardinal(int trans, const rbfx_cardinal* c, const double* d_K64, int64_t ld64,
+                           double k_fro, const float* d_LU32, int64_t ld32, const int64_t* d_perm32,
+                           const double* d_dinv32, const double* d_b, double* d_x, double tol,
+                           int32_t restart, int32_t max_inner, void* d_work, size_t work_bytes,
+                           rbfx_ir_stats* stats, void* stream);
+/* Frobenius norm of an m x n FP64 matrix whose rows i < n_scaled are scaled
+ * by d_row_scale[i] (NULL: no scaling), returned on the host (synchronises). */
+size_t rbfx_frobenius_workspace(int64_t m);
+int rbfx_frobenius_f64(int64_t m, int64_t n, const double* d_A, int64_t lda, const double* d_row_scale,
+                       int64_t n_scaled, double* h_norm, void* d_work, size_t work_bytes, void* stream);
 
 /* ---- per-launch timing ----------------------------------------------------
  * Between rbfx_profile_begin() and rbfx_profile_end() every launch of the

@@ -36,7 +36,13 @@ def test_laplace_control_gradient(precision):
 
     c = c0.clone().requires_grad_(True)
     L0 = loss_of(c)
+    st_fwd = u.last_solve_stats()
     L0.backward()
+    st_adj = u.last_solve_stats()
+    if precision == "fp32_ir":
+        # forward and adjoint both from the FP32 factorisation (no fallback)
+        assert not st_fwd["fell_back"] and not st_adj["fell_back"], (st_fwd, st_adj)
+        assert st_adj["ir_inner"] > 0 and st_adj["basis"] == "cardinal", st_adj
     g = c.grad.detach().cpu().numpy()
     # closed form from the oracle: dL/dq = B^-T dL/du
     oc = OC.SquareCloud(nx, nx, ft)
@@ -85,6 +91,8 @@ def test_source_coefficient_gradient_matches_fd():
         fd = (L(b + e) - L(b - e)) / (2 * h)
         assert abs(fd - g[k]) <= 1e-6 * max(1e-3, np.abs(g).max()), (k, fd, g[k])
     _, _, g32 = bench.solve_and_grad(cloud, bcs, b, basis, "fp32_ir")
+    st = u.last_solve_stats()
+    assert st["fell_back"] is False and st["basis"] == "cardinal", st
     assert np.linalg.norm(g32.cpu().numpy() - g) <= 1e-8 * np.linalg.norm(g)
 
 

@@ -1,0 +1,107 @@
+"""GPU parity of the FP32 preconditioner's local cardinal basis
+(updes_amd/csrc/cardinal.hip, through the C ABI): the coefficient kernel and
+the formation of K~ = diag(rs) D C against the NumPy restatement
+(tests/_cardinal_ref.py), and flexible GMRES-IR in that basis, forward and
+transposed, against the FP64 LU of the oracle's K."""
+import ctypes
+
+import numpy as np
+import pytest
+import scipy.linalg as sla
+import torch
+
+from oracle import assembly as OA, cloud as OC, rbf as R
+from tests import _cardinal_ref as CR
+from tests import _cases as C
+
+pytestmark = pytest.mark.gpu
+
+
+def _setup(kind, param, M, nx=12, ny=11, facets=None, seed=2, k=20):
+    import updes_amd as u
+    from updes_amd import engine as E
+    facets = facets or C.FACET_SETS[1]
+    J = C.jitter(nx, ny, 0.2, seed)
+    oc = OC.SquareCloud(nx, ny, facets, jitter=J)
+    uc = u.SquareCloud(Nx=nx, Ny=ny, facet_types=facets, jitter=J)
+    desc = E.OpDesc.builtin(3)  # Laplacian rows
+    prob = E.Problem(E.device_cloud(uc), kind, param, M, desc)
+    basis = E.CardinalBasis(prob, k)
+    return oc, uc, prob, basis
+
+
+@pytest.mark.parametrize("kind,param", C.KINDS)
+@pytest.mark.parametrize("M", [3, 6])
+def test_cardinal_coefficients_match_numpy(kind, param, M):
+    oc, uc, prob, basis = _setup(kind, param, M)
+    st = basis.stencil.cpu().numpy().reshape(oc.N, basis.k)
+    beta_ref, alpha_ref = CR.coefficients(oc.sorted_nodes, st, kind, param, M)
+    beta = basis.beta.cpu().numpy().reshape(oc.N, basis.k)
+    alpha = basis.alpha.cpu().numpy()[:oc.N * M].reshape(oc.N, M)
+    sb = np.abs(beta_ref).max(axis=1, keepdims=True)
+    assert (np.abs(beta - beta_ref) / sb).max() <= 1e-8
+    sa = np.maximum(np.abs(alpha_ref).max(axis=1, keepdims=True), 1e-300)
+    assert (np.abs(alpha - alpha_ref) / sa).max() <= 1e-8
+
+
+@pytest.mark.parametrize("kind,param", [(R.POLYHARMONIC, 1.0), (R.GAUSSIAN, 5.0), (R.THIN_PLATE, 1.0)])
+def test_cardinal_form_matches_numpy(kind, param):
+    from updes_amd import engine as E, _lib as L
+    M = 6
+    oc, uc, prob, basis = _setup(kind, param, M, facets=C.FACET_SETS[2])
+    K64 = E.assemble(prob, L.MAT_K, torch.float64)
+    rs = torch.empty(oc.N, dtype=torch.float64, device="cuda")
+    Kt = basis.form(K64, rs)[:, :oc.N].cpu().numpy()
+    st = basis.stencil.cpu().numpy().reshape(oc.N, basis.k)
+    Cm = CR.basis_matrix(st, basis.beta.cpu().numpy().reshape(oc.N, basis.k),
+                         basis.alpha.cpu().numpy()[:oc.N * M].reshape(oc.N, M))
+    rs_ref, Kt_ref = CR.form(K64[:, :prob.n].cpu().numpy(), Cm, st)
+    np.testing.assert_allclose(rs.cpu().numpy(), rs_ref, rtol=1e-11)
+    # FP32 output of an FP64 sum: within a couple of FP32 ulps of the row scale
+    scale = np.abs(Kt_ref).max(axis=1, keepdims=True)
+    assert (np.abs(Kt - Kt_ref) / scale).max() <= 2.5e-7
+
+
+@pytest.mark.parametrize("trans", [0, 1])
+def test_gmres_ir_cardinal_matches_fp64_lu(trans):
+    """rbfx_gmres_ir_cardinal on a 30 x 30 jittered Poisson system with a
+    Neumann facet (r^3, degree 2): converges without stalling to the FP64
+    LU's solution (forward: the nodal values; transposed: the adjoint of q)."""
+    from updes_amd import engine as E, _lib as L
+    M = 6
+    ft = {"North": "d", "South": "n", "East": "d", "West": "d"}
+    oc, uc, prob, basis = _setup(R.POLYHARMONIC, 1.0, M, nx=30, ny=30, facets=ft, seed=4)
+    fac = E.KFactor(prob, "fp32_ir", basis=basis)
+    assert fac.mode == "cardinal"
+    K = OA.assemble_K(C.LAPLACIAN, oc, R.POLYHARMONIC, 1.0, M)
+    rng = np.random.default_rng(5 + trans)
+    b = np.concatenate([rng.standard_normal(oc.N), np.zeros(M) if not trans else rng.standard_normal(M)])
+    ref = sla.lu_solve(sla.lu_factor(K), b, trans=trans)
+    x = fac.solve(torch.as_tensor(b, device="cuda"), trans=bool(trans)).cpu().numpy()
+    st = fac.last_stats.as_dict()
+    assert not st["fell_back"] and st["basis"] == "cardinal", st
+    assert st["ir_inner"] <= 40, st
+    if not trans:
+        A = OA.assemble_A(oc, R.POLYHARMONIC, 1.0, M)
+        u, u_ref = A[:oc.N] @ x, A[:oc.N] @ ref
+        assert np.linalg.norm(u - u_ref) <= 1e-9 * np.linalg.norm(u_ref), st
+    else:
+        # the adjoint of q (the first N entries) is what gradients use
+        e = np.linalg.norm(x[:oc.N] - ref[:oc.N]) / np.linalg.norm(ref[:oc.N])
+        assert e <= 1e-7, (e, st)
+    # backward error of the FP64 residual
+    r = b - (K.T @ x if trans else K @ x)
+    eta = np.linalg.norm(r) / (np.linalg.norm(K) * np.linalg.norm(x) + np.linalg.norm(b))
+    assert eta <= 1e-14, eta
+
+
+def test_cardinal_basis_rejects_bad_stencils():
+    from updes_amd import engine as E, _lib as L
+    oc, uc, prob, basis = _setup(R.POLYHARMONIC, 1.0, 3)
+    bad = basis.stencil.clone()
+    bad[5] = -3
+    c = basis.struct(None)
+    c.d_stencil = bad.data_ptr()
+    ws = L.workspace(256, "cuda")
+    st = L.lib().rbfx_cardinal_coeffs(prob.ref(), ctypes.byref(c), L.ptr(ws), ws.numel(), L.stream())
+    assert st == -1 and b"outside" in L.lib().rbfx_last_error()

@@ -52,6 +52,9 @@ def test_poisson_100_matches_oracle(precision):
     st = u.last_solve_stats()
     print("precision %s: rel-L2 vs oracle %.3e, stats %s" % (precision, err, st))
     assert err <= 1e-8
+    if precision == "fp32_ir":
+        # the result comes from the FP32 factorisation, not the FP64 fallback
+        assert st["fell_back"] is False and st["basis"] == "cardinal", st
     # discretisation error against the exact solution equals the oracle's
     e_gpu = np.linalg.norm(vals - g["exact"]) / np.linalg.norm(g["exact"])
     e_ref = np.linalg.norm(ref - g["exact"]) / np.linalg.norm(g["exact"])
@@ -77,7 +80,14 @@ def test_laplace_control_60_gradient(precision):
 
     u.clear_cache()
     c = c0.clone().requires_grad_(True)
-    loss_of(c).backward()
+    loss = loss_of(c)
+    st_fwd = u.last_solve_stats()
+    loss.backward()
+    st_adj = u.last_solve_stats()
+    if precision == "fp32_ir":
+        # forward and adjoint solves both come from the FP32 factorisation
+        assert st_fwd["fell_back"] is False and st_adj["fell_back"] is False, (st_fwd, st_adj)
+        assert st_adj["basis"] == "cardinal" and st_adj["ir_inner"] > 0, st_adj
     g = c.grad.detach().cpu().numpy()
     oc = OC.SquareCloud(nx, nx, ft)
     q = OA.assemble_q(oc, 0.0, {"North": c0.cpu().numpy(), "South": np.zeros(nx), "East": np.zeros(nx - 2),
@@ -266,3 +276,29 @@ def test_large_poisson_40k_residual_and_gradient():
         Lm = bench.solve_and_grad(cloud, bcs, b - e, basis, "fp64", want_grad=False)[1].item()
         fd = (Lp - Lm) / (2 * h)
         assert abs(fd - gk[k]) <= 1e-6 * max(1e-3, np.abs(gk).max()), (k, fd, gk[k])
+
+
+def test_large_poisson_40k_fp32_cardinal_ir_matches_fp64():
+    """N = 40,000, the benchmark workload: the FP32-LU + GMRES-IR path (FP32 LU
+    of K in the local cardinal basis) gives nodal values within rel-L2 1e-8 of
+    the FP64 LU path and the same source-coefficient gradient, with both the
+    forward and the adjoint solve taken from the FP32 factorisation (no
+    fallback: the factor never built FP64 factors)."""
+    import updes_amd as u
+    import bench
+    from updes_amd import engine as E
+    cloud, bcs, b, basis = bench.fourier_problem(200, seed=0)
+    u.clear_cache()
+    v64, l64, g64 = bench.solve_and_grad(cloud, bcs, b, basis, "fp64")
+    u.clear_cache()
+    v32, l32, g32 = bench.solve_and_grad(cloud, bcs, b, basis, "fp32_ir")
+    fac = list(E._K_CACHE.d.values())[-1]
+    st = u.last_solve_stats()
+    ev = (torch.linalg.norm(v32 - v64) / torch.linalg.norm(v64)).item()
+    eg = (torch.linalg.norm(g32 - g64) / torch.linalg.norm(g64)).item()
+    print("N=40k fp32_ir: vals rel-L2 vs fp64 %.3e, grad %.3e, adjoint stats %s" % (ev, eg, st))
+    assert fac.mode == "cardinal" and fac.lu64 is None and fac.lu32 is not None
+    assert st["fell_back"] is False
+    assert ev <= 1e-8
+    assert eg <= 1e-6
+    u.clear_cache()

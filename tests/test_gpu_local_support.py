@@ -157,8 +157,13 @@ def test_local_poisson_40x40_both_precisions(precision):
     q = OA.assemble_q(oc, -2 * np.pi ** 2 * np.sin(np.pi * x[:oc.Ni]) * np.sin(np.pi * y[:oc.Ni]), bc)
     u_ref, _, _ = OS.pde_solver_kpath(C.LAPLACIAN, oc, R.POLYHARMONIC, 1.0, 6, q)
     rel = np.linalg.norm(sol.vals.cpu().numpy() - u_ref) / np.linalg.norm(u_ref)
-    print("%s: rel-L2 %.2e stats %s" % (precision, rel, u.last_solve_stats()), flush=True)
+    st = u.last_solve_stats()
+    print("%s: rel-L2 %.2e stats %s" % (precision, rel, st), flush=True)
     assert rel < 1e-8
+    if precision == "fp32_ir":
+        # local supports: the FP32 LU of K itself preconditions (the cardinal
+        # basis needs every node in every support)
+        assert st["basis"] == "lu", st
 
 
 def test_local_laplace_control_gradient():

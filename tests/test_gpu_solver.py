@@ -105,6 +105,8 @@ def test_poisson_small_both_precisions(precision):
     assert np.linalg.norm(vals - u_ref) / np.linalg.norm(u_ref) < 1e-8
     st = u.last_solve_stats()
     assert st["precision"] == precision
+    if precision == "fp32_ir":
+        assert st["fell_back"] is False and st["basis"] == "cardinal", st
 
 
 def test_eval_value_gradient_laplacian():
@@ -262,3 +264,35 @@ def test_solves_on_successive_clouds_without_clearing_the_cache():
         assert np.linalg.norm(got - u_ref) <= 1e-9 * np.linalg.norm(u_ref), (nx, ny)
         del cloud, sol
         gc.collect()
+
+
+@pytest.mark.parametrize("nx,ny,ft,deg", [(30, 20, C.README_FACETS, 1),
+                                          (60, 60, {"North": "d", "South": "d", "East": "d", "West": "d"}, 1),
+                                          (14, 12, C.FACET_SETS[2], 2), (14, 12, C.FACET_SETS[3], 1)])
+def test_assemble_B_and_steady_sol_mat_match_oracle(nx, ny, ft, deg):
+    """B = (D inv(A))[:, :N] (assembly.py:366-401) and SteadySol.mat against
+    the oracle's assemble_B at the README (30 x 20) and adjoint-control
+    (60 x 60) sizes, plus Robin and periodic facet sets; inv(A) against the
+    oracle's inverse."""
+    import updes_amd as u
+    oc = OC.SquareCloud(nx, ny, ft)
+    uc = u.SquareCloud(Nx=nx, Ny=ny, facet_types=ft)
+    M = 3 if deg == 1 else 6
+    betas = {f: np.linspace(0.5, 1.5, len(oc.facet_nodes[f])) for f, t in oc.facet_types.items() if t == "r"}
+    rb = OA.robin_coefficients(oc, betas)
+    rc, _ = u.duplicate_robin_coeffs({f: (0.0, betas[f]) if t == "r" else 0.0 for f, t in uc.facet_types.items()}, uc)
+    B = u.assemble_B(_laplace_op, uc, u.polyharmonic, M, None, rc).cpu().numpy()
+    Bref = OA.assemble_B(C.LAPLACIAN, oc, R.POLYHARMONIC, 1.0, M, rb)
+    kappa = np.linalg.cond(OA.assemble_A(oc, R.POLYHARMONIC, 1.0, M))
+    # two FP64 inverses of A agree to ~kappa(A) eps; B = D inv(A) inherits that
+    tol = max(1e-10, 10 * kappa * np.finfo(float).eps)
+    assert np.linalg.norm(B - Bref) <= tol * np.linalg.norm(Bref), (np.linalg.norm(B - Bref) / np.linalg.norm(Bref), tol)
+    iA = u.assemble_invert_A(uc, u.polyharmonic, M).cpu().numpy()
+    A = OA.assemble_A(oc, R.POLYHARMONIC, 1.0, M)
+    assert np.linalg.norm(A @ iA - np.eye(oc.N + M)) <= 100 * kappa * np.finfo(float).eps * np.sqrt(oc.N + M)
+    # SteadySol.mat is the same B
+    bcs = {f: 0.0 for f in ft}
+    if any(t == "r" for t in uc.facet_types.values()):
+        bcs.update({f: (0.0, torch.as_tensor(betas[f])) for f, t in uc.facet_types.items() if t == "r"})
+    sol = u.pde_solver(_laplace_op, lambda x, centers, rbf, fields: 1.0, uc, bcs, u.polyharmonic, deg)
+    np.testing.assert_allclose(sol.mat.cpu().numpy(), B, rtol=0, atol=1e-12 * np.abs(B).max())

@@ -56,6 +56,13 @@ class IrStats(ctypes.Structure):
                 ("stalled", _i32), ("reserved", _i32)]
 
 
+class Cardinal(ctypes.Structure):
+    """Mirror of ``rbfx_cardinal``."""
+    _fields_ = [("N", _i64), ("M", _i64), ("k", _i64), ("d_x", _p), ("d_y", _p), ("d_stencil", _p),
+                ("d_beta", _p), ("d_alpha", _p), ("d_rs", _p), ("d_rev_ptr", _p), ("d_rev_idx", _p),
+                ("d_ptp_inv", _p)]
+
+
 class ProfEntry(ctypes.Structure):
     """Mirror of ``rbfx_prof_entry``."""
     _fields_ = [("name", ctypes.c_char * 48), ("launches", _i64), ("total_ms", _d), ("flops", _d), ("bytes", _d)]
@@ -81,12 +88,21 @@ _SIGS = {
     "rbfx_getrs_f64": (ctypes.c_int, [ctypes.c_int, _i64, _p, _i64, _p, _p, _p, _p, _sz, _p]),
     "rbfx_getrs_f32": (ctypes.c_int, [ctypes.c_int, _i64, _p, _i64, _p, _p, _p, _p, _sz, _p]),
     "rbfx_getrs_status": (ctypes.c_int, [_i64, _p, _sz, _p]),
+    "rbfx_getrs_multi_workspace": (_sz, [_i64, _i64]),
+    "rbfx_getrs_multi_f64": (ctypes.c_int, [_i64, _p, _i64, _p, _p, _i64, _p, _i64, _p, _sz, _p]),
     "rbfx_shutdown": (ctypes.c_int, []),
     "rbfx_gemv_workspace": (_sz, [ctypes.c_int, _i64, _i64]),
     "rbfx_gemv_f64": (ctypes.c_int, [ctypes.c_int, _i64, _i64, _d, _p, _i64, _p, _d, _p, _p, _sz, _p]),
     "rbfx_gmres_ir_workspace": (_sz, [_i64, _i32]),
     "rbfx_gmres_ir": (ctypes.c_int, [ctypes.c_int, _i64, ctypes.POINTER(Problem), _p, _i64, _p, _i64, _p, _p,
                                      _p, _p, _d, _i32, _i32, _p, _sz, ctypes.POINTER(IrStats), _p]),
+    "rbfx_cardinal_coeffs": (ctypes.c_int, [ctypes.POINTER(Problem), ctypes.POINTER(Cardinal), _p, _sz, _p]),
+    "rbfx_cardinal_form": (ctypes.c_int, [ctypes.POINTER(Cardinal), _p, _i64, _p, _i64, _p, _sz, _p]),
+    "rbfx_gmres_ir_cardinal_workspace": (_sz, [_i64, _i64, _i32]),
+    "rbfx_gmres_ir_cardinal": (ctypes.c_int, [ctypes.c_int, ctypes.POINTER(Cardinal), _p, _i64, _d, _p, _i64, _p,
+                                              _p, _p, _p, _d, _i32, _i32, _p, _sz, ctypes.POINTER(IrStats), _p]),
+    "rbfx_frobenius_workspace": (_sz, [_i64]),
+    "rbfx_frobenius_f64": (ctypes.c_int, [_i64, _i64, _p, _i64, _p, _i64, ctypes.POINTER(_d), _p, _sz, _p]),
     "rbfx_profile_begin": (ctypes.c_int, []),
     "rbfx_profile_end": (ctypes.c_int, [ctypes.POINTER(ProfEntry), ctypes.c_int, ctypes.POINTER(ctypes.c_int)]),
     "rbfx_gemm_f64": (ctypes.c_int, [_i64, _i64, _i64, _d, _p, _i64, _p, _i64, _d, _p, _i64, _p]),

@@ -38,11 +38,11 @@ def assemble_P(cloud, nb_monomials):
 
 
 def assemble_invert_A(cloud, rbf, nb_monomials):
-    """inv(A) (``assembly.py:87-90``); dense, for small clouds."""
+    """inv(A) (``assembly.py:87-90``): one multi-right-hand-side solve with the
+    FP64 LU factors of A."""
     kind, param = rbf_spec(rbf)
     fac = E.a_factor(cloud, kind, param, nb_monomials)
-    eye = torch.eye(fac.n, dtype=torch.float64, device=fac.a.device)
-    return torch.stack([fac.solve(eye[:, j]) for j in range(fac.n)], dim=1)
+    return E.lu_inverse(fac)[:, :fac.n]
 
 
 def assemble_K(operator, cloud, rbf, nb_monomials, diff_args=None, robin_coeffs=None):
@@ -68,12 +68,13 @@ def assemble_bd_Phi_P(cloud, rbf, nb_monomials, robin_coeffs=None):
 
 
 def assemble_B(operator, cloud, rbf, nb_monomials, diff_args, robin_coeffs):
-    """B = (D inv(A))[:, :N] (``assembly.py:366-401``); N transposed solves."""
-    K = assemble_K(operator, cloud, rbf, nb_monomials, diff_args, robin_coeffs)
+    """B = (D inv(A))[:, :N] (``assembly.py:366-401``): inv(A) by one
+    multi-right-hand-side solve, D inv(A) by one MFMA GEMM."""
+    desc = lower_diff_operator(operator, cloud, rbf, diff_args)
+    beta = _robin_beta_vector(cloud, robin_coeffs or {})
+    p = _problem(cloud, rbf, nb_monomials, desc, beta)
     kind, param = rbf_spec(rbf)
-    fa = E.a_factor(cloud, kind, param, nb_monomials)
-    rows = [fa.solve(K[i].contiguous(), trans=True)[:cloud.N] for i in range(cloud.N)]
-    return torch.stack(rows)
+    return E.b_matrix(p, E.a_factor(cloud, kind, param, nb_monomials))
 
 
 __all__ = ["assemble_A", "assemble_Phi", "assemble_P", "assemble_invert_A", "assemble_K", "assemble_op_Phi_P",

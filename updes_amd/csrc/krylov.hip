@@ -3,18 +3,22 @@
 // The reference solves B u = q with lineax QR in FP64 only
 // (updes/operators.py:612-613).  Here the collocation system K c = [q; 0] can
 // be factored in FP32 (MFMA) and the FP64 solution recovered by GMRES-IR
-// (Carson & Higham, SIAM J. Sci. Comput. 40(2), 2018): the FP32 factors
-// right-precondition restarted GMRES on the correction equation, residuals
-// b - K x are formed in FP64 either from the stored FP64 matrix (GEMV) or
-// matrix-free from the node coordinates (rbfx_matvec), and refinement stops
-// with RBFX_ERR_STALL when it fails to reach the tolerance so that the caller
-// can fall back to an FP64 factorisation.
+// (Carson & Higham, SIAM J. Sci. Comput. 40(2), 2018): FP32 factors
+// right-precondition restarted flexible GMRES on the correction equation,
+// residuals b - K x are formed in FP64 either from the stored FP64 matrix
+// (GEMV) or matrix-free from the node coordinates (rbfx_matvec), and
+// refinement stops with RBFX_ERR_STALL when it fails to reach the tolerance
+// so that the caller can fall back to an FP64 factorisation.  The
+// preconditioner is either the FP32 LU of K itself (here) or of K in a local
+// cardinal basis (cardinal.hip).
 #include <math.h>
+#include <stdlib.h>
 
 #include <algorithm>
+#include <string>
 #include <vector>
 
-#include "common.h"
+#include "krylov.h"
 
 namespace rbfx {
 
@@ -100,6 +104,9 @@ static int64_t gemv_t_chunks(int64_t m, int64_t n) {
   return want;
 }
 
+int gemv_launch(int trans, int64_t m, int64_t n, double alpha, const double* A, int64_t lda, const double* x,
+                double beta, double* y, double* part, hipStream_t s);
+
 size_t gemv_workspace(int trans, int64_t m, int64_t n) {
   if (!trans) return 0;
   return ws_round(sizeof(double) * (size_t)gemv_t_chunks(m, n) * (size_t)n);
@@ -108,6 +115,19 @@ size_t gemv_workspace(int trans, int64_t m, int64_t n) {
 int gemv(int trans, int64_t m, int64_t n, double alpha, const double* A, int64_t lda, const double* x,
          double beta, double* y, double* part, hipStream_t s) {
   if (m <= 0 || n <= 0) return RBFX_OK;
+  return gemv_launch(trans, m, n, alpha, A, lda, x, beta, y, part, s);
+}
+
+int residual_gemv(int trans, int64_t n, const double* K, int64_t ld, const double* x, double* y, double* part,
+                  hipStream_t s) {
+  const int pid = prof_start(trans ? "residual_gemv_t" : "residual_gemv", s);
+  const int st = gemv(trans, n, n, 1.0, K, ld, x, 0.0, y, part, s);
+  prof_stop(pid, s, 2.0 * n * n, 8.0 * ((double)n * n + 2.0 * n));
+  return st;
+}
+
+int gemv_launch(int trans, int64_t m, int64_t n, double alpha, const double* A, int64_t lda, const double* x,
+                double beta, double* y, double* part, hipStream_t s) {
   if (!trans) {
     if (m < 2048) {
       hipLaunchKernelGGL(gemv_n_block_kernel, dim3((unsigned)m), dim3(kGvThreads), 0, s, m, n, alpha, A,
@@ -170,6 +190,35 @@ __global__ void scale_copy_kernel(int64_t n, double a, const double* __restrict_
   if (i < n) y[i] = a * x[i];
 }
 
+__global__ void fill_kernel(int64_t n, double v, double* __restrict__ y) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) y[i] = v;
+}
+
+// Frobenius norm partials: sum of squares of rows [blockIdx.x*rows, ...),
+// row i scaled by rs[i] when i < nrs.
+__global__ __launch_bounds__(256) void fro_partial_kernel(int64_t m, int64_t n, const double* __restrict__ A,
+                                                          int64_t lda, int64_t rows, const double* __restrict__ rs,
+                                                          int64_t nrs, double* __restrict__ part) {
+  double s = 0.0;
+  const int64_t r0 = (int64_t)blockIdx.x * rows;
+  const int64_t r1 = r0 + rows < m ? r0 + rows : m;
+  for (int64_t i = r0; i < r1; ++i) {
+    const double f = (rs && i < nrs) ? rs[i] : 1.0;
+    double t = 0.0;
+    for (int64_t j = threadIdx.x; j < n; j += 256) {
+      const double v = A[i * lda + j];
+      t += v * v;
+    }
+    s += f * f * t;
+  }
+  __shared__ double red[4];
+  for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off, 64);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) part[blockIdx.x] = red[0] + red[1] + red[2] + red[3];
+}
+
 static int dot(int64_t n, const double* a, const double* b, double* part, double* out, hipStream_t s) {
   hipLaunchKernelGGL(dot_partial_kernel, dim3(kDotBlocks), dim3(256), 0, s, n, a, b, part);
   RBFX_CHECK_LAUNCH();
@@ -196,77 +245,61 @@ extern "C" int rbfx_gemv_f64(int trans, int64_t m, int64_t n, double alpha, cons
   return gemv(trans, m, n, alpha, d_A, lda, d_x, beta, d_y, static_cast<double*>(d_work), as_stream(stream));
 }
 
-extern "C" size_t rbfx_gmres_ir_workspace(int64_t n, int32_t restart) {
-  if (n <= 0 || restart <= 0) return 0;
-  size_t b = 0;
-  b += ws_round(sizeof(double) * (size_t)(restart + 1) * (size_t)n);  // V
-  b += 4 * ws_round(sizeof(double) * (size_t)n);                      // r, w, z, u
-  b += ws_round(sizeof(double) * (size_t)(2 * restart + 8));          // h, h2, norms
-  b += ws_round(sizeof(double) * kDotBlocks);                         // dot partials
-  b += ws_round(rbfx_getrs_workspace(n));
-  const size_t gw = std::max({gemv_workspace(1, restart + 1, n), gemv_workspace(1, n, n), matvec_t_ws_bytes(n)});
-  b += ws_round(gw);
-  return b + 4096;
+extern "C" size_t rbfx_frobenius_workspace(int64_t m) {
+  return ws_round(sizeof(double) * (size_t)std::max<int64_t>(1, std::min<int64_t>(m, 4096)) + 64);
 }
 
-namespace {
-struct IrOps {
-  int trans;
-  int64_t n;
-  const rbfx_problem* p;
-  const double* K64;
-  int64_t ld64;
-  const float* LU32;
-  int64_t ld32;
-  const int64_t* perm;
-  const double* dinv;
-  void* ts_work;
-  size_t ts_bytes;
-  double* part;       // gemv^T / matvec^T partials
-  size_t part_bytes;
-  hipStream_t s;
-  int applyK(const double* v, double* w) const {
-    if (K64) return gemv(trans, n, n, 1.0, K64, ld64, v, 0.0, w, part, s);
-    return rbfx_matvec(p, RBFX_MAT_K, trans, v, w, part, part_bytes, s);
-  }
-  int applyMinv(double* v) const {
-    return rbfx_getrs_f32(trans, n, LU32, ld32, perm, dinv, v, ts_work, ts_bytes, s);
-  }
-};
-}  // namespace
-
-extern "C" int rbfx_gmres_ir(int trans, int64_t n, const rbfx_problem* p, const double* d_K64, int64_t ld64,
-                             const float* d_LU32, int64_t ld32, const int64_t* d_perm32,
-                             const double* d_dinv32, const double* d_b, double* d_x, double tol,
-                             int32_t restart, int32_t max_inner, void* d_work, size_t work_bytes,
-                             rbfx_ir_stats* stats, void* stream) {
-  RBFX_REQUIRE(n > 0 && restart > 0 && max_inner > 0 && tol > 0.0, "bad gmres_ir parameters");
-  RBFX_REQUIRE(d_LU32 && d_perm32 && d_dinv32 && d_b && d_x && d_b != d_x, "bad gmres_ir pointers");
-  RBFX_REQUIRE(d_K64 || p, "need a stored K or a problem for the residual");
-  RBFX_REQUIRE(!d_K64 || ld64 >= n, "ld64 < n");
-  RBFX_REQUIRE(!p || p->N + p->M == n, "problem size != n");
-  RBFX_REQUIRE(work_bytes >= rbfx_gmres_ir_workspace(n, restart), "gmres_ir workspace too small");
+extern "C" int rbfx_frobenius_f64(int64_t m, int64_t n, const double* d_A, int64_t lda, const double* d_row_scale,
+                                  int64_t n_scaled, double* h_norm, void* d_work, size_t work_bytes, void* stream) {
+  RBFX_REQUIRE(m > 0 && n > 0 && lda >= n && d_A && h_norm, "bad frobenius args");
+  RBFX_REQUIRE(d_work && work_bytes >= rbfx_frobenius_workspace(m), "frobenius workspace too small");
   hipStream_t s = as_stream(stream);
-  const int m = restart;
-  Workspace ws(d_work, work_bytes);
+  const int64_t blocks = std::min<int64_t>(m, 4096);
+  const int64_t rows = ceil_div(m, blocks);
+  const int64_t used = ceil_div(m, rows);
+  double* part = static_cast<double*>(d_work);
+  RBFX_REQUIRE(!d_row_scale || (n_scaled >= 0 && n_scaled <= m), "bad row scale count");
+  hipLaunchKernelGGL(fro_partial_kernel, dim3((unsigned)used), dim3(256), 0, s, m, n, d_A, lda, rows, d_row_scale,
+                     n_scaled, part);
+  RBFX_CHECK_LAUNCH();
+  std::vector<double> hp(used);
+  RBFX_HIP(hipMemcpyAsync(hp.data(), part, used * sizeof(double), hipMemcpyDeviceToHost, s));
+  RBFX_HIP(hipStreamSynchronize(s));
+  double t = 0.0;
+  for (double v : hp) t += v;
+  *h_norm = sqrt(t);
+  return RBFX_OK;
+}
+
+namespace rbfx {
+
+size_t fgmres_ws_bytes(int64_t n, int m) {
+  size_t b = 0;
+  b += ws_round(sizeof(double) * (size_t)(m + 1) * (size_t)n);  // V
+  b += ws_round(sizeof(double) * (size_t)m * (size_t)n);        // Z = M^-1 V
+  b += 3 * ws_round(sizeof(double) * (size_t)n);                // r, w, u
+  b += ws_round(sizeof(double) * (size_t)(2 * m + 8));          // h, h2, norms
+  b += ws_round(sizeof(double) * kDotBlocks);                   // dot partials
+  b += ws_round(std::max(gemv_workspace(1, m + 1, n), gemv_workspace(1, m, n)));
+  return b + 2048;
+}
+
+int fgmres_ir(PairOp& pre, ResidualOp& res, int64_t n, const double* d_b, double* d_x, const FgmresOpts& o,
+              void* work, size_t work_bytes, rbfx_ir_stats* stats, hipStream_t s) {
+  const int m = o.restart;
+  RBFX_REQUIRE(n > 0 && m > 0 && o.max_inner > 0 && o.tol > 0.0, "bad GMRES-IR parameters");
+  RBFX_REQUIRE(work_bytes >= fgmres_ws_bytes(n, m), "GMRES-IR workspace too small");
+  Workspace ws(work, work_bytes);
   double* V = ws.take<double>((size_t)(m + 1) * n);
+  double* Z = ws.take<double>((size_t)m * n);
   double* r = ws.take<double>(n);
   double* w = ws.take<double>(n);
-  double* z = ws.take<double>(n);
   double* u = ws.take<double>(n);
   double* hd = ws.take<double>(2 * m + 8);
   double* dpart = ws.take<double>(kDotBlocks);
-  const size_t tsb = rbfx_getrs_workspace(n);
-  void* tsw = ws.take<char>(tsb);
-  const size_t gw = std::max({gemv_workspace(1, m + 1, n), gemv_workspace(1, n, n), matvec_t_ws_bytes(n)});
+  const size_t gw = std::max(gemv_workspace(1, m + 1, n), gemv_workspace(1, m, n));
   double* part = ws.take<double>(gw / sizeof(double) + 1);
-  RBFX_REQUIRE(part != nullptr, "gmres_ir workspace carve failed");
-  IrOps ops{trans, n, p, d_K64, ld64, d_LU32, ld32, d_perm32, d_dinv32, tsw, tsb, part, gw, s};
-  if (!d_K64) {
-    // at least the column kernel's workspace (the support-only transposed
-    // product on local supports needs more and is then not used here)
-    RBFX_REQUIRE(gw >= (trans ? matvec_t_ws_bytes(n) : 0), "matvec workspace too small");
-  }
+  RBFX_REQUIRE(part != nullptr, "GMRES-IR workspace carve failed");
   const unsigned g = (unsigned)ceil_div(n, 256);
   int st;
   rbfx_ir_stats loc{0, 0, 0.0, 0, 0};
@@ -282,56 +315,92 @@ extern "C" int rbfx_gmres_ir(int trans, int64_t n, const rbfx_problem* p, const 
   double bn2 = 0.0;
   if ((st = host_dot(d_b, d_b, bn2))) return st;
   const double bn = sqrt(bn2);
+  hipLaunchKernelGGL(fill_kernel, dim3(g), dim3(256), 0, s, n, 0.0, d_x);
+  RBFX_CHECK_LAUNCH();
   if (bn == 0.0) {
-    RBFX_HIP(hipMemsetAsync(d_x, 0, n * sizeof(double), s));
     RBFX_HIP(hipStreamSynchronize(s));
     if (stats) *stats = loc;
     return RBFX_OK;
   }
-  // x0 = M^-1 b
-  RBFX_HIP(hipMemcpyAsync(d_x, d_b, n * sizeof(double), hipMemcpyDeviceToDevice, s));
-  if ((st = ops.applyMinv(d_x))) return st;
+  hipLaunchKernelGGL(scale_copy_kernel, dim3(g), dim3(256), 0, s, n, 1.0, d_b, r);
+  RBFX_CHECK_LAUNCH();
 
   std::vector<double> H((size_t)(m + 1) * m), cs(m), sn(m), gv(m + 1), hh(2 * m + 8), y(m);
-  double prev = INFINITY;
-  int stalls = 0;
+  // RBFX_IR_TRACE=1: one stderr line per cycle with the Arnoldi estimates
+  static const bool trace = getenv("RBFX_IR_TRACE") && getenv("RBFX_IR_TRACE")[0] == '1';
+  std::string tline;
+  double best = INFINITY;
+  int noprog = 0;
+  bool est_met = false;  // the last cycle's Arnoldi estimate reached its target
   int result = RBFX_ERR_STALL;
-  for (int outer = 0;; ++outer) {
-    // r = b - K x
-    if ((st = ops.applyK(d_x, w))) return st;
-    hipLaunchKernelGGL(scale_copy_kernel, dim3(g), dim3(256), 0, s, n, 1.0, d_b, r);
-    hipLaunchKernelGGL(axpby_kernel, dim3(g), dim3(256), 0, s, n, -1.0, w, 1.0, r);
-    RBFX_CHECK_LAUNCH();
+  for (int cycle = 0;; ++cycle) {
+    if (cycle > 0) {
+      // r = b - K x
+      if ((st = res.applyK(d_x, w))) return st;
+      hipLaunchKernelGGL(scale_copy_kernel, dim3(g), dim3(256), 0, s, n, 1.0, d_b, r);
+      hipLaunchKernelGGL(axpby_kernel, dim3(g), dim3(256), 0, s, n, -1.0, w, 1.0, r);
+      RBFX_CHECK_LAUNCH();
+    }
     double rn2 = 0.0;
     if ((st = host_dot(r, r, rn2))) return st;
     const double rn = sqrt(rn2);
     const double rel = rn / bn;
-    loc.outer_iters = outer;
+    loc.outer_iters = cycle;
     loc.final_relres = rel;
+    if (trace) {
+      if (!tline.empty()) fprintf(stderr, "%s\n", tline.c_str());
+      char buf[96];
+      snprintf(buf, sizeof(buf), "[rbfx ir] n=%lld cycle %d relres %.3e inner %d:", (long long)n, cycle, rel,
+               loc.inner_iters);
+      tline = buf;
+    }
     if (!(rel == rel)) break;  // NaN
-    if (rel <= tol) {
+    if (rel <= o.tol) {
       result = RBFX_OK;
       break;
     }
-    if (loc.inner_iters >= max_inner) break;
-    if (rel > 0.5 * prev) {
-      if (++stalls >= 3) break;
+    double eta = INFINITY;
+    if (cycle > 0 && o.knorm > 0.0) {
+      double xn2 = 0.0;
+      if ((st = host_dot(d_x, d_x, xn2))) return st;
+      eta = rn / (o.knorm * sqrt(xn2) + bn);
+      if (trace) {
+        char buf[40];
+        snprintf(buf, sizeof(buf), " (eta %.1e)", eta);
+        tline += buf;
+      }
+      if (eta <= kStopEta) {
+        result = RBFX_OK;
+        break;
+      }
     }
-    prev = std::min(prev, rel);
+    if (rel <= 0.5 * best) {
+      best = rel;
+      noprog = 0;
+    } else {
+      ++noprog;
+      // the Arnoldi estimate met its target, the true residual did not
+      // follow: at the FP64 floor if the backward error is small
+      if (est_met && eta <= kFloorEta) {
+        result = RBFX_OK;
+        break;
+      }
+      if (noprog >= 3) break;
+    }
+    if (loc.inner_iters >= o.max_inner) break;
 
-    // GMRES(m) on K M^-1 y = r
+    // GMRES(m) on K M^-1 y = r, flexible form: Z keeps M^-1 V
     hipLaunchKernelGGL(scale_copy_kernel, dim3(g), dim3(256), 0, s, n, 1.0 / rn, r, V);
     RBFX_CHECK_LAUNCH();
     std::fill(gv.begin(), gv.end(), 0.0);
     gv[0] = rn;
-    const double inner_tol = std::max(0.1 * tol * bn, 1e-10 * rn);
+    const double inner_tol = std::max(0.1 * o.tol * bn, 1e-10 * rn);
     int kdone = 0;
     bool early_stall = false;
+    est_met = false;
     for (int k = 0; k < m; ++k) {
       double* vk = V + (size_t)k * n;
-      RBFX_HIP(hipMemcpyAsync(z, vk, n * sizeof(double), hipMemcpyDeviceToDevice, s));
-      if ((st = ops.applyMinv(z))) return st;
-      if ((st = ops.applyK(z, w))) return st;
+      if ((st = pre.apply(vk, Z + (size_t)k * n, w))) return st;
       // classical Gram-Schmidt, two passes
       double* h1 = hd;
       double* h2 = hd + (m + 1);
@@ -365,48 +434,153 @@ extern "C" int rbfx_gmres_ir(int trans, int64_t n, const rbfx_problem* p, const 
       col[k + 1] = 0.0;
       gv[k + 1] = -sn[k] * gv[k];
       gv[k] = cs[k] * gv[k];
-      if (fabs(gv[k + 1]) <= inner_tol || hn == 0.0 || loc.inner_iters >= max_inner) break;
+      if (trace) {
+        char buf[24];
+        snprintf(buf, sizeof(buf), " %.1e", fabs(gv[k + 1]) / bn);
+        tline += buf;
+      }
+      if (fabs(gv[k + 1]) <= inner_tol || hn == 0.0) {
+        est_met = true;
+        break;
+      }
+      if (loc.inner_iters >= o.max_inner) break;
       // Early stall: extrapolate the observed geometric rate of the residual
       // estimate; give up when it cannot reach the tolerance in the budget.
       if (k + 1 >= kRateWindow) {
         const double est = fabs(gv[k + 1]);
         const double rate = pow(est / rn, 1.0 / (k + 1));
         const double need = (rate < 1.0 && rate > 0.0) ? log(inner_tol / est) / log(rate) : 1e30;
-        if (loc.inner_iters + need > (double)max_inner) {
+        if (loc.inner_iters + need > (double)o.max_inner) {
           early_stall = true;
           break;
         }
       }
     }
-    // y = R^-1 g
+    // y = R^-1 g ; x += Z y
     for (int i = kdone - 1; i >= 0; --i) {
       double t = gv[i];
       for (int j = i + 1; j < kdone; ++j) t -= H[(size_t)j * (m + 1) + i] * y[j];
       y[i] = H[(size_t)i * (m + 1) + i] != 0.0 ? t / H[(size_t)i * (m + 1) + i] : 0.0;
     }
-    // u = V^T y ; x += M^-1 u
     RBFX_HIP(hipMemcpyAsync(hd, y.data(), kdone * sizeof(double), hipMemcpyHostToDevice, s));
-    if ((st = gemv(1, kdone, n, 1.0, V, n, hd, 0.0, u, part, s))) return st;
-    if ((st = ops.applyMinv(u))) return st;
+    if ((st = gemv(1, kdone, n, 1.0, Z, n, hd, 0.0, u, part, s))) return st;
     hipLaunchKernelGGL(axpby_kernel, dim3(g), dim3(256), 0, s, n, 1.0, u, 1.0, d_x);
     RBFX_CHECK_LAUNCH();
     RBFX_HIP(hipStreamSynchronize(s));  // y (host) must outlive the copy
     if (early_stall) {
       // report the residual of the improved iterate, then stop
-      if ((st = ops.applyK(d_x, w))) return st;
+      if ((st = res.applyK(d_x, w))) return st;
       hipLaunchKernelGGL(scale_copy_kernel, dim3(g), dim3(256), 0, s, n, 1.0, d_b, r);
       hipLaunchKernelGGL(axpby_kernel, dim3(g), dim3(256), 0, s, n, -1.0, w, 1.0, r);
       RBFX_CHECK_LAUNCH();
       double e2 = 0.0;
       if ((st = host_dot(r, r, e2))) return st;
       loc.final_relres = sqrt(e2) / bn;
-      if (loc.final_relres <= tol) result = RBFX_OK;
+      loc.outer_iters = cycle + 1;
+      if (loc.final_relres <= o.tol) result = RBFX_OK;
       break;
     }
   }
+  if (trace && !tline.empty()) fprintf(stderr, "%s -> %s\n", tline.c_str(), result == RBFX_OK ? "ok" : "stall");
   loc.stalled = result == RBFX_OK ? 0 : 1;
   if (stats) *stats = loc;
-  if (result != RBFX_OK) set_error("GMRES-IR stalled: relres %.3e after %d inner iterations", loc.final_relres,
-                                   loc.inner_iters);
+  if (result != RBFX_OK)
+    set_error("GMRES-IR stalled: relres %.3e after %d inner iterations", loc.final_relres, loc.inner_iters);
   return result;
+}
+
+}  // namespace rbfx
+
+// ---- FP32 LU of K as the preconditioner -------------------------------------
+namespace {
+struct StoredOrMatrixFreeK : ResidualOp {
+  int trans;
+  int64_t n;
+  const rbfx_problem* p;
+  const double* K64;
+  int64_t ld64;
+  double* part;
+  size_t part_bytes;
+  hipStream_t s;
+  int applyK(const double* v, double* w) override {
+    if (K64) return residual_gemv(trans, n, K64, ld64, v, w, part, s);
+    return rbfx_matvec(p, RBFX_MAT_K, trans, v, w, part, part_bytes, s);
+  }
+};
+
+struct LuPair : PairOp {
+  int trans;
+  int64_t n;
+  const float* LU32;
+  int64_t ld32;
+  const int64_t* perm;
+  const double* dinv;
+  void* ts_work;
+  size_t ts_bytes;
+  ResidualOp* K;
+  hipStream_t s;
+  int apply(const double* v, double* z, double* w) override {
+    RBFX_HIP(hipMemcpyAsync(z, v, n * sizeof(double), hipMemcpyDeviceToDevice, s));
+    int st = rbfx_getrs_f32(trans, n, LU32, ld32, perm, dinv, z, ts_work, ts_bytes, s);
+    if (st) return st;
+    if ((st = rbfx_getrs_status(n, ts_work, ts_bytes, s))) return st;
+    return K->applyK(z, w);
+  }
+};
+
+size_t residual_ws_bytes(int64_t n) {
+  return std::max({gemv_workspace(1, n, n), matvec_t_ws_bytes(n)});
+}
+}  // namespace
+
+extern "C" size_t rbfx_gmres_ir_workspace(int64_t n, int32_t restart) {
+  if (n <= 0 || restart <= 0) return 0;
+  return fgmres_ws_bytes(n, restart) + ws_round(rbfx_getrs_workspace(n)) + ws_round(residual_ws_bytes(n)) + 1024;
+}
+
+extern "C" int rbfx_gmres_ir(int trans, int64_t n, const rbfx_problem* p, const double* d_K64, int64_t ld64,
+                             const float* d_LU32, int64_t ld32, const int64_t* d_perm32,
+                             const double* d_dinv32, const double* d_b, double* d_x, double tol,
+                             int32_t restart, int32_t max_inner, void* d_work, size_t work_bytes,
+                             rbfx_ir_stats* stats, void* stream) {
+  RBFX_REQUIRE(n > 0 && restart > 0 && max_inner > 0 && tol > 0.0, "bad gmres_ir parameters");
+  RBFX_REQUIRE(d_LU32 && d_perm32 && d_dinv32 && d_b && d_x && d_b != d_x, "bad gmres_ir pointers");
+  RBFX_REQUIRE(d_K64 || p, "need a stored K or a problem for the residual");
+  RBFX_REQUIRE(!d_K64 || ld64 >= n, "ld64 < n");
+  RBFX_REQUIRE(!p || p->N + p->M == n, "problem size != n");
+  RBFX_REQUIRE(work_bytes >= rbfx_gmres_ir_workspace(n, restart), "gmres_ir workspace too small");
+  hipStream_t s = as_stream(stream);
+  Workspace ws(d_work, work_bytes);
+  const size_t fw = fgmres_ws_bytes(n, restart);
+  void* fwork = ws.take<char>(fw);
+  const size_t tsb = rbfx_getrs_workspace(n);
+  void* tsw = ws.take<char>(tsb);
+  const size_t rb = residual_ws_bytes(n);
+  double* part = ws.take<double>(rb / sizeof(double) + 1);
+  RBFX_REQUIRE(fwork && tsw && part, "gmres_ir workspace carve failed");
+  StoredOrMatrixFreeK K;
+  K.trans = trans;
+  K.n = n;
+  K.p = p;
+  K.K64 = d_K64;
+  K.ld64 = ld64;
+  K.part = part;
+  K.part_bytes = rb;
+  K.s = s;
+  LuPair pre;
+  pre.trans = trans;
+  pre.n = n;
+  pre.LU32 = d_LU32;
+  pre.ld32 = ld32;
+  pre.perm = d_perm32;
+  pre.dinv = d_dinv32;
+  pre.ts_work = tsw;
+  pre.ts_bytes = tsb;
+  pre.K = &K;
+  pre.s = s;
+  FgmresOpts o;
+  o.tol = tol;
+  o.restart = restart;
+  o.max_inner = max_inner;
+  return fgmres_ir(pre, K, n, d_b, d_x, o, fwork, fw, stats, s);
 }

@@ -17,6 +17,9 @@
 
 namespace rbfx {
 
+int gemm_f64(int64_t m, int64_t n, int64_t k, double alpha, const double* A, int64_t lda, const double* B,
+             int64_t ldb, double beta, double* C, int64_t ldc, hipStream_t s);
+
 constexpr int TB = 64;  // block size of the sweep and of the diagonal inverses
 constexpr int kTsThreads = 256;
 
@@ -366,6 +369,78 @@ static int getrs(int trans, int64_t n, const T* LU, int64_t lda, const int64_t* 
   return RBFX_OK;
 }
 
+// ---- many right-hand sides -------------------------------------------------------
+__global__ void gather_rows_kernel(const double* __restrict__ B, int64_t ldb, const int64_t* __restrict__ perm,
+                                   int64_t n, int64_t nrhs, double* __restrict__ T, int64_t ldt) {
+  const int64_t i = blockIdx.y;
+  const int64_t src = perm[i];
+  for (int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; c < nrhs; c += (int64_t)gridDim.x * blockDim.x)
+    T[i * ldt + c] = B[src * ldb + c];
+}
+
+__global__ void copy_rows_kernel(const double* __restrict__ S, int64_t lds, int64_t rows, int64_t cols,
+                                 double* __restrict__ D, int64_t ldd) {
+  const int64_t i = blockIdx.y;
+  if (i >= rows) return;
+  for (int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; c < cols; c += (int64_t)gridDim.x * blockDim.x)
+    D[i * ldd + c] = S[i * lds + c];
+}
+
+static size_t getrs_multi_ws(int64_t n, int64_t nrhs) {
+  const int64_t ld = ceil_div(nrhs, 8) * 8;
+  return ws_round(sizeof(double) * (size_t)n * ld) + ws_round(sizeof(double) * (size_t)TB * ld) + 256;
+}
+
+// A X = B for nrhs right-hand sides (B n x nrhs row-major, overwritten by X)
+// with FP64 LU factors: rows gathered by the permutation, then blocked
+// forward and backward substitution over the TB-row blocks, each block one
+// MFMA GEMM against the blocks already solved and one with its precomputed
+// diagonal inverse.
+static int getrs_multi(int64_t n, const double* LU, int64_t lda, const int64_t* perm, const double* dinv,
+                       int64_t nrhs, double* B, int64_t ldb, void* work, size_t work_bytes, hipStream_t s) {
+  RBFX_REQUIRE(n > 0 && nrhs > 0 && lda >= n && ldb >= nrhs && LU && perm && dinv && B, "bad getrs_multi args");
+  RBFX_REQUIRE(work && work_bytes >= getrs_multi_ws(n, nrhs), "getrs_multi workspace too small");
+  const int64_t ldt = ceil_div(nrhs, 8) * 8;
+  Workspace ws(work, work_bytes);
+  double* T = ws.take<double>((size_t)n * ldt);
+  double* W = ws.take<double>((size_t)TB * ldt);
+  RBFX_REQUIRE(T && W, "getrs_multi workspace carve failed");
+  const unsigned gx = (unsigned)std::min<int64_t>(ceil_div(nrhs, 256), 64);
+  RBFX_REQUIRE(n < (int64_t)65536 * 65536, "getrs_multi: n too large");
+  hipLaunchKernelGGL(gather_rows_kernel, dim3(gx, (unsigned)n), dim3(256), 0, s, B, ldb, perm, n, nrhs, T, ldt);
+  RBFX_CHECK_LAUNCH();
+  const int64_t nb = ceil_div(n, TB);
+  int st;
+  auto diag = [&](int64_t blk, int which) -> int {
+    const int64_t o = blk * TB;
+    const int64_t sz = imin64(TB, n - o);
+    const double* D = dinv + (blk * 2 + which) * TB * TB;
+    if ((st = gemm_f64(sz, nrhs, sz, 1.0, D, TB, T + o * ldt, ldt, 0.0, W, ldt, s))) return st;
+    hipLaunchKernelGGL(copy_rows_kernel, dim3(gx, (unsigned)sz), dim3(256), 0, s, W, ldt, sz, nrhs, T + o * ldt, ldt);
+    RBFX_CHECK_LAUNCH();
+    return RBFX_OK;
+  };
+  for (int64_t blk = 0; blk < nb; ++blk) {  // L (unit lower)
+    const int64_t o = blk * TB;
+    const int64_t sz = imin64(TB, n - o);
+    if (o > 0 && (st = gemm_f64(sz, nrhs, o, -1.0, LU + o * lda, lda, T, ldt, 1.0, T + o * ldt, ldt, s)))
+      return st;
+    if ((st = diag(blk, 0))) return st;
+  }
+  for (int64_t blk = nb - 1; blk >= 0; --blk) {  // U
+    const int64_t o = blk * TB;
+    const int64_t sz = imin64(TB, n - o);
+    const int64_t e = o + sz;
+    if (e < n && (st = gemm_f64(sz, nrhs, n - e, -1.0, LU + o * lda + e, lda, T + e * ldt, ldt, 1.0, T + o * ldt,
+                                ldt, s)))
+      return st;
+    if ((st = diag(blk, 1))) return st;
+  }
+  hipLaunchKernelGGL(copy_rows_kernel, dim3(gx, (unsigned)n), dim3(256), 0, s, T, ldt, n, nrhs, B, ldb);
+  RBFX_CHECK_LAUNCH();
+  return RBFX_OK;
+}
+
 // Status of the last solve queued with this workspace (synchronises).
 static int getrs_status(int64_t n, void* work, size_t work_bytes, hipStream_t s) {
   RBFX_REQUIRE(n > 0 && work, "bad getrs_status args");
@@ -405,6 +480,16 @@ static int prepare(int64_t n, const T* LU, int64_t lda, const int64_t* ipiv, int
 }  // namespace rbfx
 
 using namespace rbfx;
+
+extern "C" size_t rbfx_getrs_multi_workspace(int64_t n, int64_t nrhs) {
+  return (n <= 0 || nrhs <= 0) ? 0 : getrs_multi_ws(n, nrhs);
+}
+
+extern "C" int rbfx_getrs_multi_f64(int64_t n, const double* d_LU, int64_t lda, const int64_t* d_perm,
+                                    const double* d_dinv, int64_t nrhs, double* d_B, int64_t ldb, void* d_work,
+                                    size_t work_bytes, void* stream) {
+  return getrs_multi(n, d_LU, lda, d_perm, d_dinv, nrhs, d_B, ldb, d_work, work_bytes, as_stream(stream));
+}
 
 extern "C" int64_t rbfx_getrs_dinv_elems(int64_t n) { return n <= 0 ? 0 : ceil_div(n, TB) * 2 * TB * TB; }
 

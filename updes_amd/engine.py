@@ -286,50 +286,235 @@ class LU:
         return x
 
 
+def lu_inverse(lu):
+    """inv(A) from FP64 LU factors: one multi-right-hand-side solve with the
+    identity ((n, ld) row-major; columns past n are zero)."""
+    if lu.dtype != torch.float64:
+        raise ValueError("lu_inverse needs FP64 factors")
+    n = lu.n
+    lib = L.lib()
+    X = torch.zeros((n, _ld(n)), dtype=torch.float64, device=lu.a.device)
+    X[:, :n].fill_diagonal_(1.0)
+    ws = L.workspace(lib.rbfx_getrs_multi_workspace(n, n), X.device)
+    L.check(lib.rbfx_getrs_multi_f64(n, L.ptr(lu.a), lu.ld, L.ptr(lu.perm), L.ptr(lu.dinv), n, L.ptr(X), X.shape[1],
+                                     L.ptr(ws), ws.numel(), L.stream()))
+    return X
+
+
+def gemm(A, B, m, n, k, alpha=1.0):
+    """C = alpha A[:m, :k] B[:k, :n] (row-major FP64) with the MFMA kernel; C is (m, ld(n))."""
+    C = torch.zeros((m, _ld(n)), dtype=torch.float64, device=A.device)
+    L.check(L.lib().rbfx_gemm_f64(m, n, k, float(alpha), L.ptr(A), A.shape[1], L.ptr(B), B.shape[1], 0.0, L.ptr(C),
+                                  C.shape[1], L.stream()))
+    return C
+
+
+def b_matrix(problem, a_lu):
+    """B = (D inv(A))[:, :N] (updes/assembly.py:366-401): D is the first N rows
+    of K, inv(A) one multi-right-hand-side solve with A's factors."""
+    N, n = problem.dc.N, problem.n
+    K = assemble(problem, L.MAT_K, torch.float64)
+    inv_a = lu_inverse(a_lu)
+    return gemm(K, inv_a, N, N, n)[:, :N]
+
+
 class SolveStats:
     def __init__(self):
         self.precision = "fp64"
+        self.basis = None
         self.ir_outer = 0
         self.ir_inner = 0
         self.ir_relres = None
         self.fell_back = False
 
     def as_dict(self):
-        return dict(precision=self.precision, ir_outer=self.ir_outer, ir_inner=self.ir_inner,
+        return dict(precision=self.precision, basis=self.basis, ir_outer=self.ir_outer, ir_inner=self.ir_inner,
                     ir_relres=self.ir_relres, fell_back=self.fell_back)
+
+
+# ---- local cardinal basis (the FP32 preconditioner's change of basis) ----------
+DEFAULT_STENCIL = 20
+DEFAULT_SPECIAL = 3   # a 3 x 3 grid of special points
+
+
+def special_points(X, g):
+    """Node ids nearest to a g x g grid of points spread over the cloud's
+    bounding box (distinct ids, ascending)."""
+    if g <= 0:
+        return np.zeros(0, dtype=np.int64)
+    from scipy.spatial import cKDTree
+    lo, hi = X.min(axis=0), X.max(axis=0)
+    t = (np.arange(g) + 0.5) / g
+    pts = np.array([[lo[0] + a * (hi[0] - lo[0]), lo[1] + b * (hi[1] - lo[1])] for a in t for b in t])
+    _, ids = cKDTree(X).query(pts, 1)
+    return np.unique(np.asarray(ids, dtype=np.int64))
+
+
+def cardinal_stencil(dc, k, g=DEFAULT_SPECIAL):
+    """(N, k + #special) int32 stencils of the local cardinal functions, in
+    ascending id order (sorted ids make the gathers of the formation kernel
+    coalesce): every node's nearest nodes (itself included) and the special
+    points, the nodes nearest a g x g grid over the cloud.  The special points
+    hold each cardinal function near 0 far from its node, where its local
+    polynomial part would otherwise grow quadratically (Beatson, Cherrie and
+    Mouat's approximate cardinal functions for RBF interpolation, 1999).
+    Cached on the device cloud."""
+    cache = dc.__dict__.setdefault("_stencils", {})
+    X = dc.nodes.detach().cpu().numpy()
+    sp = special_points(X, g) if dc.N > int(k) + g * g else np.zeros(0, dtype=np.int64)
+    kt = min(int(k) + sp.size, dc.N)
+    st = cache.get((kt, g))
+    if st is None:
+        from scipy.spatial import cKDTree
+        _, nb = cKDTree(X).query(X, kt)
+        nb = np.asarray(nb, dtype=np.int64).reshape(dc.N, kt)
+        own = np.arange(dc.N)
+        missing = ~(nb == own[:, None]).any(axis=1)
+        nb[missing, -1] = own[missing]  # coincident nodes: keep the node itself
+        if sp.size:
+            # the special points, then the nearest nodes not among them (a
+            # stable sort moves the special entries of each row to its end)
+            order = np.argsort(np.isin(nb, sp), axis=1, kind="stable")
+            extra = np.take_along_axis(nb, order, axis=1)[:, :kt - sp.size]
+            nb = np.concatenate([np.broadcast_to(sp, (dc.N, sp.size)), extra], axis=1)
+        st = np.sort(nb, axis=1).astype(np.int32)
+        cache[(kt, g)] = st
+    return st
+
+
+def reverse_stencil(st):
+    """CSR of the stencil table by node id: ptr (N+1) int64 and idx int32 with
+    idx[ptr[l]:ptr[l+1]] = the entries j*k + t where st[j, t] == l, ascending."""
+    N, k = st.shape
+    flat = st.reshape(-1).astype(np.int64)
+    order = np.argsort(flat, kind="stable")
+    ptr = np.zeros(N + 1, dtype=np.int64)
+    np.cumsum(np.bincount(flat, minlength=N), out=ptr[1:])
+    return ptr, order.astype(np.int32)
+
+
+def monomial_matrix(X, M):
+    from .utils import MONOMIAL_EXPONENTS
+    return np.stack([X[:, 0] ** a * X[:, 1] ** b for a, b in MONOMIAL_EXPONENTS[:M]], axis=1) if M else \
+        np.zeros((X.shape[0], 0))
+
+
+class CardinalBasis:
+    """Local cardinal functions of one (cloud, kernel, degree): stencils,
+    coefficients (GPU kernel), reverse stencil table and (P^T P)^-1."""
+
+    def __init__(self, problem, k=DEFAULT_STENCIL, g=DEFAULT_SPECIAL):
+        dc = problem.dc
+        if dc.local:
+            raise ValueError("the cardinal basis needs every node in every support")
+        self.dc, self.N, self.M = dc, dc.N, problem.M
+        dev = dc.x.device
+        st = cardinal_stencil(dc, k, g)
+        self.k = int(st.shape[1])
+        ptr, idx = reverse_stencil(st)
+        self.stencil = torch.as_tensor(st).to(dev)
+        self.rev_ptr = torch.as_tensor(ptr).to(dev)
+        self.rev_idx = torch.as_tensor(idx).to(dev)
+        self.beta = torch.empty(self.N * self.k, dtype=torch.float64, device=dev)
+        self.alpha = torch.empty(max(self.N * self.M, 1), dtype=torch.float64, device=dev)
+        P = monomial_matrix(dc.nodes.detach().cpu().numpy(), self.M)
+        ptp_inv = np.linalg.inv(P.T @ P) if self.M else np.zeros((1, 1))
+        self.ptp_inv = torch.as_tensor(np.ascontiguousarray(ptp_inv).reshape(-1)).to(dev)
+        self._ws = L.workspace(256, dev)
+        c = self.struct(None)
+        L.check(L.lib().rbfx_cardinal_coeffs(problem.ref(), ctypes.byref(c), L.ptr(self._ws), self._ws.numel(),
+                                             L.stream()))
+
+    def struct(self, rs):
+        c = L.Cardinal()
+        c.N, c.M, c.k = self.N, self.M, self.k
+        c.d_x, c.d_y = self.dc.x.data_ptr(), self.dc.y.data_ptr()
+        c.d_stencil = self.stencil.data_ptr()
+        c.d_beta, c.d_alpha = self.beta.data_ptr(), self.alpha.data_ptr()
+        c.d_rs = rs.data_ptr() if rs is not None else None
+        c.d_rev_ptr, c.d_rev_idx = self.rev_ptr.data_ptr(), self.rev_idx.data_ptr()
+        c.d_ptp_inv = self.ptp_inv.data_ptr()
+        return c
+
+    def form(self, K64, rs):
+        """K~ = diag(rs) D C (N x ld(N), FP32) from the stored FP64 K; fills rs."""
+        N = self.N
+        out = torch.empty((N, _ld(N)), dtype=torch.float32, device=K64.device)
+        c = self.struct(rs)
+        L.check(L.lib().rbfx_cardinal_form(ctypes.byref(c), L.ptr(K64), K64.shape[1], L.ptr(out), out.shape[1],
+                                           L.ptr(self._ws), self._ws.numel(), L.stream()))
+        return out
+
+
+def frobenius(mat, rows, cols, row_scale=None):
+    """Frobenius norm of mat[:rows, :cols], rows scaled by row_scale (a prefix)."""
+    lib = L.lib()
+    ws = L.workspace(lib.rbfx_frobenius_workspace(rows), mat.device)
+    out = ctypes.c_double(0.0)
+    nrs = 0 if row_scale is None else row_scale.numel()
+    L.check(lib.rbfx_frobenius_f64(rows, cols, L.ptr(mat), mat.shape[1], L.ptr(row_scale), nrs, ctypes.byref(out),
+                                   L.ptr(ws), ws.numel(), L.stream()))
+    return out.value
 
 
 class KFactor:
     """Factorisation of the collocation matrix K for one (cloud, rbf, operator).
 
-    precision "fp64": FP64 LU.  precision "fp32_ir": FP32 LU (MFMA) used as
-    the preconditioner of GMRES-IR with matrix-free FP64 residuals; when
-    refinement stalls, an FP64 LU is built and used instead (fell_back).
+    precision "fp64": FP64 LU.  precision "fp32_ir": FP64 K is assembled and
+    kept for the residuals, the N x N matrix K~ = diag(rs) D C of K in the
+    local cardinal basis C is factored in FP32 on MFMA, and flexible GMRES on
+    K preconditioned by those factors refines to FP64 accuracy.  With local
+    supports (support_size < N) the FP32 LU of K itself preconditions, with
+    matrix-free residuals.  When refinement stalls, an FP64 LU is built and
+    used instead (fell_back).
     """
 
-    def __init__(self, problem, precision="fp64", ir_tol=1e-11, ir_restart=120, ir_max_inner=120):
+    def __init__(self, problem, precision="fp64", ir_tol=1e-11, ir_restart=40, ir_max_inner=160,
+                 ir_stencil=DEFAULT_STENCIL, basis=None):
         self.problem = problem
         self.n = problem.n
         self.precision = precision
         self.ir_tol, self.ir_restart, self.ir_max_inner = ir_tol, ir_restart, ir_max_inner
         self.lu64 = None
         self.lu32 = None
+        self.K64 = None
+        self.basis = None
+        self.mode = "fp64"
         self.last_stats = SolveStats()
         if precision == "fp64":
             self.lu64 = LU(assemble(problem, L.MAT_K, torch.float64), self.n)
-        elif precision == "fp32_ir":
+        elif precision == "fp32_ir" and problem.dc.local:
+            self.mode = "lu"
             self.lu32 = LU(assemble(problem, L.MAT_K, torch.float32), self.n)
+        elif precision == "fp32_ir":
+            self.mode = "cardinal"
+            self.K64 = assemble(problem, L.MAT_K, torch.float64)
+            self.basis = basis if basis is not None else CardinalBasis(problem, ir_stencil)
+            N = problem.dc.N
+            self.rs = torch.empty(N, dtype=torch.float64, device=self.K64.device)
+            kt = self.basis.form(self.K64, self.rs)
+            # norms of the row-scaled forward system and of the transposed one
+            self.knorm = (frobenius(self.K64, self.n, self.n, self.rs), frobenius(self.K64, self.n, self.n))
+            self.lu32 = LU(kt, N)
         else:
             raise ValueError("precision must be 'fp64' or 'fp32_ir'")
 
     def _fallback(self):
         if self.lu64 is None:
-            self.lu64 = LU(assemble(self.problem, L.MAT_K, torch.float64), self.n)
+            if self.K64 is not None:
+                # the stored FP64 K is factored in place
+                K64, self.K64 = self.K64, None
+                self.lu64 = LU(K64, self.n)
+            else:
+                self.lu64 = LU(assemble(self.problem, L.MAT_K, torch.float64), self.n)
         return self.lu64
 
     def solve(self, b, trans=False):
+        if b.numel() != self.n:
+            raise ValueError("right-hand side has %d entries, the system is %d x %d" % (b.numel(), self.n, self.n))
         st = SolveStats()
         st.precision = self.precision
+        st.basis = self.mode
         if self.precision == "fp64" or (self.lu64 is not None and self.lu32 is None):
             if self.precision != "fp64":
                 # refinement stalled on an earlier solve with these factors: keep
@@ -348,20 +533,32 @@ class KFactor:
         n = self.n
         b = b.detach().to(torch.float64).contiguous()
         x = torch.empty_like(b)
-        ws = L.workspace(max(lib.rbfx_gmres_ir_workspace(n, self.ir_restart),
-                             lib.rbfx_matvec_workspace(self.problem.ref(), int(trans))), b.device)
         stats = L.IrStats()
         lu = self.lu32
-        status = lib.rbfx_gmres_ir(int(trans), n, self.problem.ref(), None, 0, L.ptr(lu.a), lu.ld, L.ptr(lu.perm),
-                                   L.ptr(lu.dinv), L.ptr(b), L.ptr(x), float(self.ir_tol), int(self.ir_restart),
-                                   int(self.ir_max_inner), L.ptr(ws), ws.numel(), ctypes.byref(stats), L.stream())
+        if self.mode == "cardinal":
+            N, M = self.basis.N, self.basis.M
+            ws = L.workspace(lib.rbfx_gmres_ir_cardinal_workspace(N, M, self.ir_restart), b.device)
+            c = self.basis.struct(self.rs)
+            status = lib.rbfx_gmres_ir_cardinal(int(trans), ctypes.byref(c), L.ptr(self.K64), self.K64.shape[1],
+                                                float(self.knorm[1 if trans else 0]), L.ptr(lu.a), lu.ld,
+                                                L.ptr(lu.perm),
+                                                L.ptr(lu.dinv), L.ptr(b), L.ptr(x), float(self.ir_tol),
+                                                int(self.ir_restart), int(self.ir_max_inner), L.ptr(ws),
+                                                ws.numel(), ctypes.byref(stats), L.stream())
+        else:
+            ws = L.workspace(max(lib.rbfx_gmres_ir_workspace(n, self.ir_restart),
+                                 lib.rbfx_matvec_workspace(self.problem.ref(), int(trans))), b.device)
+            status = lib.rbfx_gmres_ir(int(trans), n, self.problem.ref(), None, 0, L.ptr(lu.a), lu.ld,
+                                       L.ptr(lu.perm), L.ptr(lu.dinv), L.ptr(b), L.ptr(x), float(self.ir_tol),
+                                       int(self.ir_restart), int(self.ir_max_inner), L.ptr(ws), ws.numel(),
+                                       ctypes.byref(stats), L.stream())
         st.ir_outer, st.ir_inner, st.ir_relres = stats.outer_iters, stats.inner_iters, stats.final_relres
         if status == L.RBFX_ERR_STALL:
             # later solves with this factorisation (e.g. the adjoint) go straight
-            # to the FP64 factors; the FP32 copy is released
+            # to the FP64 factors; the FP32 factors are released
             st.fell_back = True
-            lu64 = self._fallback()
             self.lu32 = None
+            lu64 = self._fallback()
             return lu64.solve(b, trans)
         L.check(status)
         return x
@@ -397,6 +594,7 @@ def clear_cache():
     """Drop cached factorisations (they can hold many GB at large N)."""
     _K_CACHE.clear()
     _A_CACHE.clear()
+    _BASIS_CACHE.clear()
     torch.cuda.empty_cache()
 
 
@@ -413,6 +611,21 @@ def a_factor(cloud, kind, param, M):
     return f
 
 
+_BASIS_CACHE = _LRU(2)
+
+
+def cardinal_basis(cloud, kind, param, M, k=DEFAULT_STENCIL):
+    """Cached local cardinal basis of (cloud, kernel, degree): it does not
+    depend on the operator or the boundary data."""
+    key = (cloud_key(cloud), kind, param, M, int(k))
+    bas = _BASIS_CACHE.get(key) if CACHE_ENABLED[0] else None
+    if bas is None:
+        bas = CardinalBasis(Problem(device_cloud(cloud), kind, param, M), k)
+        if CACHE_ENABLED[0]:
+            _BASIS_CACHE.put(key, bas)
+    return bas
+
+
 def k_factor(cloud, kind, param, M, desc, robin_beta, precision, **ir):
     rkey = None if robin_beta is None else hashlib.sha1(
         torch.as_tensor(robin_beta, dtype=torch.float64).cpu().numpy().tobytes()).hexdigest()
@@ -420,7 +633,10 @@ def k_factor(cloud, kind, param, M, desc, robin_beta, precision, **ir):
     f = _K_CACHE.get(key) if CACHE_ENABLED[0] else None
     if f is None:
         prob = Problem(device_cloud(cloud), kind, param, M, desc, robin_beta)
-        f = KFactor(prob, precision, **ir)
+        opts = dict(ir)
+        if precision == "fp32_ir" and not prob.dc.local:
+            opts["basis"] = cardinal_basis(cloud, kind, param, M, opts.get("ir_stencil", DEFAULT_STENCIL))
+        f = KFactor(prob, precision, **opts)
         if CACHE_ENABLED[0]:
             _K_CACHE.put(key, f)
     return f

@@ -432,11 +432,9 @@ def _robin_beta_vector(cloud, robin_coeffs):
 # ---- solver (operators.py:559-683) --------------------------------------------------
 def _mat_thunk(cloud, kind, param, M, fac, dc):
     def build():
-        # B = (D inv(A))[:, :N]: row i of B solves A^T y = D_i^T
-        Dfull = E.assemble(fac.problem, L.MAT_K, torch.float64)[:dc.N, :fac.n]
-        fa = E.a_factor(cloud, kind, param, M)
-        rows = [fa.solve(Dfull[i].contiguous(), trans=True)[:dc.N] for i in range(dc.N)]
-        return torch.stack(rows)
+        # B = (D inv(A))[:, :N] (updes/assembly.py:399): inv(A) by one
+        # multi-right-hand-side solve, then one MFMA GEMM
+        return E.b_matrix(fac.problem, E.a_factor(cloud, kind, param, M))
     return build
 
 
