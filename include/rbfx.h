@@ -258,6 +258,14 @@ typedef struct rbfx_cardinal {
   const int32_t* d_rev_idx;  /*   d_rev_idx[d_rev_ptr[l] .. d_rev_ptr[l+1]) = j*k + t with
                                   d_stencil[j*k+t] == l, j ascending (rbfx_gmres_ir_cardinal) */
   const double* d_ptp_inv;   /* (M*M) inverse of P^T P, row-major (rbfx_gmres_ir_cardinal)   */
+  /* Optional (all three or NULL) union tables of rbfx_cardinal_form: for each
+   * block cb of 256 consecutive nodes, the sorted distinct ids of its stencils
+   * d_union_idx[d_union_ptr[cb] .. d_union_ptr[cb+1]), and for each stencil
+   * entry j*k+t the position of its id in its block's list.  With them the K
+   * rows are staged through LDS; without them gathered from global memory. */
+  const int64_t* d_union_ptr; /* (ceil(N/256)+1)                                             */
+  const int32_t* d_union_idx;
+  const int16_t* d_union_pos; /* (N*k)                                                       */
 } rbfx_cardinal;
 
 /* Local cardinal coefficients (beta, alpha) of every node.  Synchronises;

@@ -44,11 +44,17 @@ def test_cardinal_coefficients_match_numpy(kind, param, M):
     assert (np.abs(alpha - alpha_ref) / sa).max() <= 1e-8
 
 
-@pytest.mark.parametrize("kind,param", [(R.POLYHARMONIC, 1.0), (R.GAUSSIAN, 5.0), (R.THIN_PLATE, 1.0)])
-def test_cardinal_form_matches_numpy(kind, param):
+@pytest.mark.parametrize("kind,param,nx,union", [(R.POLYHARMONIC, 1.0, 12, True), (R.GAUSSIAN, 5.0, 12, True),
+                                                  (R.THIN_PLATE, 1.0, 12, True), (R.POLYHARMONIC, 1.0, 12, False),
+                                                  (R.POLYHARMONIC, 1.0, 40, True)])
+def test_cardinal_form_matches_numpy(kind, param, nx, union):
+    """Both formation kernels: K rows staged in LDS over the per-block stencil
+    unions (several 256-column blocks and row ranges at 40 x 40), and gathered
+    from global memory."""
     from updes_amd import engine as E, _lib as L
     M = 6
-    oc, uc, prob, basis = _setup(kind, param, M, facets=C.FACET_SETS[2])
+    oc, uc, prob, basis = _setup(kind, param, M, nx=nx, ny=nx - 1, facets=C.FACET_SETS[2])
+    basis.use_union = union
     K64 = E.assemble(prob, L.MAT_K, torch.float64)
     rs = torch.empty(oc.N, dtype=torch.float64, device="cuda")
     Kt = basis.form(K64, rs)[:, :oc.N].cpu().numpy()
@@ -82,17 +88,22 @@ def test_gmres_ir_cardinal_matches_fp64_lu(trans):
     assert not st["fell_back"] and st["basis"] == "cardinal", st
     assert st["ir_inner"] <= 40, st
     if not trans:
+        # nodal values within the 1e-8 bar of the FP64 LU's (random, rough
+        # right-hand side: the worst case for the forward error of two
+        # backward-stable solves)
         A = OA.assemble_A(oc, R.POLYHARMONIC, 1.0, M)
         u, u_ref = A[:oc.N] @ x, A[:oc.N] @ ref
-        assert np.linalg.norm(u - u_ref) <= 1e-9 * np.linalg.norm(u_ref), st
+        e = np.linalg.norm(u - u_ref) / np.linalg.norm(u_ref)
     else:
         # the adjoint of q (the first N entries) is what gradients use
         e = np.linalg.norm(x[:oc.N] - ref[:oc.N]) / np.linalg.norm(ref[:oc.N])
-        assert e <= 1e-7, (e, st)
-    # backward error of the FP64 residual
+    print("trans %d: rel err vs FP64 LU %.2e, stats %s" % (trans, e, st))
+    assert e <= 1e-8, (e, st)
+    # backward stable: normwise backward error of the FP64 residual at the
+    # refinement's stopping level (2^-54)
     r = b - (K.T @ x if trans else K @ x)
     eta = np.linalg.norm(r) / (np.linalg.norm(K) * np.linalg.norm(x) + np.linalg.norm(b))
-    assert eta <= 1e-14, eta
+    assert eta <= 2.0 ** -53, eta
 
 
 def test_cardinal_basis_rejects_bad_stencils():

@@ -60,7 +60,7 @@ class Cardinal(ctypes.Structure):
     """Mirror of ``rbfx_cardinal``."""
     _fields_ = [("N", _i64), ("M", _i64), ("k", _i64), ("d_x", _p), ("d_y", _p), ("d_stencil", _p),
                 ("d_beta", _p), ("d_alpha", _p), ("d_rs", _p), ("d_rev_ptr", _p), ("d_rev_idx", _p),
-                ("d_ptp_inv", _p)]
+                ("d_ptp_inv", _p), ("d_union_ptr", _p), ("d_union_idx", _p), ("d_union_pos", _p)]
 
 
 class ProfEntry(ctypes.Structure):

@@ -34,7 +34,11 @@ namespace rbfx {
 constexpr int kCardMaxK = 48;
 constexpr int kCardMaxM = 15;
 constexpr int kCardPitch = 65;  // LDS row pitch (doubles) of the local system
-constexpr int kFormRows = 16;   // rows of K~ per workgroup of the formation kernel
+constexpr int kFormRows = 16;   // rows of K~ per workgroup of the gather formation kernel
+constexpr int kFormCols = 256;  // columns per block of the union tables (rbfx_cardinal)
+constexpr int kFormTR = 4;      // rows staged in LDS per pass of the LDS formation kernel
+constexpr int kFormUMax = 2048; // union entries of one block staged in LDS
+constexpr int kFormRB = 512;    // rows per workgroup of the LDS formation kernel
 constexpr int kRedBlocks = 128; // partial sums of the small reductions
 
 __device__ inline double wave_sum_d(double v) {
@@ -286,6 +290,104 @@ __global__ __launch_bounds__(256) void cardinal_form_kernel(FormArgs f, int64_t 
   }
 }
 
+// K~ = diag(rs) D C in FP32 with the K entries staged in LDS.  Workgroup
+// (rb, cb) owns the 256 columns of block cb and the kFormRB rows of range rb:
+// each thread keeps its column's coefficients and the positions of its
+// stencil in the block's union U (sorted distinct stencil ids of the block)
+// in registers, and every pass stages K[rows, U] for kFormTR rows in LDS
+// (U is mostly contiguous runs, so the loads coalesce) and forms those rows
+// from LDS.  A block whose union exceeds kFormUMax gathers from global
+// memory instead.  Linear block b runs on XCD b % 8: b -> row range
+// (b % 8) + 8 * (b / 8 / ncb), column block (b / 8) % ncb, so the workgroups
+// of one XCD share the K rows they read in its L2.
+template <int KMAX>
+__global__ __launch_bounds__(256) void cardinal_form_lds_kernel(FormArgs f, const int64_t* __restrict__ uptr,
+                                                                const int32_t* __restrict__ uidx,
+                                                                const int16_t* __restrict__ upos, int64_t ncb,
+                                                                int64_t nrb) {
+  __shared__ double Ks[kFormTR][kFormUMax];
+  __shared__ double Kp[kFormTR][kCardMaxM];
+  constexpr int kUPerThread = kFormUMax / 256;
+  const int64_t b = blockIdx.x;
+  const int64_t q = b >> 3;
+  const int64_t rb = (q / ncb) * 8 + (b & 7);
+  const int64_t cb = q % ncb;
+  if (rb >= nrb) return;
+  const int tid = threadIdx.x;
+  const int64_t u0 = uptr[cb];
+  const int64_t nu = uptr[cb + 1] - u0;
+  const bool lds = nu >= 0 && nu <= kFormUMax;  // uniform over the workgroup
+  const int64_t j = cb * kFormCols + tid;
+  const bool act = j < f.N;
+  int32_t pj[KMAX];
+  double bj[KMAX];
+  double aj[kCardMaxM];
+#pragma unroll
+  for (int t = 0; t < KMAX; ++t) {
+    const bool ok = act && t < f.k;
+    int32_t p = 0;
+    if (ok) {
+      if (lds) {
+        p = upos[j * f.k + t];
+        p = (p >= 0 && p < nu) ? p : 0;
+      } else {
+        p = (int32_t)checked_id(f.stencil[j * f.k + t], f.N, j, f.status);
+      }
+    }
+    pj[t] = p;
+    bj[t] = ok ? f.beta[j * f.k + t] : 0.0;
+  }
+#pragma unroll
+  for (int m = 0; m < kCardMaxM; ++m) aj[m] = (act && m < f.M) ? f.alpha[j * f.M + m] : 0.0;
+  // this thread's share of the union ids (staging loads)
+  int32_t myu[kUPerThread];
+#pragma unroll
+  for (int e = 0; e < kUPerThread; ++e) {
+    const int64_t u = tid + (int64_t)e * 256;
+    myu[e] = (lds && u < nu) ? (int32_t)checked_id(uidx[u0 + u], f.N, 0, f.status) : 0;
+  }
+  const int64_t rbeg = rb * kFormRB;
+  const int64_t rend = rbeg + kFormRB < f.N ? rbeg + kFormRB : f.N;
+  for (int64_t r0 = rbeg; r0 < rend; r0 += kFormTR) {
+    const int nr = (int)(rend - r0 < kFormTR ? rend - r0 : kFormTR);
+    if (lds) {
+      __syncthreads();  // the previous pass is done with Ks
+      for (int rr = 0; rr < nr; ++rr) {
+        const double* Ki = f.K + (r0 + rr) * f.ld;
+#pragma unroll
+        for (int e = 0; e < kUPerThread; ++e) {
+          const int64_t u = tid + (int64_t)e * 256;
+          if (u < nu) Ks[rr][u] = Ki[myu[e]];
+        }
+        if (tid < f.M) Kp[rr][tid] = Ki[f.N + tid];
+      }
+      __syncthreads();
+      for (int rr = 0; rr < nr; ++rr) {
+        double acc = 0.0;
+#pragma unroll
+        for (int t = 0; t < KMAX; ++t)
+          if (t < f.k) acc = __builtin_fma(Ks[rr][pj[t]], bj[t], acc);
+#pragma unroll
+        for (int m = 0; m < kCardMaxM; ++m)
+          if (m < f.M) acc = __builtin_fma(Kp[rr][m], aj[m], acc);
+        if (act) f.out[(r0 + rr) * f.ld32 + j] = (float)(f.rs[r0 + rr] * acc);
+      }
+    } else {
+      for (int rr = 0; rr < nr; ++rr) {
+        const double* Ki = f.K + (r0 + rr) * f.ld;
+        double acc = 0.0;
+#pragma unroll
+        for (int t = 0; t < KMAX; ++t)
+          if (t < f.k) acc = __builtin_fma(Ki[pj[t]], bj[t], acc);
+#pragma unroll
+        for (int m = 0; m < kCardMaxM; ++m)
+          if (m < f.M) acc = __builtin_fma(Ki[f.N + m], aj[m], acc);
+        if (act) f.out[(r0 + rr) * f.ld32 + j] = (float)(f.rs[r0 + rr] * acc);
+      }
+    }
+  }
+}
+
 // ---- the preconditioner's vector kernels ----------------------------------
 struct VecArgs {
   int64_t N, M, k;
@@ -309,6 +411,11 @@ __global__ void card_scale_kernel(int64_t N, int64_t n, const double* __restrict
     out[i] = v[i];
 }
 
+__global__ void card_copy_kernel(int64_t n, const double* __restrict__ v, double* __restrict__ t) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) t[i] = v[i];
+}
+
 // z[i] = rs[i] y[i] (i < N), z[i] = 0 (N <= i < n)
 __global__ void card_scale_pad_kernel(int64_t N, int64_t n, const double* __restrict__ rs,
                                       const double* __restrict__ yv, double* __restrict__ z) {
@@ -320,24 +427,71 @@ __global__ void card_scale_pad_kernel(int64_t N, int64_t n, const double* __rest
 }
 
 // z[l] = (C_beta y)[l] + (P s)[l] for l < N, over the reverse stencil table
-// (the occurrences of node l in the stencils, ascending j)
-__global__ void card_cmap_kernel(VecArgs a, const double* __restrict__ yv, const double* __restrict__ sP,
-                                 double* __restrict__ z) {
+// (the occurrences of node l in the stencils, ascending j).  A thread per
+// node; a node that occurs in more than kHeavyRev stencils (the special
+// points occur in all of them) is summed by its whole workgroup afterwards,
+// in a fixed order, so that no thread walks N entries alone.
+constexpr int kHeavyRev = 256;
+__global__ __launch_bounds__(256) void card_cmap_kernel(VecArgs a, const double* __restrict__ yv,
+                                                        const double* __restrict__ sP, double* __restrict__ z) {
+  __shared__ double red[4];
+  __shared__ int s_heavy[256];
+  __shared__ int s_nheavy;
   const int64_t l = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (l >= a.N) return;
   const int64_t nk = a.N * a.k;
-  int64_t e0 = a.rev_ptr[l], e1 = a.rev_ptr[l + 1];
-  e0 = e0 < 0 ? 0 : (e0 > nk ? nk : e0);
-  e1 = e1 < e0 ? e0 : (e1 > nk ? nk : e1);
-  double acc = 0.0;
-  for (int64_t e = e0; e < e1; ++e) {
-    int64_t idx = a.rev_idx[e];
-    idx = idx < 0 ? 0 : (idx >= nk ? nk - 1 : idx);
-    acc = __builtin_fma(a.beta[idx], yv[idx / a.k], acc);
+  const int kk = (int)a.k;
+  auto range = [&](int64_t node, int64_t& e0, int64_t& e1) {
+    e0 = a.rev_ptr[node];
+    e1 = a.rev_ptr[node + 1];
+    e0 = e0 < 0 ? 0 : (e0 > nk ? nk : e0);
+    e1 = e1 < e0 ? e0 : (e1 > nk ? nk : e1);
+  };
+  auto term = [&](int64_t e) {
+    int idx = a.rev_idx[e];
+    idx = idx < 0 ? 0 : (idx >= nk ? (int)nk - 1 : idx);
+    return a.beta[idx] * yv[idx / kk];
+  };
+  auto poly = [&](int64_t node) {
+    const double xl = a.x[node], yl = a.y[node];
+    double acc = 0.0;
+    for (int m = 0; m < a.M; ++m) acc = __builtin_fma(mono_val(m, xl, yl), sP[m], acc);
+    return acc;
+  };
+  if (threadIdx.x == 0) s_nheavy = 0;
+  __syncthreads();
+  if (l < a.N) {
+    int64_t e0, e1;
+    range(l, e0, e1);
+    if (e1 - e0 > kHeavyRev) {
+      s_heavy[atomicAdd(&s_nheavy, 1)] = threadIdx.x;
+    } else {
+      double acc = 0.0;
+      for (int64_t e = e0; e < e1; ++e) acc += term(e);
+      z[l] = acc + poly(l);
+    }
   }
-  const double xl = a.x[l], yl = a.y[l];
-  for (int m = 0; m < a.M; ++m) acc = __builtin_fma(mono_val(m, xl, yl), sP[m], acc);
-  z[l] = acc;
+  __syncthreads();
+  const int nh = s_nheavy;
+  for (int h = 0; h < nh; ++h) {
+    // the heavy nodes of this block in thread order (fixed summation order)
+    int th = 256;
+    for (int q = 0; q < nh; ++q) {
+      const int c = s_heavy[q];
+      int rank = 0;
+      for (int r = 0; r < nh; ++r) rank += s_heavy[r] < c;
+      if (rank == h) th = c;
+    }
+    const int64_t node = (int64_t)blockIdx.x * blockDim.x + th;
+    int64_t e0, e1;
+    range(node, e0, e1);
+    double acc = 0.0;
+    for (int64_t e = e0 + threadIdx.x; e < e1; e += 256) acc += term(e);
+    for (int off = 32; off > 0; off >>= 1) acc += __shfl_down(acc, off, 64);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0) z[node] = (red[0] + red[1]) + (red[2] + red[3]) + poly(node);
+    __syncthreads();
+  }
 }
 
 // t[j] = (C_beta^T v[:N])[j] + (C_alpha^T v[N:])[j]
@@ -510,7 +664,8 @@ struct CardPair : PairOp {
     if (!trans) {
       // v is a residual of the row-scaled system: diag(rs) v_D is already the
       // right-hand side of K~ (v[:N] itself)
-      RBFX_HIP(hipMemcpyAsync(t, v, a.N * sizeof(double), hipMemcpyDeviceToDevice, s));
+      hipLaunchKernelGGL(card_copy_kernel, dim3(gN), dim3(256), 0, s, a.N, v, t);
+      RBFX_CHECK_LAUNCH();
       if ((st = solve32(0))) return st;
       if (a.M > 0) {
         hipLaunchKernelGGL(card_reduce_final_kernel, dim3(1), dim3(64), 0, s, a.M, (int64_t)1, v + a.N, ptp_inv,
@@ -589,21 +744,40 @@ extern "C" int rbfx_cardinal_form(const rbfx_cardinal* c, const double* d_K64, i
   FormArgs f{c->N, c->M, c->k, d_K64, ld64, c->d_stencil, c->d_beta, c->d_alpha, c->d_rs, d_out32, ld32, status};
   hipLaunchKernelGGL(cardinal_rowscale_kernel, dim3((unsigned)ceil_div(c->N, 4)), dim3(256), 0, s, f);
   RBFX_CHECK_LAUNCH();
-  const int64_t ncb = ceil_div(c->N, 256), nrt = ceil_div(c->N, kFormRows);
-  const int64_t blocks = ceil_div(nrt, 8) * 8 * ncb;
-  RBFX_REQUIRE(blocks < (int64_t)UINT32_MAX, "cardinal_form grid too large");
+  const int64_t ncb = ceil_div(c->N, 256);
   const int pid = prof_start("cardinal_form", s);
+  if (c->d_union_ptr && c->d_union_idx && c->d_union_pos) {
+    const int64_t nrb = ceil_div(c->N, kFormRB);
+    const int64_t blocks = ceil_div(nrb, 8) * 8 * ncb;
+    RBFX_REQUIRE(blocks < (int64_t)UINT32_MAX, "cardinal_form grid too large");
+#define RBFX_FORM(KM)                                                                                         \
+  hipLaunchKernelGGL((cardinal_form_lds_kernel<KM>), dim3((unsigned)blocks), dim3(256), 0, s, f, c->d_union_ptr, \
+                     c->d_union_idx, c->d_union_pos, ncb, nrb)
+    if (c->k <= 16)
+      RBFX_FORM(16);
+    else if (c->k <= 24)
+      RBFX_FORM(24);
+    else if (c->k <= 32)
+      RBFX_FORM(32);
+    else
+      RBFX_FORM(48);
+#undef RBFX_FORM
+  } else {
+    const int64_t nrt = ceil_div(c->N, kFormRows);
+    const int64_t blocks = ceil_div(nrt, 8) * 8 * ncb;
+    RBFX_REQUIRE(blocks < (int64_t)UINT32_MAX, "cardinal_form grid too large");
 #define RBFX_FORM(KM) \
   hipLaunchKernelGGL((cardinal_form_kernel<KM>), dim3((unsigned)blocks), dim3(256), 0, s, f, ncb, nrt)
-  if (c->k <= 16)
-    RBFX_FORM(16);
-  else if (c->k <= 24)
-    RBFX_FORM(24);
-  else if (c->k <= 32)
-    RBFX_FORM(32);
-  else
-    RBFX_FORM(48);
+    if (c->k <= 16)
+      RBFX_FORM(16);
+    else if (c->k <= 24)
+      RBFX_FORM(24);
+    else if (c->k <= 32)
+      RBFX_FORM(32);
+    else
+      RBFX_FORM(48);
 #undef RBFX_FORM
+  }
   RBFX_CHECK_LAUNCH();
   // algorithmic bytes: K read once (8 n N) and K~ written (4 N^2)
   prof_stop(pid, s, 2.0 * (double)(c->k + c->M) * c->N * c->N,

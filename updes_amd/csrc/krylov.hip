@@ -520,7 +520,8 @@ struct LuPair : PairOp {
   ResidualOp* K;
   hipStream_t s;
   int apply(const double* v, double* z, double* w) override {
-    RBFX_HIP(hipMemcpyAsync(z, v, n * sizeof(double), hipMemcpyDeviceToDevice, s));
+    hipLaunchKernelGGL(scale_copy_kernel, dim3((unsigned)ceil_div(n, 256)), dim3(256), 0, s, n, 1.0, v, z);
+    RBFX_CHECK_LAUNCH();
     int st = rbfx_getrs_f32(trans, n, LU32, ld32, perm, dinv, z, ts_work, ts_bytes, s);
     if (st) return st;
     if ((st = rbfx_getrs_status(n, ts_work, ts_bytes, s))) return st;

@@ -103,7 +103,9 @@ def cloud_key(cloud):
 
 
 def device_cloud(cloud):
-    dev = torch.device("cuda")
+    """The cloud's DeviceCloud on the current device, built once and cached on
+    the cloud (with its stencil and support tables)."""
+    dev = torch.device("cuda", torch.cuda.current_device())
     dc = cloud.__dict__.get("_rbfx_device_cloud")
     if dc is None or dc.nodes.device != dev or dc.nodes.data_ptr() == 0:
         dc = DeviceCloud(cloud, dev)
@@ -333,7 +335,7 @@ class SolveStats:
 
 
 # ---- local cardinal basis (the FP32 preconditioner's change of basis) ----------
-DEFAULT_STENCIL = 20
+DEFAULT_STENCIL = 15
 DEFAULT_SPECIAL = 3   # a 3 x 3 grid of special points
 
 
@@ -393,10 +395,52 @@ def reverse_stencil(st):
     return ptr, order.astype(np.int32)
 
 
+FORM_COLS = 256   # columns per block of the union tables of the formation kernel
+
+
+def union_tables(st, cols=FORM_COLS):
+    """For each block of `cols` consecutive nodes: the sorted distinct ids of
+    their stencils (CSR ptr int64 / idx int32) and, per stencil entry, its
+    position in its block's list (int16; 0 where a list exceeds int16)."""
+    N, kt = st.shape
+    ptr = [0]
+    idx = []
+    pos = np.zeros((N, kt), dtype=np.int16)
+    for j0 in range(0, N, cols):
+        blk = st[j0:j0 + cols]
+        U = np.unique(blk)
+        if U.size <= np.iinfo(np.int16).max:
+            pos[j0:j0 + cols] = np.searchsorted(U, blk).astype(np.int16)
+        idx.append(U.astype(np.int32))
+        ptr.append(ptr[-1] + U.size)
+    return np.asarray(ptr, dtype=np.int64), np.concatenate(idx), pos
+
+
 def monomial_matrix(X, M):
     from .utils import MONOMIAL_EXPONENTS
     return np.stack([X[:, 0] ** a * X[:, 1] ** b for a, b in MONOMIAL_EXPONENTS[:M]], axis=1) if M else \
         np.zeros((X.shape[0], 0))
+
+
+def cloud_tables(dc, k, g, M):
+    """Device copies of the stencil table, its reverse table and (P^T P)^-1:
+    they depend on the cloud alone (and on the degree), so they are built once
+    per device cloud, like its support tables."""
+    cache = dc.__dict__.setdefault("_card_tables", {})
+    key = (int(k), int(g), int(M))
+    t = cache.get(key)
+    if t is None:
+        dev = dc.x.device
+        st = cardinal_stencil(dc, k, g)
+        ptr, idx = reverse_stencil(st)
+        P = monomial_matrix(dc.nodes.detach().cpu().numpy(), M)
+        ptp_inv = np.linalg.inv(P.T @ P) if M else np.zeros((1, 1))
+        uptr, uidx, upos = union_tables(st)
+        t = (torch.as_tensor(st).to(dev), torch.as_tensor(ptr).to(dev), torch.as_tensor(idx).to(dev),
+             torch.as_tensor(np.ascontiguousarray(ptp_inv).reshape(-1)).to(dev),
+             (torch.as_tensor(uptr).to(dev), torch.as_tensor(uidx).to(dev), torch.as_tensor(upos).to(dev)))
+        cache[key] = t
+    return t
 
 
 class CardinalBasis:
@@ -409,17 +453,11 @@ class CardinalBasis:
             raise ValueError("the cardinal basis needs every node in every support")
         self.dc, self.N, self.M = dc, dc.N, problem.M
         dev = dc.x.device
-        st = cardinal_stencil(dc, k, g)
-        self.k = int(st.shape[1])
-        ptr, idx = reverse_stencil(st)
-        self.stencil = torch.as_tensor(st).to(dev)
-        self.rev_ptr = torch.as_tensor(ptr).to(dev)
-        self.rev_idx = torch.as_tensor(idx).to(dev)
+        self.stencil, self.rev_ptr, self.rev_idx, self.ptp_inv, self.union = cloud_tables(dc, k, g, self.M)
+        self.use_union = True
+        self.k = int(self.stencil.shape[1])
         self.beta = torch.empty(self.N * self.k, dtype=torch.float64, device=dev)
         self.alpha = torch.empty(max(self.N * self.M, 1), dtype=torch.float64, device=dev)
-        P = monomial_matrix(dc.nodes.detach().cpu().numpy(), self.M)
-        ptp_inv = np.linalg.inv(P.T @ P) if self.M else np.zeros((1, 1))
-        self.ptp_inv = torch.as_tensor(np.ascontiguousarray(ptp_inv).reshape(-1)).to(dev)
         self._ws = L.workspace(256, dev)
         c = self.struct(None)
         L.check(L.lib().rbfx_cardinal_coeffs(problem.ref(), ctypes.byref(c), L.ptr(self._ws), self._ws.numel(),
@@ -434,6 +472,8 @@ class CardinalBasis:
         c.d_rs = rs.data_ptr() if rs is not None else None
         c.d_rev_ptr, c.d_rev_idx = self.rev_ptr.data_ptr(), self.rev_idx.data_ptr()
         c.d_ptp_inv = self.ptp_inv.data_ptr()
+        if self.use_union:
+            c.d_union_ptr, c.d_union_idx, c.d_union_pos = (t.data_ptr() for t in self.union)
         return c
 
     def form(self, K64, rs):
@@ -469,7 +509,7 @@ class KFactor:
     used instead (fell_back).
     """
 
-    def __init__(self, problem, precision="fp64", ir_tol=1e-11, ir_restart=40, ir_max_inner=160,
+    def __init__(self, problem, precision="fp64", ir_tol=1e-12, ir_restart=40, ir_max_inner=160,
                  ir_stencil=DEFAULT_STENCIL, basis=None):
         self.problem = problem
         self.n = problem.n
