@@ -167,9 +167,11 @@ int rbfx_getrf_f32(int64_t n, float* d_A, int64_t lda, int64_t* d_ipiv,
                    void* d_work, size_t work_bytes, int32_t* d_info, void* stream);
 
 /* Prepares the solve data of a factorisation: d_perm (n, int64) the row
- * permutation (row i of P A is row perm[i] of A) and d_dinv (the inverses of
- * the 64x64 diagonal blocks of L and U, rbfx_getrs_dinv_elems(n) elements).
- * Synchronises the stream (copies ipiv to the host once).                   */
+ * permutation (row i of P A is row perm[i] of A) and d_dinv
+ * (rbfx_getrs_dinv_elems(n) doubles: the FP64 inverses of the 64 x 64
+ * diagonal blocks of L and U, then those of the 512 x 512 diagonal blocks
+ * that the solves apply).  Synchronises the stream (copies ipiv to the host
+ * once).                                                                    */
 int64_t rbfx_getrs_dinv_elems(int64_t n);
 int rbfx_getrs_prepare_f64(int64_t n, const double* d_LU, int64_t lda,
                            const int64_t* d_ipiv, int64_t* d_perm, double* d_dinv,

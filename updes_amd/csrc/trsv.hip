@@ -21,7 +21,6 @@ int gemm_f64(int64_t m, int64_t n, int64_t k, double alpha, const double* A, int
              int64_t ldb, double beta, double* C, int64_t ldc, hipStream_t s);
 
 constexpr int TB = 64;  // block size of the sweep and of the diagonal inverses
-constexpr int kTsThreads = 256;
 
 __host__ __device__ inline int64_t imin64(int64_t a, int64_t b) { return a < b ? a : b; }
 
@@ -90,270 +89,427 @@ __global__ void scatter_perm_kernel(const double* __restrict__ y, const int64_t*
   if (i < n) b[perm ? perm[i] : i] = failed ? __builtin_nan("") : y[i];
 }
 
-// ---- the sweep ---------------------------------------------------------------
-// FWD: block rows processed 0..nb-1, else nb-1..0.
-// TRANS=false: the tile (I,J) of the factor itself is used (row-contiguous);
-// TRANS=true : the tile (J,I) transposed.
-// WHICH: which diagonal inverse (0 = L, 1 = U) is applied.
-// Factors are read in their storage type TF; the arithmetic is FP64, so FP32
-// factors are applied exactly as stored (the FP32 preconditioner of GMRES-IR).
-template <typename TF, bool FWD, bool TRANS, int WHICH>
-__global__ __launch_bounds__(kTsThreads) void sweep_kernel(const TF* __restrict__ F, int64_t lda, int64_t n,
-                                                           const double* __restrict__ dinv,
-                                                           double* __restrict__ y, unsigned* __restrict__ ctl,
-                                                           uint64_t wait_ticks) {
-  typedef double T;
-  __shared__ T acc[TB];
-  __shared__ T part[4][TB];
-  __shared__ T Ds[TB][TB + 1];
-  __shared__ T red[TRANS ? 1 : TB][TB + 1];  // no-transpose partial sums, row x column
-  __shared__ int s_I;
-  __shared__ unsigned s_avail;
+// ---- superblock sweep ---------------------------------------------------------
+// The chain of a blocked triangular solve is what bounds it at these sizes:
+// every exchange between workgroups is a coherent store, a counter and a
+// poll through the memory side of the chip (~2 us each).  The superblock
+// sweep makes the chain n/SB steps instead of n/TB: the solution advances by
+// SB = 512 rows at a time, each superblock's rows solved with its 512 x 512
+// diagonal inverse (built once per factorisation by sbinv_kernel), by NCH
+// workgroups of CH rows:
+//   1. fold the tiles of the superblocks already solved, in solve order
+//      (each superblock counts its solved chunks), into private partial sums;
+//   2. publish the chunk's CH accumulated values b - sum, count them in;
+//   3. once all NCH chunks of the superblock are in, apply rows of its
+//      diagonal inverse to the superblock's SB values and publish the CH
+//      solution entries.
+// Tickets are taken in start order and each superblock has NCH consecutive
+// tickets, so the peers a workgroup waits for in step 3 have all started
+// before any workgroup of a later superblock: the waits always end.
+constexpr int SB = 512;
+constexpr int CH = 32;
+constexpr int NCH = SB / CH;
+constexpr int kCtlLine = 32;  // words per superblock control line
+
+// 512 x 512 inverse of a diagonal superblock of L (unit lower, which = 0) or
+// U (which = 1) in FP64, from the 64 x 64 inverses D_i of its sub-blocks:
+//   lower:  X_ii = D_i,  X_ij = -D_i sum_{j<=k<i} L_ik X_kj   (i > j)
+//   upper:  X_ii = D_i,  X_ij = -D_i sum_{i<k<=j} U_ik X_kj   (i < j)
+// One workgroup per (superblock, triangle); 64 x 64 tile products through
+// LDS, each thread a 4 x 4 block of the tile.  Rows and columns past n are
+// identity.
+template <typename TF>
+__global__ __launch_bounds__(256) void sbinv_kernel(const TF* __restrict__ LU, int64_t lda, int64_t n,
+                                                    const double* __restrict__ dinv64, double* __restrict__ dinv512) {
+  __shared__ double As[TB][TB + 1];
+  __shared__ double Bs[TB][TB + 1];
+  const int64_t K = blockIdx.x >> 1;
+  const int which = blockIdx.x & 1;
+  const int64_t o = K * SB;
+  const int sz = (int)imin64(SB, n - o);
+  const int nb = (sz + TB - 1) / TB;
+  constexpr int NSUB = SB / TB;
+  double* X = dinv512 + (K * 2 + which) * (int64_t)SB * SB;
   const int tid = threadIdx.x;
-  const int64_t nb = ceil_div(n, TB);
+  const int tr = (tid >> 4) * 4, tc = (tid & 15) * 4;
+  // X = blockdiag(D_i), identity past the matrix
+  for (int64_t e = tid; e < (int64_t)SB * SB; e += 256) {
+    const int r = (int)(e / SB), c = (int)(e % SB);
+    const int bi = r / TB, bj = c / TB;
+    double v = 0.0;
+    if (bi == bj) {
+      if (bi < nb)
+        v = dinv64[((o / TB + bi) * 2 + which) * TB * TB + (r % TB) * TB + (c % TB)];
+      else
+        v = r == c ? 1.0 : 0.0;
+    }
+    X[e] = v;
+  }
+  __syncthreads();
+  double acc[4][4];
+  auto tile_mma = [&](double sign) {
+#pragma unroll
+    for (int q = 0; q < TB; ++q) {
+      double a[4], b[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        a[u] = As[tr + u][q];
+        b[u] = Bs[q][tc + u];
+      }
+#pragma unroll
+      for (int u = 0; u < 4; ++u)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) acc[u][v] = __builtin_fma(sign * a[u], b[v], acc[u][v]);
+    }
+  };
+  for (int j = 0; j < nb; ++j) {
+    for (int step = 1; step < nb; ++step) {
+      // lower: rows below j; upper: rows above j, nearest first
+      const int i = which == 0 ? j + step : j - step;
+      if (i < 0 || i >= nb) break;
+#pragma unroll
+      for (int u = 0; u < 4; ++u)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) acc[u][v] = 0.0;
+      const int k0 = which == 0 ? j : i + 1, k1 = which == 0 ? i - 1 : j;
+      for (int k = k0; k <= k1; ++k) {
+        // As = factor tile (i, k) (strictly lower / upper: no diagonal), Bs = X tile (k, j)
+        for (int e = tid; e < TB * TB; e += 256) {
+          const int r = e / TB, c = e % TB;
+          const int gr = i * TB + r, gc = k * TB + c;
+          As[r][c] = (gr < sz && gc < sz) ? (double)LU[(o + gr) * lda + o + gc] : 0.0;
+          Bs[r][c] = X[(int64_t)(k * TB + r) * SB + j * TB + c];
+        }
+        __syncthreads();
+        tile_mma(1.0);
+        __syncthreads();
+      }
+      // X_ij = -D_i T: As = D_i, Bs = T
+      for (int e = tid; e < TB * TB; e += 256) {
+        const int r = e / TB, c = e % TB;
+        As[r][c] = X[(int64_t)(i * TB + r) * SB + i * TB + c];
+      }
+#pragma unroll
+      for (int u = 0; u < 4; ++u)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) Bs[tr + u][tc + v] = acc[u][v];
+      __syncthreads();
+#pragma unroll
+      for (int u = 0; u < 4; ++u)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) acc[u][v] = 0.0;
+      tile_mma(-1.0);
+#pragma unroll
+      for (int u = 0; u < 4; ++u)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) X[(int64_t)(i * TB + tr + u) * SB + j * TB + tc + v] = acc[u][v];
+      __syncthreads();
+    }
+  }
+  (void)NSUB;
+}
+
+// ctl[0] ticket, ctl[2] status of the solve; the s-th processed superblock
+// owns the 128-byte line ctl[kCtlLine (s + 1) ..]: word e < 3 counts its
+// chunks that have published stage e (0: accumulated values, 1: first
+// solution, 2: its residual), word 3 its solved chunks.  Separate lines keep
+// the polls of one superblock's counters off the others'.
+//
+// REFINE (FP64 U factors): the inverse of an ill-conditioned 512 x 512 block
+// of U applied to the accumulated values loses digits that a substitution
+// keeps (measured on the adjoint gradients).  One step of refinement inside
+// the block restores them: x0 = D acc, r = acc - T_KK x0, x = x0 + D r, at two
+// more exchanges per superblock.  The FP32 factors of the preconditioner
+// need no refinement: an application error there only perturbs the
+// preconditioner.
+template <typename TF, bool FWD, bool TRANS, int WHICH, bool REFINE>
+__global__ __launch_bounds__(256) void sbsweep_kernel(const TF* __restrict__ F, int64_t lda, int64_t n,
+                                                      const double* __restrict__ D512, double* __restrict__ y,
+                                                      double* __restrict__ P, unsigned* __restrict__ ctl,
+                                                      uint64_t wait_ticks) {
+  typedef double T;
+  __shared__ T xs[SB];             // folded solution / accumulated values of the superblock
+  __shared__ T xs2[SB];            // first solution, then its residual (REFINE)
+  __shared__ T red[4][CH + 1];     // cross-wave partial sums
+  __shared__ int s_t;
+  __shared__ unsigned s_avail;
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  const int64_t nsb = ceil_div(n, SB);
   unsigned* ticket = ctl;
-  unsigned* done = ctl + 1;
   int* status = reinterpret_cast<int*>(ctl + 2);
   if (tid == 0) {
-    const unsigned t = __hip_atomic_fetch_add(ticket, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    s_I = (int)t;
+    s_t = (int)__hip_atomic_fetch_add(ticket, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     s_avail = 0;
-    // an earlier sweep of this solve gave up: nothing to compute
-    if (__hip_atomic_load(status, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) s_I = INT_MAX;
+    if (__hip_atomic_load(status, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) s_t = INT_MAX;
   }
   __syncthreads();
-  const int seq = s_I;
-  if (seq >= nb) return;
-  const int64_t I = FWD ? seq : nb - 1 - seq;
-  const int64_t oI = I * TB;
-  const int szI = (int)imin64(TB, n - oI);
+  const int t = s_t;
+  if ((int64_t)t >= nsb * NCH) return;
+  const int s = t / NCH, c = t % NCH;
+  unsigned* parts = ctl + kCtlLine * (s + 1);
+  const int64_t I = FWD ? s : nsb - 1 - s;
+  const int64_t oI = I * SB;
+  const int szI = (int)imin64(SB, n - oI);
+  const int rc0 = c * CH;                                   // first row of the chunk in the superblock
+  const int nr = szI - rc0 < 0 ? 0 : (szI - rc0 < CH ? szI - rc0 : CH);
+  // Thread layout of every CH x SB product, for the chunk's rows:
+  //   no-transpose: wave w takes rows 8w..8w+7, lane l the columns l, l+64,
+  //     ..., l+448 (coalesced rows of the factor); reduced over the lanes;
+  //   transpose: lane l & 31 takes chunk column l & 31, the 8 half-waves
+  //     g = 2w + (l >> 5) the rows 64g .. 64g+63 of F[rows][chunk columns];
+  //     reduced over the half-waves.
+  const int tcol = lane & 31, tgrp = 2 * w + (lane >> 5);
 
-  // Thread ownership of a tile, kept for every off-diagonal tile: lanes run
-  // along the stored rows (coalesced), a wave owns 16 of the other index.
-  //   no-transpose: column rr = tid%64 of block J, rows q*16 .. q*16+15 of
-  //                 block I (16 partial row sums per thread, summed over the
-  //                 lanes at the end)
-  //   transpose:    column rr = tid%64 of block I, rows q*16 .. q*16+15 of block J
-  const int rr = tid & 63;
-  const int q = tid >> 6;
-
-  // prefetch the diagonal inverse and the tile of the last dependency (the
-  // one on the critical path) before waiting
-  const double* D = dinv + (I * 2 + WHICH) * TB * TB;
-  for (int e = tid; e < TB * TB; e += kTsThreads) Ds[e / TB][e % TB] = D[e];
-  T pre[16];
-  if (seq > 0) {
-    const int64_t Jl = FWD ? I - 1 : I + 1;
-    const int64_t oJ = Jl * TB;
-    const int szJ = (int)imin64(TB, n - oJ);
-#pragma unroll
-    for (int k = 0; k < 16; ++k) {
-      const int kk = q * 16 + k;
-      if (TRANS)
-        pre[k] = (rr < szI && kk < szJ) ? (T)F[(oJ + kk) * lda + oI + rr] : T(0);
+  // bounded wait of thread 0 until *ctr >= target; false if the solve failed.
+  // far: not on the critical path, poll less often
+  auto wait_count = [&](const unsigned* ctr, unsigned target, unsigned& v, bool far) -> bool {
+    uint32_t spins = 0;
+    WaitClock clk(wait_ticks);
+    while ((v = ld_coh(ctr)) < target) {
+      if (far)
+        __builtin_amdgcn_s_sleep(8);
       else
-        pre[k] = (kk < szI && rr < szJ) ? (T)F[(oI + kk) * lda + oJ + rr] : T(0);
-    }
-  }
-
-  // fold in the dependencies in completion order d = 0 .. seq-1 as they
-  // become available (private partial sums per thread, no barrier per tile)
-  T s = 0;
-  T ps[TRANS ? 1 : 16];
-#pragma unroll
-  for (int k = 0; k < (TRANS ? 1 : 16); ++k) ps[k] = 0;
-  int d = 0;
-  while (d < seq) {
-    if ((unsigned)d >= s_avail) {
-      __syncthreads();
-      if (tid == 0) {
-        unsigned v;
-        uint32_t spins = 0;
-        WaitClock clk(wait_ticks);
-        while ((v = ld_coh(done)) <= (unsigned)d) {
-          __builtin_amdgcn_s_sleep(1);
-          ++spins;
-          if (clk.expired()) {
-            __hip_atomic_store(status, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            v = 0xffffffffu;
-            break;
-          }
-          if ((spins & 255u) == 0 &&
-              __hip_atomic_load(status, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) {
-            v = 0xffffffffu;
-            break;
-          }
-        }
-        s_avail = v;
+        __builtin_amdgcn_s_sleep(1);
+      ++spins;
+      if (clk.expired()) {
+        __hip_atomic_store(status, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        return false;
       }
-      __syncthreads();
-      if (s_avail == 0xffffffffu) return;
+      if ((spins & 255u) == 0 && __hip_atomic_load(status, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0)
+        return false;
     }
-    const int lim = (unsigned)seq < s_avail ? seq : (int)s_avail;
-    // two tiles at a time while neither is the last dependency: all
-    // their loads are issued before any is used (folding one tile at a time
-    // is bound by the load latency once the factor no longer fits in the
-    // Infinity Cache).  Every product is an explicit FMA in the same order as
-    // tile by tile, so the result does not depend on how the tiles group
-    constexpr int FB = 2;
-    for (; d + FB <= lim && d + FB <= seq - 1; d += FB) {
-      T f[FB][16];
-      if constexpr (!TRANS) {
-        T yk[FB];
+    return true;
+  };
+  // reduce the per-thread partial sums of a product into red; after the
+  // barrier, thread r < CH reads its row's value with row_value(r)
+  auto reduce_rows = [&](T* ps) {
+    if constexpr (!TRANS) {
 #pragma unroll
-        for (int u = 0; u < FB; ++u) {
-          const int64_t oJ = (FWD ? d + u : nb - 1 - (d + u)) * TB;
-          const bool ok = rr < (int)imin64(TB, n - oJ);
-          yk[u] = ok ? ld_coh(y + oJ + rr) : T(0);
-#pragma unroll
-          for (int k = 0; k < 16; ++k)
-            f[u][k] = (ok && q * 16 + k < szI) ? (T)F[(oI + q * 16 + k) * lda + oJ + rr] : T(0);
-        }
-#pragma unroll
-        for (int u = 0; u < FB; ++u)
-#pragma unroll
-          for (int k = 0; k < 16; ++k) ps[k] = __builtin_fma(f[u][k], yk[u], ps[k]);
-      } else {
-        T yk[FB][16];
-#pragma unroll
-        for (int u = 0; u < FB; ++u) {
-          const int64_t oJ = (FWD ? d + u : nb - 1 - (d + u)) * TB;
-          const int szJ = (int)imin64(TB, n - oJ);
-#pragma unroll
-          for (int k = 0; k < 16; ++k) {
-            const bool ok = rr < szI && q * 16 + k < szJ;
-            f[u][k] = ok ? (T)F[(oJ + q * 16 + k) * lda + oI + rr] : T(0);
-            yk[u][k] = ok ? ld_coh(y + oJ + q * 16 + k) : T(0);
-          }
-        }
-#pragma unroll
-        for (int u = 0; u < FB; ++u)
-#pragma unroll
-          for (int k = 0; k < 16; ++k) s = __builtin_fma(f[u][k], yk[u][k], s);
+      for (int k = 0; k < 8; ++k) {
+        T v = ps[k];
+        for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+        if (lane == 0) red[0][8 * w + k] = v;
       }
+    } else {
+      const T v = ps[0] + __shfl_xor(ps[0], 32, 64);
+      if (lane < 32) red[w][tcol] = v;
     }
-    for (; d < lim; ++d) {
-      const int64_t J = FWD ? d : nb - 1 - d;
-      const int64_t oJ = J * TB;
-      const int szJ = (int)imin64(TB, n - oJ);
-      if (!TRANS) {
-        // one y entry per thread (its column), 16 coalesced row segments
-        if (rr >= szJ) continue;
-        const T yk = ld_coh(y + oJ + rr);
-        if (d == seq - 1) {
+  };
+  auto row_value = [&](int r) -> T {
+    return TRANS ? (red[0][r] + red[1][r]) + (red[2][r] + red[3][r]) : red[0][r];
+  };
+  // rows of the diagonal inverse (or its transpose) applied to vec (LDS)
+  const T* D = D512 + (I * 2 + WHICH) * (int64_t)SB * SB;
+  auto apply_D = [&](const T* vec) {
+    T ps[TRANS ? 1 : 8];
+    if constexpr (!TRANS) {
 #pragma unroll
-          for (int k = 0; k < 16; ++k) ps[k] = __builtin_fma(pre[k], yk, ps[k]);
-        } else {
+      for (int k = 0; k < 8; ++k) {
+        const int r = 8 * w + k;
+        T v = 0;
+        if (r < nr) {
+          const T* drow = D + (int64_t)(rc0 + r) * SB;
 #pragma unroll
-          for (int k = 0; k < 16; ++k)
-            if (q * 16 + k < szI) ps[k] = __builtin_fma((T)F[(oI + q * 16 + k) * lda + oJ + rr], yk, ps[k]);
+          for (int m = 0; m < SB / 64; ++m) v = __builtin_fma(drow[lane + 64 * m], vec[lane + 64 * m], v);
         }
-        continue;
+        ps[k] = v;
       }
-      if (rr >= szI) continue;
-      if (d == seq - 1) {
-#pragma unroll
-        for (int k = 0; k < 16; ++k) {
-          const int kk = q * 16 + k;
-          if (kk < szJ) s = __builtin_fma(pre[k], ld_coh(y + oJ + kk), s);
+    } else {
+      T v = 0;
+      if (tcol < nr)
+        for (int q = 0; q < SB / 8; ++q) {
+          const int qq = tgrp * (SB / 8) + q;
+          v = __builtin_fma(D[(int64_t)qq * SB + rc0 + tcol], vec[qq], v);
         }
-      } else {
-#pragma unroll
-        for (int k = 0; k < 16; ++k) {
-          const int kk = q * 16 + k;
-          if (kk < szJ) s = __builtin_fma((T)F[(oJ + kk) * lda + oI + rr], ld_coh(y + oJ + kk), s);
-        }
-      }
+      ps[0] = v;
     }
-  }
-
-  // acc = b_I - (sum of the folded tiles)
-  if (!TRANS) {
-    // row sums over the 64 column lanes: through LDS, 4 threads per row
-#pragma unroll
-    for (int k = 0; k < 16; ++k) red[q * 16 + k][rr] = ps[k];
+    reduce_rows(ps);
+  };
+  // publish the chunk's CH values of stage e into buf, wait for the
+  // superblock's other chunks, then stage the superblock's SB values in dst
+  auto exchange = [&](T* buf, int e, T value, T* dst) -> bool {
+    if (tid < nr) st_coh(buf + oI + rc0 + tid, value);
+    __builtin_amdgcn_s_waitcnt(0);
     __syncthreads();
-    const int r = tid >> 2, qq = tid & 3;
-    T t = 0;
-#pragma unroll
-    for (int c = 0; c < 16; ++c) t += red[r][qq * 16 + c];
-    t += __shfl_xor(t, 1, 64);
-    t += __shfl_xor(t, 2, 64);
-    if (qq == 0) acc[r] = r < szI ? y[oI + r] - t : T(0);
-  } else {
-    part[q][rr] = s;
+    if (tid == 0) {
+      __hip_atomic_fetch_add(parts + e, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      unsigned v;
+      s_avail = wait_count(parts + e, (unsigned)NCH, v, false) ? 0u : 0xffffffffu;
+    }
     __syncthreads();
-    if (tid < TB) acc[tid] = tid < szI ? y[oI + tid] - (part[0][tid] + part[1][tid] + part[2][tid] + part[3][tid]) : T(0);
+    if (s_avail == 0xffffffffu) return false;
+    for (int q = tid; q < SB; q += 256) dst[q] = q < szI ? ld_coh(buf + oI + q) : T(0);
+    __syncthreads();
+    return true;
+  };
+
+  // 1. fold the superblocks solved before this one, in completion order
+  T ps[TRANS ? 1 : 8];
+#pragma unroll
+  for (int k = 0; k < (TRANS ? 1 : 8); ++k) ps[k] = 0;
+  for (int d = 0; d < s; ++d) {
+    __syncthreads();
+    if (tid == 0) {
+      unsigned v;
+      s_avail = wait_count(ctl + kCtlLine * (d + 1) + 3, (unsigned)NCH, v, d + 1 < s) ? 0u : 0xffffffffu;
+    }
+    __syncthreads();
+    if (s_avail == 0xffffffffu) return;
+    const int64_t J = FWD ? d : nsb - 1 - d;
+    const int64_t oJ = J * SB;
+    const int szJ = (int)imin64(SB, n - oJ);
+    __syncthreads();  // xs free
+    for (int e = tid; e < SB; e += 256) xs[e] = e < szJ ? ld_coh(y + oJ + e) : T(0);
+    __syncthreads();
+    if constexpr (!TRANS) {
+#pragma unroll
+      for (int k = 0; k < 8; ++k) {
+        const int r = 8 * w + k;
+        if (r < nr) {
+          const TF* row = F + (oI + rc0 + r) * lda + oJ;
+          T sacc = 0;
+#pragma unroll
+          for (int m = 0; m < SB / 64; ++m) {
+            const int col = lane + 64 * m;
+            if (col < szJ) sacc = __builtin_fma((T)row[col], xs[col], sacc);
+          }
+          ps[k] += sacc;
+        }
+      }
+    } else {
+      if (tcol < nr) {
+        T sacc = 0;
+        for (int q = 0; q < SB / 8; ++q) {
+          const int jr = tgrp * (SB / 8) + q;
+          if (jr < szJ) sacc = __builtin_fma((T)F[(oJ + jr) * lda + oI + rc0 + tcol], xs[jr], sacc);
+        }
+        ps[0] += sacc;
+      }
+    }
   }
+  // 2. accumulated values b - (sum of the folded tiles); exchange them
   __syncthreads();
-  // diagonal block: y_I = D acc (D = inverse of the diagonal block, or its transpose)
-  {
-    const int r = tid >> 2, qq = tid & 3;
-    T t = 0;
+  reduce_rows(ps);
+  __syncthreads();
+  T acc_r = tid < nr ? y[oI + rc0 + tid] - row_value(tid) : T(0);
+  if (!exchange(P, 0, acc_r, xs)) return;
+  // 3. x = rows of D applied to the superblock's accumulated values
+  apply_D(xs);
+  __syncthreads();
+  T x = tid < nr ? row_value(tid) : T(0);
+  if constexpr (REFINE) {
+    // x0 exchanged; r = acc - T_KK x0 on the chunk's rows; exchanged; x += D r
+    T* X0 = P + n;
+    T* Rb = P + 2 * n;
+    if (!exchange(X0, 1, x, xs2)) return;
+    T pr[TRANS ? 1 : 8];
+    if constexpr (!TRANS) {
+      // T_KK = U_KK: row rc0 + r, columns q >= rc0 + r
 #pragma unroll
-    for (int c = 0; c < 16; ++c) {
-      const int cc = qq * 16 + c;
-      t += (TRANS ? Ds[cc][r] : Ds[r][cc]) * acc[cc];
+      for (int k = 0; k < 8; ++k) {
+        const int r = 8 * w + k;
+        T v = 0;
+        if (r < nr) {
+          const TF* row = F + (oI + rc0 + r) * lda + oI;
+#pragma unroll
+          for (int m = 0; m < SB / 64; ++m) {
+            const int q = lane + 64 * m;
+            if (q >= rc0 + r && q < szI) v = __builtin_fma((T)row[q], xs2[q], v);
+          }
+        }
+        pr[k] = v;
+      }
+    } else {
+      // T_KK = U_KK^T: (U^T x0)[rc0 + r] = sum over q <= rc0 + r of U[q][rc0 + r] x0[q]
+      T v = 0;
+      if (tcol < nr)
+        for (int q0 = 0; q0 < SB / 8; ++q0) {
+          const int q = tgrp * (SB / 8) + q0;
+          if (q <= rc0 + tcol && q < szI) v = __builtin_fma((T)F[(oI + q) * lda + oI + rc0 + tcol], xs2[q], v);
+        }
+      pr[0] = v;
     }
-    t += __shfl_xor(t, 1, 64);
-    t += __shfl_xor(t, 2, 64);
-    if (qq == 0 && r < szI) st_coh(y + oI + r, t);
+    __syncthreads();
+    reduce_rows(pr);
+    __syncthreads();
+    const T r_r = tid < nr ? xs[rc0 + tid] - row_value(tid) : T(0);
+    if (!exchange(Rb, 2, r_r, xs2)) return;
+    apply_D(xs2);
+    __syncthreads();
+    if (tid < nr) x += row_value(tid);
   }
-  // publish: the slice's coherent stores are complete before the counter moves
+  if (tid < nr) st_coh(y + oI + rc0 + tid, x);
   __builtin_amdgcn_s_waitcnt(0);
   __syncthreads();
-  if (tid == 0) __hip_atomic_fetch_add(done, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  if (tid == 0) __hip_atomic_fetch_add(parts + 3, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Clears the ticket and the superblocks' control lines; the solve's status
+// word ctl[2] stays.
+__global__ void reset_sweep_ctl_kernel(unsigned* ctl, int64_t nsb) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < 2 || (i >= kCtlLine && i < kCtlLine * (nsb + 1))) ctl[i] = 0u;
 }
 
 template <typename T>
-static int launch_sweep(bool fwd, bool trans, int which, const T* F, int64_t lda, int64_t n,
-                        const double* dinv, double* y, unsigned* ctl, hipStream_t s) {
-  const unsigned nb = (unsigned)ceil_div(n, TB);
-  // reset ticket/done before each sweep (the status stays)
-  hipLaunchKernelGGL(gather_perm_kernel, dim3(1), dim3(1), 0, s, (const double*)nullptr,
-                     (const int64_t*)nullptr, (int64_t)0, (double*)nullptr, ctl, 0);
+static int launch_sbsweep(bool fwd, bool trans, int which, const T* F, int64_t lda, int64_t n,
+                          const double* D512, double* y, double* P, unsigned* ctl, hipStream_t s) {
+  const int64_t nsb = ceil_div(n, (int64_t)SB);
+  hipLaunchKernelGGL(reset_sweep_ctl_kernel, dim3((unsigned)ceil_div(kCtlLine * (nsb + 1), 256)), dim3(256), 0, s,
+                     ctl, nsb);
   RBFX_CHECK_LAUNCH();
   const uint64_t ticks = device_info().wait_ticks;
+  const unsigned grid = (unsigned)(nsb * NCH);
   const int pid = prof_start(sizeof(T) == 8 ? "sweep_f64" : "sweep_f32", s);
-#define RBFX_SWEEP(F_, T_, W_)                                                               \
-  hipLaunchKernelGGL((sweep_kernel<T, F_, T_, W_>), dim3(nb), dim3(kTsThreads), 0, s, F, lda, n, \
-                     dinv, y, ctl, ticks)
-  if (fwd && !trans && which == 0) RBFX_SWEEP(true, false, 0);
-  else if (!fwd && !trans && which == 1) RBFX_SWEEP(false, false, 1);
-  else if (fwd && trans && which == 1) RBFX_SWEEP(true, true, 1);
-  else if (!fwd && trans && which == 0) RBFX_SWEEP(false, true, 0);
+  // one refinement step in the U blocks of FP64 factors
+  constexpr bool R64 = sizeof(T) == 8;
+#define RBFX_SBSWEEP(F_, T_, W_)                                                                              \
+  hipLaunchKernelGGL((sbsweep_kernel<T, F_, T_, W_, R64 && W_ == 1>), dim3(grid), dim3(256), 0, s, F, lda, n, \
+                     D512, y, P, ctl, ticks)
+  if (fwd && !trans && which == 0) RBFX_SBSWEEP(true, false, 0);
+  else if (!fwd && !trans && which == 1) RBFX_SBSWEEP(false, false, 1);
+  else if (fwd && trans && which == 1) RBFX_SBSWEEP(true, true, 1);
+  else if (!fwd && trans && which == 0) RBFX_SBSWEEP(false, true, 0);
   else {
     set_error("invalid sweep combination");
     return RBFX_ERR_ARG;
   }
-#undef RBFX_SWEEP
+#undef RBFX_SBSWEEP
   prof_stop(pid, s, (double)n * n, (double)sizeof(T) * n * (n + 1) / 2.0);
   RBFX_CHECK_LAUNCH();
   return RBFX_OK;
 }
 
-// Workspace of one solve: the permuted vector, then the control words.
-static bool carve_getrs(void* work, size_t work_bytes, int64_t n, double*& y, unsigned*& ctl) {
+// Workspace of one solve: the permuted vector, the superblocks' accumulated
+// values, then the control words (3 + one per superblock).
+static int64_t getrs_ctl_words(int64_t n) { return kCtlLine * (ceil_div(n, (int64_t)SB) + 1) + 8; }
+// P holds three n-vectors: the accumulated values, and for the refined U
+// sweeps the first solutions and their residuals.
+static bool carve_getrs(void* work, size_t work_bytes, int64_t n, double*& y, double*& P, unsigned*& ctl) {
   Workspace ws(work, work_bytes);
   y = ws.take<double>(n);
-  ctl = ws.take<unsigned>(64);
-  return y && ctl;
+  P = ws.take<double>(3 * n);
+  ctl = ws.take<unsigned>(getrs_ctl_words(n));
+  return y && P && ctl;
 }
+static int64_t dinv64_elems(int64_t n) { return ceil_div(n, TB) * 2 * TB * TB; }
 
 template <typename T>
 static int getrs(int trans, int64_t n, const T* LU, int64_t lda, const int64_t* perm, const double* dinv,
                  double* b, void* work, size_t work_bytes, hipStream_t s) {
   RBFX_REQUIRE(n > 0 && lda >= n && LU && perm && dinv && b, "bad getrs args");
-  double* y;
+  RBFX_REQUIRE(n < (int64_t)INT32_MAX / 2, "getrs: n too large");
+  double *y, *P;
   unsigned* ctl;
-  RBFX_REQUIRE(carve_getrs(work, work_bytes, n, y, ctl), "getrs workspace too small");
+  RBFX_REQUIRE(carve_getrs(work, work_bytes, n, y, P, ctl), "getrs workspace too small");
+  const double* D512 = dinv + dinv64_elems(n);
   const unsigned g = (unsigned)ceil_div(n, 256);
   int st;
   if (!trans) {
     hipLaunchKernelGGL(gather_perm_kernel, dim3(g), dim3(256), 0, s, b, perm, n, y, ctl, 1);
     RBFX_CHECK_LAUNCH();
-    if ((st = launch_sweep<T>(true, false, 0, LU, lda, n, dinv, y, ctl, s))) return st;
-    if ((st = launch_sweep<T>(false, false, 1, LU, lda, n, dinv, y, ctl, s))) return st;
+    if ((st = launch_sbsweep<T>(true, false, 0, LU, lda, n, D512, y, P, ctl, s))) return st;
+    if ((st = launch_sbsweep<T>(false, false, 1, LU, lda, n, D512, y, P, ctl, s))) return st;
     hipLaunchKernelGGL(scatter_perm_kernel, dim3(g), dim3(256), 0, s, y, (const int64_t*)nullptr, n, b,
                        ctl);
     RBFX_CHECK_LAUNCH();
@@ -361,8 +517,8 @@ static int getrs(int trans, int64_t n, const T* LU, int64_t lda, const int64_t* 
     hipLaunchKernelGGL(gather_perm_kernel, dim3(g), dim3(256), 0, s, b, (const int64_t*)nullptr, n, y,
                        ctl, 1);
     RBFX_CHECK_LAUNCH();
-    if ((st = launch_sweep<T>(true, true, 1, LU, lda, n, dinv, y, ctl, s))) return st;
-    if ((st = launch_sweep<T>(false, true, 0, LU, lda, n, dinv, y, ctl, s))) return st;
+    if ((st = launch_sbsweep<T>(true, true, 1, LU, lda, n, D512, y, P, ctl, s))) return st;
+    if ((st = launch_sbsweep<T>(false, true, 0, LU, lda, n, D512, y, P, ctl, s))) return st;
     hipLaunchKernelGGL(scatter_perm_kernel, dim3(g), dim3(256), 0, s, y, perm, n, b, ctl);
     RBFX_CHECK_LAUNCH();
   }
@@ -444,9 +600,9 @@ static int getrs_multi(int64_t n, const double* LU, int64_t lda, const int64_t* 
 // Status of the last solve queued with this workspace (synchronises).
 static int getrs_status(int64_t n, void* work, size_t work_bytes, hipStream_t s) {
   RBFX_REQUIRE(n > 0 && work, "bad getrs_status args");
-  double* y;
+  double *y, *P;
   unsigned* ctl;
-  RBFX_REQUIRE(carve_getrs(work, work_bytes, n, y, ctl), "getrs workspace too small");
+  RBFX_REQUIRE(carve_getrs(work, work_bytes, n, y, P, ctl), "getrs workspace too small");
   unsigned st = 0;
   RBFX_HIP(hipMemcpyAsync(&st, ctl + 2, sizeof(st), hipMemcpyDeviceToHost, s));
   RBFX_HIP(hipStreamSynchronize(s));
@@ -473,6 +629,9 @@ static int prepare(int64_t n, const T* LU, int64_t lda, const int64_t* ipiv, int
   RBFX_HIP(hipMemcpyAsync(perm, pm.data(), n * sizeof(int64_t), hipMemcpyHostToDevice, s));
   hipLaunchKernelGGL((diag_inv_kernel<T>), dim3((unsigned)ceil_div(n, TB)), dim3(TB), 0, s, LU, lda, n, dinv);
   RBFX_CHECK_LAUNCH();
+  hipLaunchKernelGGL((sbinv_kernel<T>), dim3((unsigned)(2 * ceil_div(n, (int64_t)SB))), dim3(256), 0, s, LU, lda, n,
+                     (const double*)dinv, dinv + dinv64_elems(n));
+  RBFX_CHECK_LAUNCH();
   RBFX_HIP(hipStreamSynchronize(s));  // pm must outlive the copy
   return RBFX_OK;
 }
@@ -491,10 +650,14 @@ extern "C" int rbfx_getrs_multi_f64(int64_t n, const double* d_LU, int64_t lda, 
   return getrs_multi(n, d_LU, lda, d_perm, d_dinv, nrhs, d_B, ldb, d_work, work_bytes, as_stream(stream));
 }
 
-extern "C" int64_t rbfx_getrs_dinv_elems(int64_t n) { return n <= 0 ? 0 : ceil_div(n, TB) * 2 * TB * TB; }
+extern "C" int64_t rbfx_getrs_dinv_elems(int64_t n) {
+  return n <= 0 ? 0 : dinv64_elems(n) + ceil_div(n, (int64_t)SB) * 2 * (int64_t)SB * SB;
+}
 
 extern "C" size_t rbfx_getrs_workspace(int64_t n) {
-  return ws_round((size_t)n * sizeof(double)) + ws_round(64 * sizeof(unsigned)) + 512;
+  if (n <= 0) return 0;
+  return ws_round((size_t)n * sizeof(double)) + ws_round((size_t)3 * n * sizeof(double)) +
+         ws_round(getrs_ctl_words(n) * sizeof(unsigned)) + 512;
 }
 
 extern "C" int rbfx_getrs_prepare_f64(int64_t n, const double* d_LU, int64_t lda, const int64_t* d_ipiv,
