@@ -95,6 +95,29 @@ def test_getrf_getrs(dtype, n):
 
 
 @pytest.mark.parametrize("dtype", [torch.float64, torch.float32])
+@pytest.mark.parametrize("n", [511, 1301, 2100, 4099])
+def test_getrs_applies_stored_factors(dtype, n):
+    """The superblock sweeps apply the stored factors (FP32 ones included) in
+    FP64: x matches an FP64 substitution with the same factors (SciPy) to
+    near machine precision, for ragged last superblocks and empty chunks."""
+    import scipy.linalg as sl
+    from updes_amd import engine as E
+    _lib()
+    a = _random_well(n, 7 * n)
+    m = torch.zeros((n, E._ld(n)), dtype=dtype)
+    m[:, :n] = torch.as_tensor(a).to(dtype)
+    lu = E.LU(m.cuda(), n)
+    LU = lu.a[:, :n].double().cpu().numpy()
+    piv = lu.ipiv.cpu().numpy()
+    b = np.random.default_rng(n).standard_normal(n)
+    for trans in (False, True):
+        x = lu.solve(torch.as_tensor(b).cuda(), trans=trans).cpu().numpy()
+        ref = sl.lu_solve((LU, piv), b, trans=1 if trans else 0)
+        rel = np.linalg.norm(x - ref) / np.linalg.norm(ref)
+        assert rel < 1e-11, (trans, rel)
+
+
+@pytest.mark.parametrize("dtype", [torch.float64, torch.float32])
 @pytest.mark.parametrize("n,groups,narrow", [(500, 1, 0), (700, 1, 0), (1301, 2, 0), (1301, 3, 0),
                                              (1301, 0, 1), (700, 1, 1), (1301, 3, 600)])
 def test_getrf_multirow_leaves(dtype, n, groups, narrow):

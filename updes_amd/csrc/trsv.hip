@@ -209,11 +209,13 @@ __global__ __launch_bounds__(256) void sbinv_kernel(const TF* __restrict__ LU, i
   (void)NSUB;
 }
 
-// ctl[0] ticket, ctl[2] status of the solve; the s-th processed superblock
-// owns the 128-byte line ctl[kCtlLine (s + 1) ..]: word e < 3 counts its
-// chunks that have published stage e (0: accumulated values, 1: first
-// solution, 2: its residual), word 3 its solved chunks.  Separate lines keep
-// the polls of one superblock's counters off the others'.
+// ctl[0] ticket, ctl[1] superblocks solved, ctl[2] status of the solve; the
+// s-th processed superblock owns the 128-byte line ctl[kCtlLine (s + 1) ..]:
+// word e < 3 counts its chunks that have published stage e (0: accumulated
+// values, 1: first solution, 2: its residual), word 3 its solved chunks.
+// Separate lines keep the polls of one superblock's counters off the others'.
+// Superblocks complete in order (each folds its predecessor), so ctl[1] >= k
+// means superblocks 0 .. k-1 are solved.
 //
 // REFINE (FP64 U factors): the inverse of an ill-conditioned 512 x 512 block
 // of U applied to the accumulated values loses digits that a substitution
@@ -222,24 +224,33 @@ __global__ __launch_bounds__(256) void sbinv_kernel(const TF* __restrict__ LU, i
 // more exchanges per superblock.  The FP32 factors of the preconditioner
 // need no refinement: an application error there only perturbs the
 // preconditioner.
+constexpr int kSweepThreads = 512;
+constexpr int kSweepWaves = kSweepThreads / 64;
 template <typename TF, bool FWD, bool TRANS, int WHICH, bool REFINE>
-__global__ __launch_bounds__(256) void sbsweep_kernel(const TF* __restrict__ F, int64_t lda, int64_t n,
-                                                      const double* __restrict__ D512, double* __restrict__ y,
-                                                      double* __restrict__ P, unsigned* __restrict__ ctl,
-                                                      uint64_t wait_ticks) {
+__global__ __launch_bounds__(kSweepThreads) void sbsweep_kernel(const TF* __restrict__ F, int64_t lda, int64_t n,
+                                                                const double* __restrict__ D512, double* __restrict__ y,
+                                                                double* __restrict__ P, unsigned* __restrict__ ctl,
+                                                                uint64_t wait_ticks) {
   typedef double T;
+  constexpr int RW = CH / kSweepWaves;        // chunk rows per wave (no-transpose)
+  constexpr int NG = 2 * kSweepWaves;         // half-wave groups (transpose)
+  constexpr int NV = SB * CH / kSweepThreads;  // operand values per thread
+  constexpr int NP = TRANS ? 1 : RW;          // partial sums per thread
+  static_assert(RW * kSweepWaves == CH && NV * NG == SB && NV == 8 * RW, "sweep layout");
   __shared__ T xs[SB];             // folded solution / accumulated values of the superblock
   __shared__ T xs2[SB];            // first solution, then its residual (REFINE)
-  __shared__ T red[4][CH + 1];     // cross-wave partial sums
+  __shared__ T red[NG][CH + 1];    // cross-group partial sums
   __shared__ int s_t;
-  __shared__ unsigned s_avail;
+  __shared__ unsigned s_avail, s_solved;
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
   const int64_t nsb = ceil_div(n, SB);
   unsigned* ticket = ctl;
+  unsigned* solved = ctl + 1;
   int* status = reinterpret_cast<int*>(ctl + 2);
   if (tid == 0) {
     s_t = (int)__hip_atomic_fetch_add(ticket, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     s_avail = 0;
+    s_solved = 0;
     if (__hip_atomic_load(status, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) s_t = INT_MAX;
   }
   __syncthreads();
@@ -252,12 +263,13 @@ __global__ __launch_bounds__(256) void sbsweep_kernel(const TF* __restrict__ F, 
   const int szI = (int)imin64(SB, n - oI);
   const int rc0 = c * CH;                                   // first row of the chunk in the superblock
   const int nr = szI - rc0 < 0 ? 0 : (szI - rc0 < CH ? szI - rc0 : CH);
-  // Thread layout of every CH x SB product, for the chunk's rows:
-  //   no-transpose: wave w takes rows 8w..8w+7, lane l the columns l, l+64,
-  //     ..., l+448 (coalesced rows of the factor); reduced over the lanes;
-  //   transpose: lane l & 31 takes chunk column l & 31, the 8 half-waves
-  //     g = 2w + (l >> 5) the rows 64g .. 64g+63 of F[rows][chunk columns];
-  //     reduced over the half-waves.
+  // Thread layout of every CH x SB operand, for the chunk's rows:
+  //   no-transpose: wave w takes rows RW w .. RW w + RW-1, lane l the columns
+  //     l, l+64, ..., l+448 (coalesced rows of the factor); reduced over the
+  //     lanes;
+  //   transpose: lane l & 31 takes chunk column l & 31, the NG half-waves
+  //     g = 2w + (l >> 5) the rows NV g .. NV g + NV-1 of M[rows][chunk
+  //     columns]; reduced over the half-waves.
   const int tcol = lane & 31, tgrp = 2 * w + (lane >> 5);
 
   // bounded wait of thread 0 until *ctr >= target; false if the solve failed.
@@ -285,44 +297,124 @@ __global__ __launch_bounds__(256) void sbsweep_kernel(const TF* __restrict__ F, 
   auto reduce_rows = [&](T* ps) {
     if constexpr (!TRANS) {
 #pragma unroll
-      for (int k = 0; k < 8; ++k) {
+      for (int k = 0; k < RW; ++k) {
         T v = ps[k];
         for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
-        if (lane == 0) red[0][8 * w + k] = v;
+        if (lane == 0) red[0][RW * w + k] = v;
       }
     } else {
-      const T v = ps[0] + __shfl_xor(ps[0], 32, 64);
-      if (lane < 32) red[w][tcol] = v;
+      if (tid < NG * 32) red[tgrp][tcol] = ps[0];
     }
   };
   auto row_value = [&](int r) -> T {
-    return TRANS ? (red[0][r] + red[1][r]) + (red[2][r] + red[3][r]) : red[0][r];
-  };
-  // rows of the diagonal inverse (or its transpose) applied to vec (LDS)
-  const T* D = D512 + (I * 2 + WHICH) * (int64_t)SB * SB;
-  auto apply_D = [&](const T* vec) {
-    T ps[TRANS ? 1 : 8];
     if constexpr (!TRANS) {
+      return red[0][r];
+    } else {
+      T a = 0;
 #pragma unroll
-      for (int k = 0; k < 8; ++k) {
-        const int r = 8 * w + k;
-        T v = 0;
-        if (r < nr) {
-          const T* drow = D + (int64_t)(rc0 + r) * SB;
+      for (int g = 0; g < NG; ++g) a += red[g][r];
+      return a;
+    }
+  };
+  // Loads of the operands are issued before the wait that releases their
+  // use, so the chain of the sweep does not wait on HBM behind the other
+  // workgroups' folds.
+  // the factor's tile of column block oJ (szJ valid); diag: the triangle of
+  // the diagonal block (U_KK, or its transpose) only
+  // Addresses are a wave-uniform base (scalar registers) plus one 32-bit
+  // per-lane offset, so the NV loads in flight hold no vector address each.
+  const int wu = __builtin_amdgcn_readfirstlane(w);
+  const uint32_t hoff = (uint32_t)(lane >> 5) * NV;  // row offset of the upper half-wave (transpose)
+  // Full tiles (every superblock but the last, off the diagonal) load
+  // unmasked; the masks of the others compare against per-call limits, so no
+  // per-value index is kept across the fold loop.
+  auto load_F = [&](TF(&v)[NV], int64_t oJ, int szJ, bool diag) {
+    const bool full = szJ == SB && nr == CH && !diag;
+    if constexpr (!TRANS) {
+      const int clim = szJ - lane;  // columns lane + 64m valid for 64m < clim
 #pragma unroll
-          for (int m = 0; m < SB / 64; ++m) v = __builtin_fma(drow[lane + 64 * m], vec[lane + 64 * m], v);
+      for (int k = 0; k < RW; ++k) {
+        const int r = RW * wu + k;
+        const TF* row = F + (oI + rc0 + r) * lda + oJ;
+        const int dlim = rc0 + r - lane;  // diagonal: 64m >= dlim
+        if (full) {
+#pragma unroll
+          for (int m = 0; m < 8; ++m) v[8 * k + m] = row[(uint32_t)(lane + 64 * m)];
+        } else if (r < nr) {
+#pragma unroll
+          for (int m = 0; m < 8; ++m)
+            v[8 * k + m] = (64 * m < clim && (!diag || 64 * m >= dlim)) ? row[(uint32_t)(lane + 64 * m)] : TF(0);
+        } else {
+#pragma unroll
+          for (int m = 0; m < 8; ++m) v[8 * k + m] = TF(0);
         }
-        ps[k] = v;
       }
     } else {
-      T v = 0;
-      if (tcol < nr)
-        for (int q = 0; q < SB / 8; ++q) {
-          const int qq = tgrp * (SB / 8) + q;
-          v = __builtin_fma(D[(int64_t)qq * SB + rc0 + tcol], vec[qq], v);
-        }
-      ps[0] = v;
+      const TF* base = F + (oJ + (int64_t)2 * wu * NV) * lda + oI + rc0;
+      const uint32_t off = hoff * (uint32_t)lda + (uint32_t)tcol;
+      if (full) {
+#pragma unroll
+        for (int q = 0; q < NV; ++q) v[q] = (base + q * lda)[off];
+      } else {
+        const int g0 = tgrp * NV;
+        const int qlim = tcol < nr ? szJ - g0 : 0;  // rows g0 + q valid for q < qlim
+        const int dlim = rc0 + tcol - g0;           // diagonal: q <= dlim
+#pragma unroll
+        for (int q = 0; q < NV; ++q)
+          v[q] = (q < qlim && (!diag || q <= dlim)) ? (base + q * lda)[off] : TF(0);
+      }
     }
+  };
+  // rows of the superblock's diagonal inverse (columns for the transpose)
+  const T* D = D512 + (I * 2 + WHICH) * (int64_t)SB * SB;
+  auto load_D = [&](T(&v)[NV]) {
+    if constexpr (!TRANS) {
+#pragma unroll
+      for (int k = 0; k < RW; ++k) {
+        const int r = RW * wu + k;
+        const T* row = D + (int64_t)(rc0 + r) * SB;
+        if (r < nr) {
+#pragma unroll
+          for (int m = 0; m < 8; ++m) v[8 * k + m] = row[(uint32_t)(lane + 64 * m)];
+        } else {
+#pragma unroll
+          for (int m = 0; m < 8; ++m) v[8 * k + m] = T(0);
+        }
+      }
+    } else {
+      const T* base = D + (int64_t)2 * wu * NV * SB + rc0;
+      const uint32_t off = hoff * SB + (uint32_t)tcol;
+      if (nr == CH) {
+#pragma unroll
+        for (int q = 0; q < NV; ++q) v[q] = (base + q * SB)[off];
+      } else {
+#pragma unroll
+        for (int q = 0; q < NV; ++q) v[q] = tcol < nr ? (base + q * SB)[off] : T(0);
+      }
+    }
+  };
+  // ps += the operand held in v times the SB-vector vec (LDS), per thread
+  auto product = [&](const auto& v, const T* vec, T* ps) {
+    if constexpr (!TRANS) {
+#pragma unroll
+      for (int k = 0; k < RW; ++k) {
+        T a = 0;
+#pragma unroll
+        for (int m = 0; m < 8; ++m) a = __builtin_fma((T)v[8 * k + m], vec[lane + 64 * m], a);
+        ps[k] += a;
+      }
+    } else {
+      T a = 0;
+#pragma unroll
+      for (int q = 0; q < NV; ++q) a = __builtin_fma((T)v[q], vec[tgrp * NV + q], a);
+      ps[0] += a;
+    }
+  };
+  auto apply = [&](const T(&dv)[NV], const T* vec) {
+    T ps[NP];
+#pragma unroll
+    for (int k = 0; k < NP; ++k) ps[k] = 0;
+    product(dv, vec, ps);
     reduce_rows(ps);
   };
   // publish the chunk's CH values of stage e into buf, wait for the
@@ -338,55 +430,50 @@ __global__ __launch_bounds__(256) void sbsweep_kernel(const TF* __restrict__ F, 
     }
     __syncthreads();
     if (s_avail == 0xffffffffu) return false;
-    for (int q = tid; q < SB; q += 256) dst[q] = q < szI ? ld_coh(buf + oI + q) : T(0);
+    for (int q = tid; q < SB; q += kSweepThreads) dst[q] = q < szI ? ld_coh(buf + oI + q) : T(0);
     __syncthreads();
     return true;
   };
 
-  // 1. fold the superblocks solved before this one, in completion order
-  T ps[TRANS ? 1 : 8];
+  // 1. fold the superblocks solved before this one, in solve order.  The
+  // predecessor's completion is polled on its own line; earlier superblocks
+  // through the solved count, so a workgroup that starts late folds them
+  // without a poll each.  FP32 factors leave room to hold the diagonal
+  // inverse's rows during the last wait as well; FP64 ones load them after
+  // the fold.
+  constexpr bool kEarlyD = sizeof(TF) == 4;
+  T ps[NP];
 #pragma unroll
-  for (int k = 0; k < (TRANS ? 1 : 8); ++k) ps[k] = 0;
+  for (int k = 0; k < NP; ++k) ps[k] = 0;
+  T dv[NV];
+  if (kEarlyD && s == 0) load_D(dv);
   for (int d = 0; d < s; ++d) {
-    __syncthreads();
-    if (tid == 0) {
-      unsigned v;
-      s_avail = wait_count(ctl + kCtlLine * (d + 1) + 3, (unsigned)NCH, v, d + 1 < s) ? 0u : 0xffffffffu;
-    }
-    __syncthreads();
-    if (s_avail == 0xffffffffu) return;
     const int64_t J = FWD ? d : nsb - 1 - d;
     const int64_t oJ = J * SB;
     const int szJ = (int)imin64(SB, n - oJ);
+    TF tv[NV];
+    load_F(tv, oJ, szJ, false);
+    if (kEarlyD && d == s - 1) load_D(dv);
     __syncthreads();  // xs free
-    for (int e = tid; e < SB; e += 256) xs[e] = e < szJ ? ld_coh(y + oJ + e) : T(0);
-    __syncthreads();
-    if constexpr (!TRANS) {
-#pragma unroll
-      for (int k = 0; k < 8; ++k) {
-        const int r = 8 * w + k;
-        if (r < nr) {
-          const TF* row = F + (oI + rc0 + r) * lda + oJ;
-          T sacc = 0;
-#pragma unroll
-          for (int m = 0; m < SB / 64; ++m) {
-            const int col = lane + 64 * m;
-            if (col < szJ) sacc = __builtin_fma((T)row[col], xs[col], sacc);
-          }
-          ps[k] += sacc;
-        }
+    if (tid == 0 && s_solved <= (unsigned)d) {
+      unsigned v;
+      bool ok;
+      if (d + 1 < s) {
+        ok = wait_count(solved, (unsigned)(d + 1), v, true);
+      } else {
+        ok = wait_count(ctl + kCtlLine * (d + 1) + 3, (unsigned)NCH, v, false);
+        v = d + 1;
       }
-    } else {
-      if (tcol < nr) {
-        T sacc = 0;
-        for (int q = 0; q < SB / 8; ++q) {
-          const int jr = tgrp * (SB / 8) + q;
-          if (jr < szJ) sacc = __builtin_fma((T)F[(oJ + jr) * lda + oI + rc0 + tcol], xs[jr], sacc);
-        }
-        ps[0] += sacc;
-      }
+      s_solved = v;
+      s_avail = ok ? 0u : 0xffffffffu;
     }
+    __syncthreads();
+    if (s_avail == 0xffffffffu) return;
+    for (int e = tid; e < SB; e += kSweepThreads) xs[e] = e < szJ ? ld_coh(y + oJ + e) : T(0);
+    __syncthreads();
+    product(tv, xs, ps);
   }
+  if (!kEarlyD) load_D(dv);
   // 2. accumulated values b - (sum of the folded tiles); exchange them
   __syncthreads();
   reduce_rows(ps);
@@ -394,54 +481,40 @@ __global__ __launch_bounds__(256) void sbsweep_kernel(const TF* __restrict__ F, 
   T acc_r = tid < nr ? y[oI + rc0 + tid] - row_value(tid) : T(0);
   if (!exchange(P, 0, acc_r, xs)) return;
   // 3. x = rows of D applied to the superblock's accumulated values
-  apply_D(xs);
+  apply(dv, xs);
   __syncthreads();
   T x = tid < nr ? row_value(tid) : T(0);
   if constexpr (REFINE) {
-    // x0 exchanged; r = acc - T_KK x0 on the chunk's rows; exchanged; x += D r
+    // x0 exchanged; r = acc - U_KK x0 (or U_KK^T x0) on the chunk's rows;
+    // exchanged; x += D r.  The diagonal tile loads during the first
+    // exchange, D's rows again during the second.
     T* X0 = P + n;
     T* Rb = P + 2 * n;
+    TF uv[NV];
+    load_F(uv, oI, szI, true);
     if (!exchange(X0, 1, x, xs2)) return;
-    T pr[TRANS ? 1 : 8];
-    if constexpr (!TRANS) {
-      // T_KK = U_KK: row rc0 + r, columns q >= rc0 + r
+    T pr[NP];
 #pragma unroll
-      for (int k = 0; k < 8; ++k) {
-        const int r = 8 * w + k;
-        T v = 0;
-        if (r < nr) {
-          const TF* row = F + (oI + rc0 + r) * lda + oI;
-#pragma unroll
-          for (int m = 0; m < SB / 64; ++m) {
-            const int q = lane + 64 * m;
-            if (q >= rc0 + r && q < szI) v = __builtin_fma((T)row[q], xs2[q], v);
-          }
-        }
-        pr[k] = v;
-      }
-    } else {
-      // T_KK = U_KK^T: (U^T x0)[rc0 + r] = sum over q <= rc0 + r of U[q][rc0 + r] x0[q]
-      T v = 0;
-      if (tcol < nr)
-        for (int q0 = 0; q0 < SB / 8; ++q0) {
-          const int q = tgrp * (SB / 8) + q0;
-          if (q <= rc0 + tcol && q < szI) v = __builtin_fma((T)F[(oI + q) * lda + oI + rc0 + tcol], xs2[q], v);
-        }
-      pr[0] = v;
-    }
+    for (int k = 0; k < NP; ++k) pr[k] = 0;
+    product(uv, xs2, pr);
+    load_D(dv);
     __syncthreads();
     reduce_rows(pr);
     __syncthreads();
     const T r_r = tid < nr ? xs[rc0 + tid] - row_value(tid) : T(0);
     if (!exchange(Rb, 2, r_r, xs2)) return;
-    apply_D(xs2);
+    apply(dv, xs2);
     __syncthreads();
     if (tid < nr) x += row_value(tid);
   }
   if (tid < nr) st_coh(y + oI + rc0 + tid, x);
   __builtin_amdgcn_s_waitcnt(0);
   __syncthreads();
-  if (tid == 0) __hip_atomic_fetch_add(parts + 3, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  if (tid == 0) {
+    // the superblock's last chunk to finish counts it solved
+    if (__hip_atomic_fetch_add(parts + 3, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == (unsigned)NCH - 1)
+      __hip_atomic_fetch_add(solved, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
 }
 
 // Clears the ticket and the superblocks' control lines; the solve's status
@@ -464,8 +537,8 @@ static int launch_sbsweep(bool fwd, bool trans, int which, const T* F, int64_t l
   // one refinement step in the U blocks of FP64 factors
   constexpr bool R64 = sizeof(T) == 8;
 #define RBFX_SBSWEEP(F_, T_, W_)                                                                              \
-  hipLaunchKernelGGL((sbsweep_kernel<T, F_, T_, W_, R64 && W_ == 1>), dim3(grid), dim3(256), 0, s, F, lda, n, \
-                     D512, y, P, ctl, ticks)
+  hipLaunchKernelGGL((sbsweep_kernel<T, F_, T_, W_, R64 && W_ == 1>), dim3(grid), dim3(kSweepThreads), 0, s, F, \
+                     lda, n, D512, y, P, ctl, ticks)
   if (fwd && !trans && which == 0) RBFX_SBSWEEP(true, false, 0);
   else if (!fwd && !trans && which == 1) RBFX_SBSWEEP(false, false, 1);
   else if (fwd && trans && which == 1) RBFX_SBSWEEP(true, true, 1);
