@@ -206,6 +206,20 @@ int rbfx_getrs_multi_f64(int64_t n, const double* d_LU, int64_t lda, const int64
  * expired (each gives up after about 4 s of real time; the solve then wrote
  * NaN into d_b instead of a partial result), else RBFX_OK.               */
 int rbfx_getrs_status(int64_t n, void* d_work, size_t work_bytes, void* stream);
+/* Allocation-free rbfx_getrs_f64: the same solve with the same kernels, its
+ * scratch (permuted vector, superblock values, control words) in a fixed
+ * device array of the library instead of a workspace.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * n is at most RBFX_GETRS_NOALLOC_MAX_N (larger n: RBFX_ERR_ARG, nothing
+ * queued); the array (about 2 MB) is held on every device the call runs on
+ * until the process ends.  All calls on one device share it: calls that
+ * overlap on one GPU race.  A device-side wait that expired leaves NaN in
+ * d_b (there is no status query for this variant).                        */
+#define RBFX_GETRS_NOALLOC_MAX_N 65536
+int rbfx_getrs_noalloc_f64(int trans, int64_t n, const double* d_LU, int64_t lda,
+                           const int64_t* d_perm, const double* d_dinv, double* d_b,
+                           void* stream);
 
 /* ---- K5 FP64 GEMV ---------------------------------------------------------
  * y = alpha op(A) x + beta y, op = A (trans=0) or A^T (trans=1); A m x n.   */

@@ -14,8 +14,10 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "_lib", "librbfx.so")
 
 RBFX_OK = 0
+RBFX_ERR_ARG = -1
 RBFX_ERR_TIMEOUT = -4
 RBFX_ERR_STALL = -5
+GETRS_NOALLOC_MAX_N = 65536  # RBFX_GETRS_NOALLOC_MAX_N of rbfx.h
 
 POLYHARMONIC, GAUSSIAN, THIN_PLATE, MULTIQUADRIC, INV_MULTIQUADRIC = 0, 1, 2, 3, 4
 VAL, DX, DY, LAP, DXX, DYY, DXY = range(7)
@@ -88,6 +90,7 @@ _SIGS = {
     "rbfx_getrs_f64": (ctypes.c_int, [ctypes.c_int, _i64, _p, _i64, _p, _p, _p, _p, _sz, _p]),
     "rbfx_getrs_f32": (ctypes.c_int, [ctypes.c_int, _i64, _p, _i64, _p, _p, _p, _p, _sz, _p]),
     "rbfx_getrs_status": (ctypes.c_int, [_i64, _p, _sz, _p]),
+    "rbfx_getrs_noalloc_f64": (ctypes.c_int, [ctypes.c_int, _i64, _p, _i64, _p, _p, _p, _p]),
     "rbfx_getrs_multi_workspace": (_sz, [_i64, _i64]),
     "rbfx_getrs_multi_f64": (ctypes.c_int, [_i64, _p, _i64, _p, _p, _i64, _p, _i64, _p, _sz, _p]),
     "rbfx_shutdown": (ctypes.c_int, []),

@@ -13,6 +13,8 @@
 #include <climits>
 #include <vector>
 
+#include <mutex>
+
 #include "common.h"
 
 namespace rbfx {
@@ -555,7 +557,7 @@ static int launch_sbsweep(bool fwd, bool trans, int which, const T* F, int64_t l
 
 // Workspace of one solve: the permuted vector, the superblocks' accumulated
 // values, then the control words (3 + one per superblock).
-static int64_t getrs_ctl_words(int64_t n) { return kCtlLine * (ceil_div(n, (int64_t)SB) + 1) + 8; }
+constexpr int64_t getrs_ctl_words(int64_t n) { return kCtlLine * ((n + SB - 1) / SB + 1) + 8; }
 // P holds three n-vectors: the accumulated values, and for the refined U
 // sweeps the first solutions and their residuals.
 static bool carve_getrs(void* work, size_t work_bytes, int64_t n, double*& y, double*& P, unsigned*& ctl) {
@@ -567,14 +569,12 @@ static bool carve_getrs(void* work, size_t work_bytes, int64_t n, double*& y, do
 }
 static int64_t dinv64_elems(int64_t n) { return ceil_div(n, TB) * 2 * TB * TB; }
 
+// The solve itself on carved scratch: y (n), P (3n), ctl (getrs_ctl_words).
+// Every scratch word that must start at zero is cleared by a kernel of the
+// solve (no memset), and nothing is allocated.
 template <typename T>
-static int getrs(int trans, int64_t n, const T* LU, int64_t lda, const int64_t* perm, const double* dinv,
-                 double* b, void* work, size_t work_bytes, hipStream_t s) {
-  RBFX_REQUIRE(n > 0 && lda >= n && LU && perm && dinv && b, "bad getrs args");
-  RBFX_REQUIRE(n < (int64_t)INT32_MAX / 2, "getrs: n too large");
-  double *y, *P;
-  unsigned* ctl;
-  RBFX_REQUIRE(carve_getrs(work, work_bytes, n, y, P, ctl), "getrs workspace too small");
+static int getrs_run(int trans, int64_t n, const T* LU, int64_t lda, const int64_t* perm, const double* dinv,
+                     double* b, double* y, double* P, unsigned* ctl, hipStream_t s) {
   const double* D512 = dinv + dinv64_elems(n);
   const unsigned g = (unsigned)ceil_div(n, 256);
   int st;
@@ -597,6 +597,18 @@ static int getrs(int trans, int64_t n, const T* LU, int64_t lda, const int64_t* 
   }
   return RBFX_OK;
 }
+
+template <typename T>
+static int getrs(int trans, int64_t n, const T* LU, int64_t lda, const int64_t* perm, const double* dinv,
+                 double* b, void* work, size_t work_bytes, hipStream_t s) {
+  RBFX_REQUIRE(n > 0 && lda >= n && LU && perm && dinv && b, "bad getrs args");
+  RBFX_REQUIRE(n < (int64_t)INT32_MAX / 2, "getrs: n too large");
+  double *y, *P;
+  unsigned* ctl;
+  RBFX_REQUIRE(carve_getrs(work, work_bytes, n, y, P, ctl), "getrs workspace too small");
+  return getrs_run<T>(trans, n, LU, lda, perm, dinv, b, y, P, ctl, s);
+}
+
 
 // ---- many right-hand sides -------------------------------------------------------
 __global__ void gather_rows_kernel(const double* __restrict__ B, int64_t ldb, const int64_t* __restrict__ perm,
@@ -753,6 +765,49 @@ extern "C" int rbfx_getrs_f64(int trans, int64_t n, const double* d_LU, int64_t 
                               const double* d_dinv, double* d_b, void* d_work, size_t work_bytes,
                               void* stream) {
   return getrs<double>(trans, n, d_LU, lda, d_perm, d_dinv, d_b, d_work, work_bytes, as_stream(stream));
+}
+
+namespace rbfx {
+// ---- allocation-free solve ------------------------------------------------------
+// Scratch of rbfx_getrs_noalloc_f64 for n <= RBFX_GETRS_NOALLOC_MAX_N, carved
+// like a getrs workspace: y (n doubles), P (3n doubles), control words.  One
+// copy per device.  Host code never uses the symbol itself: its device
+// address comes from hipGetSymbolAddress, once per device.
+constexpr int64_t kNoAllocMaxN = RBFX_GETRS_NOALLOC_MAX_N;
+constexpr size_t kNoAllocBytes = 4 * (((size_t)kNoAllocMaxN * 8 + 255) / 256 * 256) +
+                                 ((size_t)getrs_ctl_words(kNoAllocMaxN) * 4 + 255) / 256 * 256;
+static __device__ __align__(256) unsigned char scratch[kNoAllocBytes];
+static std::mutex g_scratch_mu;
+static void* g_scratch_ptr[64];
+
+static int scratch_address(void*& out) {
+  int dev = 0;
+  RBFX_HIP(hipGetDevice(&dev));
+  RBFX_REQUIRE(dev >= 0 && dev < 64, "device index %d out of range", dev);
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    RBFX_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    RBFX_REQUIRE(p != nullptr, "no device address for the solve's scratch array");
+    g_scratch_ptr[dev] = p;
+  }
+  out = g_scratch_ptr[dev];
+  return RBFX_OK;
+}
+}  // namespace rbfx
+
+extern "C" int rbfx_getrs_noalloc_f64(int trans, int64_t n, const double* d_LU, int64_t lda,
+                                      const int64_t* d_perm, const double* d_dinv, double* d_b, void* stream) {
+  RBFX_REQUIRE(n > 0 && lda >= n && d_LU && d_perm && d_dinv && d_b, "bad getrs args");
+  RBFX_REQUIRE(n <= kNoAllocMaxN, "rbfx_getrs_noalloc_f64: n = %lld exceeds its fixed scratch (n <= %lld)",
+               (long long)n, (long long)kNoAllocMaxN);
+  void* base = nullptr;
+  int st;
+  if ((st = scratch_address(base))) return st;
+  double *y, *P;
+  unsigned* ctl;
+  RBFX_REQUIRE(carve_getrs(base, kNoAllocBytes, n, y, P, ctl), "solve scratch too small");
+  return getrs_run<double>(trans, n, d_LU, lda, d_perm, d_dinv, d_b, y, P, ctl, as_stream(stream));
 }
 
 extern "C" int rbfx_getrs_status(int64_t n, void* d_work, size_t work_bytes, void* stream) {
