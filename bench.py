@@ -15,10 +15,14 @@ of the faster path whose values meet rel-L2 <= 1e-8 against the FP64 path
 (the FP64 path meets it by definition); both paths, the refinement statistics
 and the error of the mixed path are reported beside it.
 
+Each path is also timed without the gradient (forward solve only), so the
+gradient's cost relative to the forward solve is reported.
+
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--side 200]
                        [--precisions fp64,fp32_ir] [--copy-probe-mib M]
-                       [--gemm-replay] [--no-cpu-baseline] [--no-kernel-profile]
-                       [--support-size K]
+                       [--gemm-replay [f64|f32]] [--no-cpu-baseline] [--no-kernel-profile]
+                       [--support-size K] [--cpu-budget SECONDS] [--cpu-only]
+--cpu-only prints the CPU baseline alone (no GPU work), for long sample series.
 --copy-probe-mib runs one plain device copy of M MiB before the workload, so a
 counter run (FETCH_SIZE / WRITE_SIZE) can be scaled by a copy of known size.
 --gemm-replay only replays the trailing-update GEMM launches of the FP64 LU
@@ -112,32 +116,67 @@ def profile_step(fn):
     return out
 
 
-def cpu_baseline(seconds_budget=30.0):
-    """Restated reference algorithm (inv(A), D inv(A), QR) on a 60 x 60 sample of
-    the same workload, timed on this host, scaled by (n_40k / n_sample)^3."""
+CPU_SIDES = ((30, 20), (60, 60), (80, 80), (100, 100))
+
+
+def cpu_baseline(seconds_budget=30.0, n_full=40006):
+    """Restated reference algorithm on samples of the same workload, timed on
+    this host for the bench's unit (one solve + one source gradient).
+
+    Per sample (Nx x Ny nodes, the workload's boundary types): assembly of A
+    and D, inv(A), B = (D inv(A))[:, :N], Householder QR solve of B u = q and
+    coefficients inv(A) [u; 0] (operators.py:559-618), then the gradient's
+    transposed solve through the same QR (lambda = Q R^-T g: the reverse mode
+    of the reference's QR linear solve).  The two smallest samples always run;
+    larger ones while their n^3-predicted time fits the budget; the N=40k rate is the
+    largest sample's time scaled by (n_full / n)^3, labelled as extrapolated.
+    """
+    import scipy.linalg as sla
     sys.path.insert(0, HERE)
-    from oracle import assembly as OA, cloud as OC, rbf as R, solver as OS
-    side = 60
+    from oracle import assembly as OA, cloud as OC, rbf as R
     ft = {"North": "d", "South": "n", "East": "d", "West": "d"}
-    oc = OC.SquareCloud(side, side, ft)
-    rng = np.random.default_rng(0)
-    x, y = oc.sorted_nodes.T
-    q = np.where(np.arange(oc.N) < oc.Ni, -np.sin(np.pi * x) * np.sin(np.pi * y), rng.standard_normal(oc.N) * 0.1)
     desc = {"rbf": [0, 0, 0, 1, 0, 0, 0], "mon": [0, 0, 0, 1, 0, 0, 0]}
-    times = []
-    t_start = time.time()
-    while len(times) < 3 and (time.time() - t_start) < seconds_budget:
-        t0 = time.perf_counter()
-        OS.pde_solver_bpath(desc, oc, R.POLYHARMONIC, 1.0, 6, q)
-        times.append(time.perf_counter() - t0)
-    t = float(np.median(times))
-    n_s, n_full = oc.N + 6, 40006
-    scale = (n_full / n_s) ** 3
+    points = []
+    spent = 0.0
+    for nx, ny in CPU_SIDES:
+        oc = OC.SquareCloud(nx, ny, ft)
+        n = oc.N + 6
+        reps = 2 if len(points) < 2 else 1
+        if len(points) >= 2:  # the smallest sample is dominated by per-node work, not n^3
+            pred = points[-1]["s_per_step"] * (n / points[-1]["n"]) ** 3 * reps
+            if spent + pred > seconds_budget:
+                break
+        rng = np.random.default_rng(0)
+        x, y = oc.sorted_nodes.T
+        q = np.where(np.arange(oc.N) < oc.Ni, -np.sin(np.pi * x) * np.sin(np.pi * y), rng.standard_normal(oc.N) * 0.1)
+        interior = np.arange(oc.N) < oc.Ni
+        times = []
+        for _ in range(reps):
+            t0 = time.perf_counter()
+            # forward: the steps of OS.pde_solver_bpath, with the QR kept for the reverse mode
+            D = OA.assemble_D(desc, oc, R.POLYHARMONIC, 1.0, 6)
+            invA = np.linalg.inv(OA.assemble_A(oc, R.POLYHARMONIC, 1.0, 6))
+            B = (D @ invA)[:, :oc.N]
+            Q, Rm = np.linalg.qr(B)
+            u = sla.solve_triangular(Rm, Q.T @ q)
+            c = invA @ np.concatenate([u, np.zeros(6)])
+            # reverse: dL/dq for L = 1/2 mean(u_interior^2)
+            g = np.where(interior, u / oc.Ni, 0.0)
+            lam = Q @ sla.solve_triangular(Rm, g, trans="T")
+            times.append(time.perf_counter() - t0)
+            spent += times[-1]
+        assert np.all(np.isfinite(lam)) and np.all(np.isfinite(c))
+        points.append({"N": oc.N, "n": n, "s_per_step": float(min(times))})
+    last = points[-1]
+    scale = (n_full / last["n"]) ** 3
     cores = len(os.sched_getaffinity(0))
-    return {"value": 1.0 / (t * scale), "unit": "solves/s", "cores": cores, "kind": "port",
+    return {"value": 1.0 / (last["s_per_step"] * scale), "unit": "solves/s", "cores": cores, "kind": "port",
+            "extrapolated": True, "points": points,
             "sample": "restated reference B-path (numpy/OpenBLAS FP64: assembly, inv(A), D@inv(A), QR solve, "
-                      "coefficients) on the same workload at 60x60 (N=3,600), median of %d runs = %.3f s, "
-                      "scaled to N=40,000 by (40006/3606)^3 = %.0f" % (len(times), t, scale)}
+                      "coefficients) plus the source gradient's transposed QR solve, measured at N = %s "
+                      "(best of 2 below N=6,400, one run above); the N=40,000 value is the N=%d time scaled by (%d/%d)^3 = %.0f "
+                      "(extrapolated, not measured)"
+                      % (", ".join(str(p["N"]) for p in points), last["N"], n_full, last["n"], scale)}
 
 
 def measured_traffic(kernel, side):
@@ -153,7 +192,7 @@ def measured_traffic(kernel, side):
     return None
 
 
-def gemm_replay(n, nb=512):
+def gemm_replay(n, nb=512, dtype="f64"):
     """The trailing-update GEMM launches of the blocked LU of an n x n matrix
     (panel width nb): for every panel step, the update of the next panel's
     columns and of the rest, on the same sub-blocks of one row-major matrix
@@ -161,14 +200,16 @@ def gemm_replay(n, nb=512):
     from updes_amd import _lib as L, engine as E
     lib = L.lib()
     ld = E._ld(n)
-    M = torch.randn(n, ld, dtype=torch.float64, device="cuda") * 1e-3
+    tdt = torch.float64 if dtype == "f64" else torch.float32
+    M = torch.randn(n, ld, dtype=tdt, device="cuda") * 1e-3
     base = M.data_ptr()
-    el = 8
+    el = M.element_size()
+    fn = lib.rbfx_gemm_f64 if dtype == "f64" else lib.rbfx_gemm_f32
 
     def gemm(m_, n_, k_, a_r, a_c, b_r, b_c, c_r, c_c):
-        L.check(lib.rbfx_gemm_f64(m_, n_, k_, -1.0, ctypes.c_void_p(base + el * (a_r * ld + a_c)), ld,
-                                  ctypes.c_void_p(base + el * (b_r * ld + b_c)), ld, 1.0,
-                                  ctypes.c_void_p(base + el * (c_r * ld + c_c)), ld, L.stream()))
+        L.check(fn(m_, n_, k_, -1.0, ctypes.c_void_p(base + el * (a_r * ld + a_c)), ld,
+                   ctypes.c_void_p(base + el * (b_r * ld + b_c)), ld, 1.0,
+                   ctypes.c_void_p(base + el * (c_r * ld + c_c)), ld, L.stream()))
 
     count = 0
     for k0 in range(0, n, nb):
@@ -183,7 +224,7 @@ def gemm_replay(n, nb=512):
             gemm(rest, rest - Wn, W, k1, k0, k0, k1 + Wn, k1, k1 + Wn)
         count += 1
     torch.cuda.synchronize()
-    print("[bench] gemm replay: n=%d, %d panel steps" % (n, count), file=sys.stderr, flush=True)
+    print("[bench] gemm replay (%s): n=%d, %d panel steps" % (dtype, n, count), file=sys.stderr, flush=True)
 
 
 def main():
@@ -198,15 +239,21 @@ def main():
     ap.add_argument("--no-kernel-profile", action="store_true")
     ap.add_argument("--support-size", default="max",
                     help="local support size K (< N) or 'max' (every node, the default)")
-    ap.add_argument("--gemm-replay", action="store_true",
-                    help="run only the FP64 factorisation's GEMM launches (same shapes and operand blocks) "
-                         "on one matrix, for counter runs of the GEMM alone")
+    ap.add_argument("--gemm-replay", nargs="?", const="f64", choices=["f64", "f32"],
+                    help="run only the factorisation's GEMM launches (same shapes and operand blocks; FP64 "
+                         "by default, or f32) on one matrix, for counter runs of the GEMM alone")
+    ap.add_argument("--cpu-budget", type=float, default=30.0,
+                    help="seconds of host work for the CPU baseline's samples")
+    ap.add_argument("--cpu-only", action="store_true", help="print the CPU baseline alone and exit")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: the dense solve of one collocation system runs on a single MI355X, "
               "and there is no multi-GPU path to measure.", file=sys.stderr)
         sys.exit(2)
 
+    if args.cpu_only:
+        print(json.dumps({"cpu_baseline": cpu_baseline(args.cpu_budget)}))
+        return
     import updes_amd as u
     from updes_amd import engine as E
     from updes_amd import _lib as L
@@ -222,7 +269,7 @@ def main():
         print("[bench] copy probe: %d bytes read, %d bytes written" % (8 * nel, 8 * nel), file=sys.stderr, flush=True)
         del src, dst
     if args.gemm_replay:
-        gemm_replay(args.side * args.side + 6)
+        gemm_replay(args.side * args.side + 6, dtype=args.gemm_replay)
         return
     support = args.support_size if args.support_size == "max" else int(args.support_size)
     t_cloud = time.perf_counter()
@@ -246,8 +293,16 @@ def main():
             stats = u.last_solve_stats()
         torch.cuda.synchronize()
         dt = (time.perf_counter() - t0) / args.steps
-        results[prec] = dict(ms=dt * 1e3, vals=vals, grad=g, loss=float(loss), stats=stats)
-        print("[bench] %s: %.1f ms per solve+grad, stats %s" % (prec, dt * 1e3, stats), file=sys.stderr, flush=True)
+        # the forward solve alone, for the gradient's cost relative to it
+        t0 = time.perf_counter()
+        for s in range(args.steps):
+            u.clear_cache()
+            solve_and_grad(cloud, bcs, b, basis, prec, want_grad=False)
+        torch.cuda.synchronize()
+        fwd = (time.perf_counter() - t0) / args.steps
+        results[prec] = dict(ms=dt * 1e3, fwd_ms=fwd * 1e3, vals=vals, grad=g, loss=float(loss), stats=stats)
+        print("[bench] %s: %.1f ms per solve+grad, %.1f ms per solve, stats %s" % (prec, dt * 1e3, fwd * 1e3, stats),
+              file=sys.stderr, flush=True)
 
     # kernel-level profile of one solve+gradient per path (outside the timed loop)
     prof = {}
@@ -281,6 +336,8 @@ def main():
                      "precision": head, "factorisation_used": used, "grads_per_s": 1e3 / r["ms"]}
     for prec, rr in results.items():
         d = {"solves_per_s": 1e3 / rr["ms"], "grads_per_s": 1e3 / rr["ms"], "ms_per_solve_plus_grad": rr["ms"],
+             "ms_per_solve_forward_only": rr["fwd_ms"], "step_over_forward": rr["ms"] / rr["fwd_ms"],
+             "gradient_over_forward": (rr["ms"] - rr["fwd_ms"]) / rr["fwd_ms"],
              "ir": rr["stats"], "meets_1e-8": verified[prec]}
         if "err" in rr:
             d["rel_l2_vals_vs_fp64"] = rr["err"]
@@ -309,7 +366,7 @@ def main():
                               for k, v in pr.items()} for p, pr in prof.items()}
     if not args.no_cpu_baseline:
         print("[bench] cpu baseline ...", file=sys.stderr, flush=True)
-        out["cpu_baseline"] = cpu_baseline()
+        out["cpu_baseline"] = cpu_baseline(args.cpu_budget)
     print(json.dumps(out))
 
 
