@@ -269,7 +269,10 @@ def main():
         print("[bench] copy probe: %d bytes read, %d bytes written" % (8 * nel, 8 * nel), file=sys.stderr, flush=True)
         del src, dst
     if args.gemm_replay:
-        gemm_replay(args.side * args.side + 6, dtype=args.gemm_replay)
+        # the FP64 LU factors K (n = N + 6); the FP32 one factors the N x N
+        # matrix of the cardinal basis
+        nrep = args.side * args.side + (6 if args.gemm_replay == "f64" else 0)
+        gemm_replay(nrep, dtype=args.gemm_replay)
         return
     support = args.support_size if args.support_size == "max" else int(args.support_size)
     t_cloud = time.perf_counter()
