@@ -359,6 +359,15 @@ int rbfx_set_panel_groups(int max_groups);
  * value disables them).  No counterpart in the reference.                  */
 int rbfx_set_narrow_leaf_rows(int64_t rows);
 
+/* ---- test hook --------------------------------------------------------------
+ * Real-time bound of every device-side wait launched after the call
+ * (process-wide, seconds; a negative value restores the default, 4 s).  A
+ * bound of 0 makes every wait that is not satisfied at once expire, so the
+ * failure handling of the solves (NaN result, RBFX_ERR_TIMEOUT) can be
+ * exercised from the host while every kernel still exits normally.  No
+ * counterpart in the reference.                                            */
+int rbfx_set_wait_seconds(double seconds);
+
 #ifdef __cplusplus
 }
 #endif

@@ -165,3 +165,35 @@ def test_getrf_matches_lapack_pivots():
     lu_ref, piv_ref = sla.lu_factor(a)
     assert np.array_equal(lu.ipiv.cpu().numpy(), piv_ref)
     assert np.abs(lu.a[:, :n].cpu().numpy() - lu_ref).max() < 1e-10
+
+
+def test_expired_sweep_wait_fails_fast_and_the_next_solve_recovers():
+    """Failure handling of the triangular sweeps, forced from the host: with a
+    wait bound of 0 the waits that are not satisfied at once expire, the
+    solve's kernels exit, NaN is written, and LU.solve raises RbfxError
+    (RBFX_ERR_TIMEOUT) within seconds.  With the default bound restored the
+    same factors solve normally (each solve clears its status word)."""
+    import time
+    from updes_amd import engine as E
+    L = _lib()
+    lib = L.lib()
+    n = 20000
+    a = _random_well(n, 11)
+    m = torch.zeros((n, E._ld(n)), dtype=torch.float32)
+    m[:, :n] = torch.as_tensor(a, dtype=torch.float32)
+    lu = E.LU(m.cuda(), n)
+    b = torch.as_tensor(np.random.default_rng(3).standard_normal(n), device="cuda")
+    L.check(lib.rbfx_set_wait_seconds(0.0))
+    try:
+        t0 = time.perf_counter()
+        with pytest.raises(L.RbfxError) as ei:
+            lu.solve(b)
+        assert ei.value.status == L.RBFX_ERR_TIMEOUT
+        assert time.perf_counter() - t0 < 10.0
+    finally:
+        L.check(lib.rbfx_set_wait_seconds(-1.0))
+    x = lu.solve(b).cpu().numpy()
+    aa = a.astype(np.float32).astype(np.float64)
+    res = aa @ x - b.cpu().numpy()
+    assert np.isfinite(x).all()
+    assert np.abs(res).max() < 1e-3 * np.linalg.norm(aa, 1) * np.linalg.norm(x, 1) * n / 100

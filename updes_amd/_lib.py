@@ -91,6 +91,7 @@ _SIGS = {
     "rbfx_getrs_f32": (ctypes.c_int, [ctypes.c_int, _i64, _p, _i64, _p, _p, _p, _p, _sz, _p]),
     "rbfx_getrs_status": (ctypes.c_int, [_i64, _p, _sz, _p]),
     "rbfx_getrs_noalloc_f64": (ctypes.c_int, [ctypes.c_int, _i64, _p, _i64, _p, _p, _p, _p]),
+    "rbfx_set_wait_seconds": (ctypes.c_int, [ctypes.c_double]),
     "rbfx_getrs_multi_workspace": (_sz, [_i64, _i64]),
     "rbfx_getrs_multi_f64": (ctypes.c_int, [_i64, _p, _i64, _p, _p, _i64, _p, _i64, _p, _sz, _p]),
     "rbfx_shutdown": (ctypes.c_int, []),

@@ -1,5 +1,6 @@
 // Library-level entry points: ABI version, error reporting, device queries.
 #include <stdarg.h>
+#include <atomic>
 #include <stdio.h>
 
 #include "common.h"
@@ -18,6 +19,9 @@ void set_error(const char* fmt, ...) {
 const char* last_error() { return g_err; }
 
 static DeviceInfo g_info[64];
+static int g_clock_khz[64];
+// Wait bound in seconds (rbfx_set_wait_seconds); negative: kWaitSeconds.
+static std::atomic<double> g_wait_seconds{-1.0};
 
 const DeviceInfo& device_info() {
   int dev = 0;
@@ -35,9 +39,11 @@ const DeviceInfo& device_info() {
     int khz = 0;
     if (hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev) != hipSuccess || khz <= 0)
       khz = 100000;  // CDNA real-time counter: 100 MHz
-    d.wait_ticks = (uint64_t)(kWaitSeconds * 1e3 * khz);
+    g_clock_khz[dev] = khz;
     d.device = dev;
   }
+  const double secs = g_wait_seconds.load(std::memory_order_relaxed);
+  d.wait_ticks = (uint64_t)((secs < 0.0 ? kWaitSeconds : secs) * 1e3 * g_clock_khz[dev]);
   return d;
 }
 
@@ -46,6 +52,11 @@ const DeviceInfo& device_info() {
 extern "C" int rbfx_abi_version(void) { return RBFX_ABI_VERSION; }
 
 extern "C" const char* rbfx_last_error(void) { return rbfx::last_error(); }
+
+extern "C" int rbfx_set_wait_seconds(double seconds) {
+  rbfx::g_wait_seconds.store(seconds, std::memory_order_relaxed);
+  return RBFX_OK;
+}
 
 extern "C" int rbfx_init(void) {
   int n = 0;
