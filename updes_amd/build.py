@@ -19,7 +19,7 @@ LIBDIR = os.path.join(HERE, "_lib")
 LIB = os.path.join(LIBDIR, "librbfx.so")
 ARCH = os.environ.get("RBFX_OFFLOAD_ARCH", "gfx950")
 
-SOURCES = ["api.cpp", "assembly.hip", "eval.hip", "lu.hip", "trsv.hip", "krylov.hip", "cardinal.hip"]
+SOURCES = ["api.cpp", "host_logic.cpp", "assembly.hip", "eval.hip", "lu.hip", "trsv.hip", "krylov.hip", "cardinal.hip"]
 
 # Packed FP32 instructions are disabled for the device code (the host pass
 # warns that it does not know the feature; that warning is harmless).

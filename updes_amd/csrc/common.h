@@ -6,6 +6,7 @@
 #include <string>
 
 #include "../../include/rbfx.h"
+#include "host_logic.h"
 
 namespace rbfx {
 
@@ -57,22 +58,6 @@ inline hipStream_t as_stream(void* s) { return reinterpret_cast<hipStream_t>(s);
 
 __host__ __device__ inline int64_t ceil_div(int64_t a, int64_t b) { return (a + b - 1) / b; }
 
-// Bump allocator over a caller-provided workspace (256-byte aligned slices).
-struct Workspace {
-  char* base;
-  size_t size;
-  size_t used = 0;
-  Workspace(void* p, size_t n) : base(static_cast<char*>(p)), size(n) {}
-  template <typename T>
-  T* take(size_t count) {
-    size_t off = (used + 255) & ~size_t(255);
-    size_t bytes = count * sizeof(T);
-    if (off + bytes > size) return nullptr;
-    used = off + bytes;
-    return reinterpret_cast<T*>(base + off);
-  }
-};
-inline size_t ws_round(size_t bytes) { return (bytes + 255) & ~size_t(255); }
 
 // Every device-side wait gives up after kWaitSeconds of real time, read from
 // the constant-rate counter (wall_clock64), whatever each poll costs; it

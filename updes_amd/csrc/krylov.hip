@@ -325,13 +325,12 @@ int fgmres_ir(PairOp& pre, ResidualOp& res, int64_t n, const double* d_b, double
   hipLaunchKernelGGL(scale_copy_kernel, dim3(g), dim3(256), 0, s, n, 1.0, d_b, r);
   RBFX_CHECK_LAUNCH();
 
-  std::vector<double> H((size_t)(m + 1) * m), cs(m), sn(m), gv(m + 1), hh(2 * m + 8), y(m);
+  HessenbergLS ls(m);
+  std::vector<double> hh(2 * m + 8), col(m + 2);
   // RBFX_IR_TRACE=1: one stderr line per cycle with the Arnoldi estimates
   static const bool trace = getenv("RBFX_IR_TRACE") && getenv("RBFX_IR_TRACE")[0] == '1';
   std::string tline;
-  double best = INFINITY;
-  int noprog = 0;
-  bool est_met = false;  // the last cycle's Arnoldi estimate reached its target
+  IrMonitor mon{o.tol, kStopEta, kFloorEta};
   int result = RBFX_ERR_STALL;
   for (int cycle = 0;; ++cycle) {
     if (cycle > 0) {
@@ -354,13 +353,8 @@ int fgmres_ir(PairOp& pre, ResidualOp& res, int64_t n, const double* d_b, double
                loc.inner_iters);
       tline = buf;
     }
-    if (!(rel == rel)) break;  // NaN
-    if (rel <= o.tol) {
-      result = RBFX_OK;
-      break;
-    }
     double eta = INFINITY;
-    if (cycle > 0 && o.knorm > 0.0) {
+    if (rel == rel && rel > o.tol && cycle > 0 && o.knorm > 0.0) {
       double xn2 = 0.0;
       if ((st = host_dot(d_x, d_x, xn2))) return st;
       eta = rn / (o.knorm * sqrt(xn2) + bn);
@@ -369,35 +363,23 @@ int fgmres_ir(PairOp& pre, ResidualOp& res, int64_t n, const double* d_b, double
         snprintf(buf, sizeof(buf), " (eta %.1e)", eta);
         tline += buf;
       }
-      if (eta <= kStopEta) {
-        result = RBFX_OK;
-        break;
-      }
     }
-    if (rel <= 0.5 * best) {
-      best = rel;
-      noprog = 0;
-    } else {
-      ++noprog;
-      // the Arnoldi estimate met its target, the true residual did not
-      // follow: at the FP64 floor if the backward error is small
-      if (est_met && eta <= kFloorEta) {
-        result = RBFX_OK;
-        break;
-      }
-      if (noprog >= 3) break;
+    const IrMonitor::Decision dec = mon.next(rel, eta);
+    if (dec == IrMonitor::kConverged) {
+      result = RBFX_OK;
+      break;
     }
+    if (dec == IrMonitor::kStalled) break;
     if (loc.inner_iters >= o.max_inner) break;
 
     // GMRES(m) on K M^-1 y = r, flexible form: Z keeps M^-1 V
     hipLaunchKernelGGL(scale_copy_kernel, dim3(g), dim3(256), 0, s, n, 1.0 / rn, r, V);
     RBFX_CHECK_LAUNCH();
-    std::fill(gv.begin(), gv.end(), 0.0);
-    gv[0] = rn;
+    ls.start(rn);
     const double inner_tol = std::max(0.1 * o.tol * bn, 1e-10 * rn);
     int kdone = 0;
     bool early_stall = false;
-    est_met = false;
+    mon.est_met = false;
     for (int k = 0; k < m; ++k) {
       double* vk = V + (size_t)k * n;
       if ((st = pre.apply(vk, Z + (size_t)k * n, w))) return st;
@@ -412,7 +394,6 @@ int fgmres_ir(PairOp& pre, ResidualOp& res, int64_t n, const double* d_b, double
       RBFX_HIP(hipMemcpyAsync(hh.data(), hd, (2 * m + 6) * sizeof(double), hipMemcpyDeviceToHost, s));
       RBFX_HIP(hipStreamSynchronize(s));
       const double hn = sqrt(hh[2 * m + 4]);
-      double* col = &H[(size_t)k * (m + 1)];
       for (int i = 0; i <= k; ++i) col[i] = hh[i] + hh[m + 1 + i];
       col[k + 1] = hn;
       ++loc.inner_iters;
@@ -421,48 +402,27 @@ int fgmres_ir(PairOp& pre, ResidualOp& res, int64_t n, const double* d_b, double
         hipLaunchKernelGGL(scale_copy_kernel, dim3(g), dim3(256), 0, s, n, 1.0 / hn, w, V + (size_t)(k + 1) * n);
         RBFX_CHECK_LAUNCH();
       }
-      // Givens rotations
-      for (int i = 0; i < k; ++i) {
-        const double t = cs[i] * col[i] + sn[i] * col[i + 1];
-        col[i + 1] = -sn[i] * col[i] + cs[i] * col[i + 1];
-        col[i] = t;
-      }
-      const double den = hypot(col[k], col[k + 1]);
-      cs[k] = den > 0.0 ? col[k] / den : 1.0;
-      sn[k] = den > 0.0 ? col[k + 1] / den : 0.0;
-      col[k] = den;
-      col[k + 1] = 0.0;
-      gv[k + 1] = -sn[k] * gv[k];
-      gv[k] = cs[k] * gv[k];
+      const double est = ls.add_column(k, col.data());
       if (trace) {
         char buf[24];
-        snprintf(buf, sizeof(buf), " %.1e", fabs(gv[k + 1]) / bn);
+        snprintf(buf, sizeof(buf), " %.1e", est / bn);
         tline += buf;
       }
-      if (fabs(gv[k + 1]) <= inner_tol || hn == 0.0) {
-        est_met = true;
+      if (est <= inner_tol || hn == 0.0) {
+        mon.est_met = true;
         break;
       }
       if (loc.inner_iters >= o.max_inner) break;
       // Early stall: extrapolate the observed geometric rate of the residual
       // estimate; give up when it cannot reach the tolerance in the budget.
-      if (k + 1 >= kRateWindow) {
-        const double est = fabs(gv[k + 1]);
-        const double rate = pow(est / rn, 1.0 / (k + 1));
-        const double need = (rate < 1.0 && rate > 0.0) ? log(inner_tol / est) / log(rate) : 1e30;
-        if (loc.inner_iters + need > (double)o.max_inner) {
-          early_stall = true;
-          break;
-        }
+      if (k + 1 >= kRateWindow && loc.inner_iters + iterations_needed(rn, est, k + 1, inner_tol) > (double)o.max_inner) {
+        early_stall = true;
+        break;
       }
     }
     // y = R^-1 g ; x += Z y
-    for (int i = kdone - 1; i >= 0; --i) {
-      double t = gv[i];
-      for (int j = i + 1; j < kdone; ++j) t -= H[(size_t)j * (m + 1) + i] * y[j];
-      y[i] = H[(size_t)i * (m + 1) + i] != 0.0 ? t / H[(size_t)i * (m + 1) + i] : 0.0;
-    }
-    RBFX_HIP(hipMemcpyAsync(hd, y.data(), kdone * sizeof(double), hipMemcpyHostToDevice, s));
+    const double* y = ls.solve(kdone);
+    RBFX_HIP(hipMemcpyAsync(hd, y, kdone * sizeof(double), hipMemcpyHostToDevice, s));
     if ((st = gemv(1, kdone, n, 1.0, Z, n, hd, 0.0, u, part, s))) return st;
     hipLaunchKernelGGL(axpby_kernel, dim3(g), dim3(256), 0, s, n, 1.0, u, 1.0, d_x);
     RBFX_CHECK_LAUNCH();

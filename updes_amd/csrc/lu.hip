@@ -1066,12 +1066,12 @@ static int leaf(LuCtx<T>& c, int64_t r0, int w) {
       std::min<int64_t>(cap > 0 ? std::min(cap, di.num_cus) : di.num_cus, kKeyMaxGroups - 1);
   // As few workgroups as one row per thread allows (every column's exchange
   // polls all G records); more rows per thread only when the CUs run out.
-  const int rows = (int)ceil_div(m, (int64_t)kPanelThreads * gmax);
-  RBFX_REQUIRE(rows <= kPanelMaxRows, "matrix too large for the register-resident panel (m=%lld)",
-               (long long)m);
-  int G = (int)ceil_div(m, (int64_t)kPanelThreads * rows);
-  const int64_t R = ceil_div(m, G);
-  G = (int)ceil_div(m, R);
+  LeafShape shape;
+  RBFX_REQUIRE(leaf_shape(m, gmax, kPanelThreads, kPanelMaxRows, shape),
+               "matrix too large for the register-resident panel (m=%lld)", (long long)m);
+  const int rows = shape.rows;
+  const int G = shape.G;
+  const int64_t R = shape.R;
   PanelArgs<T> a;
   a.A = c.A + r0 * c.lda + r0;
   a.lda = c.lda;
