@@ -145,6 +145,33 @@ int rbfx_eval(int kernel, double param, int64_t N, int64_t M, const double* d_cx
               const double* d_px, const double* d_py, uint32_t ops_mask,
               double* d_out, void* stream);
 
+/* ---- right-hand side q ------------------------------------------------------
+ * One term of the interior right-hand side: weight * w_i * op(field)(x_i),
+ * the field given by its N + M RBF and monomial coefficients (as the
+ * coefficient solve returns them), w_i = d_weight_row[i] or 1.            */
+typedef struct rbfx_rhs_term {
+  int32_t op;                  /* RBFX_VAL .. RBFX_DXY                        */
+  int32_t reserved;            /* 0                                           */
+  double weight;
+  const double* d_weight_row;  /* (Ni) per-row factor, or NULL               */
+  const double* d_coeffs;      /* (N + M) coefficients of the field          */
+} rbfx_rhs_term;
+
+/* The right-hand side [q; 0] of the augmented system (N + M entries, the
+ * node order of p): interior rows i < Ni get constant + d_const_row[i] (or
+ * 0 when NULL) + the sum of the terms at x_i (the lowered rhs_operator with
+ * its fields); boundary rows Ni <= i < N get d_bc[i - Ni] (the boundary
+ * values in node order, a Robin facet's value the first element of its
+ * tuple); the M monomial rows get 0.  As in the reference, the values of
+ * periodic facets are taken as given: pde_solver zeroes them beforehand
+ * (updes/operators.py:546-555).  p needs N, M, Ni..Npd, kernel, param, d_x
+ * and d_y.  Workspace: rbfx_rhs_workspace(p) bytes (none without terms).
+ * Replaces assemble_q (updes/assembly.py:434-485).                         */
+size_t rbfx_rhs_workspace(const rbfx_problem* p);
+int rbfx_rhs(const rbfx_problem* p, double constant, const double* d_const_row, int nterms,
+             const rbfx_rhs_term* terms, const double* d_bc, double* d_q, void* d_work,
+             size_t work_bytes, void* stream);
+
 /* Transpose of rbfx_eval for one operator t (adjoint of the evaluation):
  * out[j] = sum_i w_i op_t phi(p_i, c_j)  (j < N),
  * out[N+k] = sum_i w_i op_t p_k(p_i)     (k < M).  No reference counterpart

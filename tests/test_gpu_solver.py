@@ -296,3 +296,60 @@ def test_assemble_B_and_steady_sol_mat_match_oracle(nx, ny, ft, deg):
         bcs.update({f: (0.0, torch.as_tensor(betas[f])) for f, t in uc.facet_types.items() if t == "r"})
     sol = u.pde_solver(_laplace_op, lambda x, centers, rbf, fields: 1.0, uc, bcs, u.polyharmonic, deg)
     np.testing.assert_allclose(sol.mat.cpu().numpy(), B, rtol=0, atol=1e-12 * np.abs(B).max())
+
+
+@pytest.mark.parametrize("fs", [2, 3, 5])
+def test_rhs_abi_matches_oracle(fs):
+    """rbfx_rhs through the C ABI: interior rows = constant + row array +
+    weighted operator values of fields given by their coefficients (the
+    oracle's coefficients, so the check isolates the right-hand side), the
+    boundary rows the given values in node order, the monomial rows 0
+    (assembly.py:434-485); and the package's assemble_q (which now assembles
+    through it) against the oracle's, with Robin tuples and periodic facets."""
+    import updes_amd as u
+    from updes_amd import engine as E, _lib as L
+    from updes_amd.operators import assemble_q
+    facets = C.FACET_SETS[fs]
+    nx, ny = 11, 10
+    oc = OC.SquareCloud(nx, ny, facets)
+    uc = u.SquareCloud(Nx=nx, Ny=ny, facet_types=facets)
+    X = oc.sorted_nodes
+    M = 6
+    f1 = np.sin(2 * X[:, 0]) + X[:, 1] ** 2
+    f2 = np.cos(X[:, 1]) * X[:, 0]
+    c1 = OA.core_compute_coefficients(f1, oc, R.POLYHARMONIC, 1.0, M)
+    c2 = OA.core_compute_coefficients(f2, oc, R.POLYHARMONIC, 1.0, M)
+    xi = X[:oc.Ni]
+    row = 1.0 + xi[:, 0]
+    crow = np.sin(xi[:, 0])
+    ops1 = OS.eval_ops(xi, c1, X, R.POLYHARMONIC, 1.0)
+    ops2 = OS.eval_ops(xi, c2, X, R.POLYHARMONIC, 1.0)
+    bc = np.random.default_rng(fs).standard_normal(oc.N - oc.Ni)
+    expect = np.concatenate([0.7 + crow + 2.0 * row * ops1[R.LAP] - 0.5 * ops1[R.DX] + ops2[R.VAL], bc, np.zeros(M)])
+    prob = E.Problem(E.device_cloud(uc), L.POLYHARMONIC, 1.0, M)
+    dev = torch.device("cuda")
+    t = lambda a: torch.as_tensor(np.ascontiguousarray(a), dtype=torch.float64, device=dev)
+    terms = [(L.LAP, 2.0, t(row), t(c1)), (L.DX, -0.5, None, t(c1)), (L.VAL, 1.0, None, t(c2))]
+    q = E.rhs(prob, t(0.7 + crow), t(bc), terms).cpu().numpy()
+    assert np.abs(q - expect).max() <= 1e-12 * np.abs(expect).max()
+    # the package's assemble_q (callables and arrays; Robin values as
+    # pde_solver passes them after splitting the (value, beta) tuples)
+    fd = lambda p: torch.sin(torch.pi * p[0]) + p[1]
+    bcs, arrs = {}, {}
+    for f, ty in facets.items():
+        pts = X[oc.facet_nodes[f]]
+        if ty == "d":
+            bcs[f], arrs[f] = fd, np.sin(np.pi * pts[:, 0]) + pts[:, 1]
+        elif ty == "r":
+            v = pts[:, 1] - 0.3
+            bcs[f], arrs[f] = torch.as_tensor(v), v
+        elif ty == "n":
+            v = np.linspace(-1, 1, len(pts))
+            bcs[f], arrs[f] = torch.as_tensor(v), v
+        else:
+            # periodic facets: zeros, as pde_solver passes them (operators.py:546-555)
+            bcs[f], arrs[f] = 0.0, np.zeros(len(pts))
+    rhs = lambda x, centers=None, rbf=None, fields=None: x[0] * x[1] + 1.0
+    got = assemble_q(rhs, bcs, uc, u.polyharmonic, M, None).cpu().numpy()
+    ref = OA.assemble_q(oc, xi[:, 0] * xi[:, 1] + 1.0, arrs)
+    assert np.abs(got - ref).max() <= 1e-14 * np.abs(ref).max()

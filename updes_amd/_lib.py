@@ -65,6 +65,11 @@ class Cardinal(ctypes.Structure):
                 ("d_ptp_inv", _p), ("d_union_ptr", _p), ("d_union_idx", _p), ("d_union_pos", _p)]
 
 
+class RhsTerm(ctypes.Structure):
+    """Mirror of ``rbfx_rhs_term``."""
+    _fields_ = [("op", _i32), ("reserved", _i32), ("weight", _d), ("d_weight_row", _p), ("d_coeffs", _p)]
+
+
 class ProfEntry(ctypes.Structure):
     """Mirror of ``rbfx_prof_entry``."""
     _fields_ = [("name", ctypes.c_char * 48), ("launches", _i64), ("total_ms", _d), ("flops", _d), ("bytes", _d)]
@@ -92,6 +97,9 @@ _SIGS = {
     "rbfx_getrs_status": (ctypes.c_int, [_i64, _p, _sz, _p]),
     "rbfx_getrs_noalloc_f64": (ctypes.c_int, [ctypes.c_int, _i64, _p, _i64, _p, _p, _p, _p]),
     "rbfx_set_wait_seconds": (ctypes.c_int, [ctypes.c_double]),
+    "rbfx_rhs_workspace": (_sz, [ctypes.POINTER(Problem)]),
+    "rbfx_rhs": (ctypes.c_int, [ctypes.POINTER(Problem), _d, _p, ctypes.c_int, ctypes.POINTER(RhsTerm), _p, _p, _p,
+                                _sz, _p]),
     "rbfx_getrs_multi_workspace": (_sz, [_i64, _i64]),
     "rbfx_getrs_multi_f64": (ctypes.c_int, [_i64, _p, _i64, _p, _p, _i64, _p, _i64, _p, _sz, _p]),
     "rbfx_shutdown": (ctypes.c_int, []),

@@ -79,6 +79,22 @@ class EvalOp(torch.autograd.Function):
         return E.eval_t(kind, param, centers, M, points, op, g), None, None, None, None, None
 
 
+class RhsAssemble(torch.autograd.Function):
+    """q (N,) from the interior values q_int (Ni,) and the boundary values bc
+    (N - Ni,) in node order, through rbfx_rhs (``assembly.py:434-485``); the
+    backward is the restriction of dL/dq to the same rows."""
+
+    @staticmethod
+    def forward(ctx, q_int, bc, problem):
+        ctx.Ni = q_int.numel()
+        dc = problem.dc
+        return E.rhs(problem, q_int.detach().contiguous(), bc.detach().contiguous())[:dc.N]
+
+    @staticmethod
+    def backward(ctx, g):
+        return g[:ctx.Ni], g[ctx.Ni:], None
+
+
 class _Rollout(torch.autograd.Function):
     @staticmethod
     def forward(ctx, u0, step, nsteps, every, *params):

@@ -124,6 +124,28 @@ __global__ __launch_bounds__(256) void eval_t_mono_kernel(int64_t N, int64_t npt
   if (threadIdx.x == 0) out[N + m] = red[0] + red[1] + red[2] + red[3];
 }
 
+// q's rows without the terms: interior constant (+ row array), boundary
+// values, zeros for the monomial rows.
+__global__ __launch_bounds__(256) void rhs_init_kernel(int64_t N, int64_t M, int64_t Ni, double c0,
+                                                      const double* __restrict__ crow,
+                                                      const double* __restrict__ bc, double* __restrict__ q) {
+  const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (i >= N + M) return;
+  double v = 0.0;
+  if (i < Ni)
+    v = c0 + (crow ? crow[i] : 0.0);
+  else if (i < N)
+    v = bc[i - Ni];
+  q[i] = v;
+}
+
+// q[i] += w * row[i] * vals[i] over the interior rows.
+__global__ __launch_bounds__(256) void rhs_accum_kernel(int64_t Ni, double w, const double* __restrict__ row,
+                                                       const double* __restrict__ vals, double* __restrict__ q) {
+  const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (i < Ni) q[i] += w * (row ? row[i] : 1.0) * vals[i];
+}
+
 }  // namespace rbfx
 
 using namespace rbfx;
@@ -163,6 +185,47 @@ extern "C" int rbfx_eval_t(int kernel, double param, int64_t N, int64_t M, const
   if (M > 0) {
     hipLaunchKernelGGL(eval_t_mono_kernel, dim3((unsigned)M), dim3(256), 0, s, N, npts, d_px, d_py,
                        op, d_w, d_out);
+    RBFX_CHECK_LAUNCH();
+  }
+  return RBFX_OK;
+}
+
+extern "C" size_t rbfx_rhs_workspace(const rbfx_problem* p) {
+  if (!p || p->Ni < 0) return 0;
+  return ws_round(sizeof(double) * (size_t)RBFX_NOPS * (size_t)p->Ni);
+}
+
+extern "C" int rbfx_rhs(const rbfx_problem* p, double constant, const double* d_const_row, int nterms,
+                        const rbfx_rhs_term* terms, const double* d_bc, double* d_q, void* d_work,
+                        size_t work_bytes, void* stream) {
+  RBFX_REQUIRE(p != nullptr && d_q != nullptr, "null pointer");
+  RBFX_REQUIRE(p->N > 0 && p->M >= 0 && p->M <= 15, "bad N / M");
+  RBFX_REQUIRE(p->kernel >= 0 && p->kernel <= 4, "unknown kernel %d", p->kernel);
+  RBFX_REQUIRE(p->Ni >= 0 && p->Nd >= 0 && p->Nn >= 0 && p->Nr >= 0 && p->Npd >= 0, "negative row-class size");
+  RBFX_REQUIRE(p->Ni + p->Nd + p->Nn + p->Nr + 2 * p->Npd == p->N, "row classes do not add up to N");
+  RBFX_REQUIRE(d_bc || p->N == p->Ni, "boundary values missing");
+  RBFX_REQUIRE(nterms >= 0 && (nterms == 0 || terms), "bad terms");
+  hipStream_t s = as_stream(stream);
+  const int64_t N = p->N, M = p->M, Ni = p->Ni;
+  hipLaunchKernelGGL(rhs_init_kernel, dim3((unsigned)ceil_div(N + M, 256)), dim3(256), 0, s, N, M, Ni, constant,
+                     d_const_row, d_bc, d_q);
+  RBFX_CHECK_LAUNCH();
+  if (nterms == 0 || Ni == 0) return RBFX_OK;
+  RBFX_REQUIRE(p->d_x && p->d_y, "node coordinates missing");
+  Workspace ws(d_work, work_bytes);
+  double* vals = ws.take<double>((size_t)RBFX_NOPS * (size_t)Ni);
+  RBFX_REQUIRE(vals != nullptr, "rhs workspace too small");
+  const Kern k = make_kern(p->kernel, p->param);
+  for (int t = 0; t < nterms; ++t) {
+    const rbfx_rhs_term& tm = terms[t];
+    RBFX_REQUIRE(tm.op >= 0 && tm.op < RBFX_NOPS, "term %d: unknown operator %d", t, tm.op);
+    RBFX_REQUIRE(tm.d_coeffs != nullptr, "term %d: coefficients missing", t);
+    // the interior nodes are the first Ni nodes: evaluate op at them
+    hipLaunchKernelGGL(eval_kernel, dim3((unsigned)ceil_div(Ni, kEvPts)), dim3(kEvThreads), 0, s, k, N, M, p->d_x,
+                       p->d_y, tm.d_coeffs, Ni, p->d_x, p->d_y, 1u << tm.op, vals);
+    RBFX_CHECK_LAUNCH();
+    hipLaunchKernelGGL(rhs_accum_kernel, dim3((unsigned)ceil_div(Ni, 256)), dim3(256), 0, s, Ni, tm.weight,
+                       tm.d_weight_row, vals + (size_t)tm.op * Ni, d_q);
     RBFX_CHECK_LAUNCH();
   }
   return RBFX_OK;

@@ -190,6 +190,29 @@ class Problem:
 
 
 # ---- primitives -------------------------------------------------------------
+def rhs(problem, q_int, bc, terms=()):
+    """[q; 0] (n,) through rbfx_rhs: interior values q_int (Ni,) plus the
+    terms (op, weight, weight_row or None, coeffs), boundary values bc
+    (N - Ni,) in node order."""
+    dc = problem.dc
+    dev = dc.x.device
+    q = torch.empty(problem.n, dtype=torch.float64, device=dev)
+    arr = (L.RhsTerm * max(1, len(terms)))()
+    keep = []
+    for t, (op, w, row, coeffs) in enumerate(terms):
+        coeffs = coeffs.detach().to(dev, torch.float64).contiguous()
+        row = None if row is None else row.detach().to(dev, torch.float64).contiguous()
+        keep += [coeffs, row]
+        arr[t].op, arr[t].reserved, arr[t].weight = int(op), 0, float(w)
+        arr[t].d_weight_row = None if row is None else row.data_ptr()
+        arr[t].d_coeffs = coeffs.data_ptr()
+    lib = L.lib()
+    ws = L.workspace(lib.rbfx_rhs_workspace(problem.ref()), dev) if terms else None
+    L.check(lib.rbfx_rhs(problem.ref(), 0.0, L.ptr(q_int), len(terms), arr, L.ptr(bc), L.ptr(q), L.ptr(ws),
+                         0 if ws is None else ws.numel(), L.stream()))
+    return q
+
+
 def assemble(problem, matrix=L.MAT_K, dtype=torch.float64, also32=False):
     """Dense row-major (n, ld) matrix; with also32 returns (fp64, fp32)."""
     n = problem.n

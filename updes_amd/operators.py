@@ -412,15 +412,18 @@ def _bc_vector(cloud, bcs, N, dev):
 
 
 def assemble_q(operator, boundary_conditions, cloud, rbf, nb_monomials, rhs_args):
-    """q (N,) (``assembly.py:434-485``)."""
+    """q (N,) (``assembly.py:434-485``): the interior values of the lowered
+    rhs operator and the facets' values in node order, assembled by rbfx_rhs."""
     dev = _dev()
     N, Ni = cloud.N, cloud.Ni
     q_int = rhs_values(operator, cloud, rbf, nb_monomials, rhs_args)
     ids, vals = _bc_vector(cloud, boundary_conditions, N, dev)
-    all_ids = torch.cat([torch.arange(Ni, device=dev)] + ids)
-    all_vals = torch.cat([q_int] + vals)
-    q = torch.zeros(N, dtype=torch.float64, device=dev)
-    return q.index_put((all_ids,), all_vals)
+    bc = torch.zeros(N - Ni, dtype=torch.float64, device=dev)
+    if ids:
+        bc = bc.index_put((torch.cat(ids) - Ni,), torch.cat(vals).to(torch.float64))
+    kind, param = rbf_spec(rbf)
+    prob = E.Problem(E.device_cloud(cloud), kind, param, nb_monomials)
+    return AD.RhsAssemble.apply(q_int.to(torch.float64), bc, prob)
 
 
 def _robin_beta_vector(cloud, robin_coeffs):
