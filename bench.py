@@ -21,8 +21,10 @@ gradient's cost relative to the forward solve is reported.
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--side 200]
                        [--precisions fp64,fp32_ir] [--copy-probe-mib M]
                        [--gemm-replay [f64|f32]] [--no-cpu-baseline] [--no-kernel-profile]
-                       [--support-size K] [--cpu-budget SECONDS] [--cpu-only]
+                       [--support-size K] [--cpu-budget SECONDS] [--cpu-only] [--assembly-only]
 --cpu-only prints the CPU baseline alone (no GPU work), for long sample series.
+--assembly-only times the FP64 assembly of K alone beside a plain device fill
+of the same matrix (the write ceiling), for counter runs of that kernel.
 --copy-probe-mib runs one plain device copy of M MiB before the workload, so a
 counter run (FETCH_SIZE / WRITE_SIZE) can be scaled by a copy of known size.
 --gemm-replay only replays the trailing-update GEMM launches of the FP64 LU
@@ -227,6 +229,46 @@ def gemm_replay(n, nb=512, dtype="f64"):
     print("[bench] gemm replay (%s): n=%d, %d panel steps" % (dtype, n, count), file=sys.stderr, flush=True)
 
 
+def assembly_replay(cloud, reps=5):
+    """The FP64 assembly of K alone (rbfx_assemble, `reps` launches into one
+    matrix) beside a plain device fill of the same matrix (torch fill_, the
+    achievable write rate), each timed with HIP events on the launch stream;
+    for counter runs on the assembly kernel alone."""
+    from updes_amd import _lib as L, engine as E
+    from updes_amd.operators import lower_diff_operator
+    import updes_amd as u
+    desc = lower_diff_operator(lap_op, cloud, u.polyharmonic, None)
+    prob = E.Problem(E.device_cloud(cloud), L.POLYHARMONIC, 1.0, 6, desc)
+    n = prob.n
+    ld = E._ld(n)
+    K = torch.empty((n, ld), dtype=torch.float64, device="cuda")
+    lib = L.lib()
+    stream = torch.cuda.current_stream()
+
+    def timed(fn):
+        fn()
+        torch.cuda.synchronize()
+        ts = []
+        for _ in range(reps):
+            a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            a.record(stream)
+            fn()
+            b.record(stream)
+            torch.cuda.synchronize()
+            ts.append(a.elapsed_time(b))
+        return float(np.median(ts))
+
+    fill_ms = timed(lambda: K.fill_(0.5))
+    asm_ms = timed(lambda: L.check(lib.rbfx_assemble(prob.ref(), L.MAT_K, 0, n, L.ptr(K), ld, None, 0,
+                                                     ctypes.c_void_p(stream.cuda_stream))))
+    written = 8.0 * n * n
+    out = {"assembly_replay": {"n": n, "ld": ld, "bytes_written_algorithmic": written,
+                               "fill_ms": fill_ms, "fill_gbps": 8.0 * n * ld / fill_ms / 1e6,
+                               "assembly_ms": asm_ms, "assembly_gbps": written / asm_ms / 1e6,
+                               "assembly_over_fill_rate": (written / asm_ms) / (8.0 * n * ld / fill_ms)}}
+    print(json.dumps(out))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -245,6 +287,9 @@ def main():
     ap.add_argument("--cpu-budget", type=float, default=30.0,
                     help="seconds of host work for the CPU baseline's samples")
     ap.add_argument("--cpu-only", action="store_true", help="print the CPU baseline alone and exit")
+    ap.add_argument("--assembly-only", action="store_true",
+                    help="time the FP64 assembly of K alone beside a plain fill of the same matrix, and exit "
+                         "(for counter runs of the assembly kernel)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: the dense solve of one collocation system runs on a single MI355X, "
@@ -278,6 +323,9 @@ def main():
     t_cloud = time.perf_counter()
     cloud, bcs, b, basis = fourier_problem(args.side, support_size=support)
     t_cloud = time.perf_counter() - t_cloud
+    if args.assembly_only:
+        assembly_replay(cloud)
+        return
     n = cloud.N + 6
     print("[bench] N=%d n=%d, precisions %s" % (cloud.N, n, args.precisions), file=sys.stderr, flush=True)
 

@@ -197,8 +197,10 @@ int rbfx_getrf_f32(int64_t n, float* d_A, int64_t lda, int64_t* d_ipiv,
  * permutation (row i of P A is row perm[i] of A) and d_dinv
  * (rbfx_getrs_dinv_elems(n) doubles: the FP64 inverses of the 64 x 64
  * diagonal blocks of L and U, then those of the 512 x 512 diagonal blocks
- * that the solves apply).  Synchronises the stream (copies ipiv to the host
- * once).                                                                    */
+ * that the solves apply, then a condition estimate of each 512 x 512 block:
+ * the FP64 solves substitute over the 64-row sub-blocks of an
+ * ill-conditioned block instead of applying its inverse).  Synchronises the
+ * stream (copies ipiv to the host once).                                    */
 int64_t rbfx_getrs_dinv_elems(int64_t n);
 int rbfx_getrs_prepare_f64(int64_t n, const double* d_LU, int64_t lda,
                            const int64_t* d_ipiv, int64_t* d_perm, double* d_dinv,

@@ -211,33 +211,32 @@ __device__ inline double elem_rbf(const ProbDev& P, const RowCtx& c, int64_t j, 
     case RC_VAL: {
       if (j == c.i) return 0.0;
       const double dx = c.xi - xj, dy = c.yi - yj;
-      return radial_value(P.k, sqrt(dx * dx + dy * dy));
+      return radial_value(P.k, sqrt(dist2(dx, dy)));
     }
     case RC_N: {
       if (j == c.i) return 0.0;
       const double dx = c.xi - xj, dy = c.yi - yj;
-      return radial_f1(P.k, sqrt(dx * dx + dy * dy)) * (dx * c.nx + dy * c.ny);
+      return radial_f1(P.k, sqrt(dist2(dx, dy))) * __builtin_fma(dx, c.nx, dy * c.ny);
     }
     case RC_R: {
       if (j == c.i) return 0.0;
       const double dx = c.xi - xj, dy = c.yi - yj;
-      const double r = sqrt(dx * dx + dy * dy);
+      const double r = sqrt(dist2(dx, dy));
       if (r == 0.0) return c.beta * P.k.phi0;
       double phi, f1, g2;
       radial(P.k, r, phi, f1, g2);
-      return c.beta * phi + f1 * (dx * c.nx + dy * c.ny);
+      return __builtin_fma(c.beta, phi, f1 * __builtin_fma(dx, c.nx, dy * c.ny));
     }
     case RC_PD: {
       const double dx1 = c.xi - xj, dy1 = c.yi - yj;
       const double dx2 = c.x2 - xj, dy2 = c.y2 - yj;
-      return radial_value(P.k, sqrt(dx1 * dx1 + dy1 * dy1)) -
-             radial_value(P.k, sqrt(dx2 * dx2 + dy2 * dy2));
+      return radial_value(P.k, sqrt(dist2(dx1, dy1))) - radial_value(P.k, sqrt(dist2(dx2, dy2)));
     }
     case RC_PN: {
       const double dx1 = c.xi - xj, dy1 = c.yi - yj;
       const double dx2 = c.x2 - xj, dy2 = c.y2 - yj;
-      return radial_f1(P.k, sqrt(dx1 * dx1 + dy1 * dy1)) * (dx1 * c.nx + dy1 * c.ny) +
-             radial_f1(P.k, sqrt(dx2 * dx2 + dy2 * dy2)) * (dx2 * c.n2x + dy2 * c.n2y);
+      return __builtin_fma(radial_f1(P.k, sqrt(dist2(dx1, dy1))), __builtin_fma(dx1, c.nx, dy1 * c.ny),
+                           radial_f1(P.k, sqrt(dist2(dx2, dy2))) * __builtin_fma(dx2, c.n2x, dy2 * c.n2y));
     }
     default:  // RC_PT: row N+k holds p_k(x_j)
     {
@@ -284,8 +283,46 @@ typedef double vd2 __attribute__((ext_vector_type(2)));
 typedef float vf2 __attribute__((ext_vector_type(2)));
 
 constexpr int kAsmThreads = 256;
-constexpr int kAsmCols = 2 * kAsmThreads;  // columns per block
-constexpr int kAsmRows = 16;               // rows per block
+// Each thread writes kAsmPairs adjacent column pairs of every row of its
+// block (pairs 2 kAsmThreads columns apart, so every store instruction of a
+// wave covers 1 KB contiguous).  More pairs per thread (amortising the
+// wave-uniform row context) measured slower: 4 pairs 5.47 vs 5.16 ms for K
+// at 40k (176 VGPRs, two waves per SIMD).
+constexpr int kAsmPairs = 1;
+constexpr int kAsmCols = 2 * kAsmThreads * kAsmPairs;  // columns per block
+constexpr int kAsmRows = 16;                            // rows per block
+
+// sqrt of x for 2^-900 <= x <= 2^900: the library's FP64 square-root
+// sequence (rsq estimate, a Goldschmidt step, two Newton corrections)
+// without its range scaling and its 0 / inf fixups, so the same result.
+__device__ inline double sqrt_mid(double x) {
+  const double g = __builtin_amdgcn_rsq(x);
+  double s = x * g, h = 0.5 * g;
+  const double r = __builtin_fma(-h, s, 0.5);
+  s = __builtin_fma(s, r, s);
+  h = __builtin_fma(h, r, h);
+  double d = __builtin_fma(-s, s, x);
+  s = __builtin_fma(d, h, s);
+  d = __builtin_fma(-s, s, x);
+  return __builtin_fma(d, h, s);
+}
+
+// Entry of an interior collocation row (the operator applied to
+// phi(., x_j) at x_i) without branches: apply_op's arithmetic, r = 0 and
+// the zero diagonal (j == i) as selects.  mid: every r^2 of the wave is in
+// sqrt_mid's range (r^2 = 0 aside).
+template <int KIND, bool FAST>
+__device__ inline double interior_entry(const Kern& k, const OpCoef& c, double dx, double dy, bool diag,
+                                        bool mid) {
+  const double r2 = dist2(dx, dy);
+  const bool zero = !(r2 > 0.0);
+  const double rs2 = zero ? 1.0 : r2;
+  const double r = mid ? sqrt_mid(rs2) : sqrt(rs2);
+  const double v = apply_op_pos(k, c, dx, dy, rs2, r);
+  return diag ? 0.0 : (zero ? c.v * k.phi0 : v);
+}
+
+__device__ inline bool in_sqrt_mid(double r2) { return !(r2 > 0.0) || (r2 >= 0x1p-900 && r2 <= 0x1p900); }
 
 template <int KIND, bool FAST, bool LOC, bool W64, bool W32>
 __global__ __launch_bounds__(kAsmThreads) void assemble_kernel(ProbDev P, int64_t row0,
@@ -293,38 +330,50 @@ __global__ __launch_bounds__(kAsmThreads) void assemble_kernel(ProbDev P, int64_
                                                                int64_t ld64, float* out32,
                                                                int64_t ld32) {
   pin_kind<KIND, FAST>(P.k);
-  const int64_t j0 = (int64_t)blockIdx.x * kAsmCols + 2 * threadIdx.x;
+  const int64_t jb = (int64_t)blockIdx.x * kAsmCols + 2 * threadIdx.x;
   const int64_t rb = (int64_t)blockIdx.y * kAsmRows;
-  if (j0 >= P.n) return;
-  const bool two = (j0 + 1) < P.n;
-  const bool c0rbf = j0 < P.N, c1rbf = (j0 + 1) < P.N;
-  const double xj0 = c0rbf ? P.x[j0] : 0.0, yj0 = c0rbf ? P.y[j0] : 0.0;
-  const double xj1 = c1rbf ? P.x[j0 + 1] : 0.0, yj1 = c1rbf ? P.y[j0 + 1] : 0.0;
+  if (jb >= P.n) return;
+  double xj[kAsmPairs][2], yj[kAsmPairs][2];
+#pragma unroll
+  for (int q = 0; q < kAsmPairs; ++q)
+#pragma unroll
+    for (int e = 0; e < 2; ++e) {
+      const int64_t j = jb + 2 * kAsmThreads * q + e;
+      xj[q][e] = j < P.N ? P.x[j] : 0.0;
+      yj[q][e] = j < P.N ? P.y[j] : 0.0;
+    }
   const int64_t rend = rb + kAsmRows < nrows ? rb + kAsmRows : nrows;
   for (int64_t rr = rb; rr < rend; ++rr) {
     const int64_t r = row0 + rr;
     const RowCtx c = row_ctx<LOC>(P, r);
-    const double v0 = c0rbf ? elem_rbf<LOC>(P, c, j0, xj0, yj0) : elem_mono(c, (int)(j0 - P.N));
-    double v1 = 0.0;
-    if (two) v1 = c1rbf ? elem_rbf<LOC>(P, c, j0 + 1, xj1, yj1) : elem_mono(c, (int)(j0 + 1 - P.N));
-    if (W64) {
-      double* dst = out64 + rr * ld64 + j0;
-      if (two && (((uintptr_t)dst & 15) == 0)) {
-        vd2 v = {v0, v1};
-        __builtin_nontemporal_store(v, reinterpret_cast<vd2*>(dst));
-      } else {
-        __builtin_nontemporal_store(v0, dst);
-        if (two) __builtin_nontemporal_store(v1, dst + 1);
+#pragma unroll
+    for (int q = 0; q < kAsmPairs; ++q) {
+      const int64_t j0 = jb + 2 * kAsmThreads * q;
+      if (j0 >= P.n) break;
+      const bool two = (j0 + 1) < P.n;
+      const double v0 = j0 < P.N ? elem_rbf<LOC>(P, c, j0, xj[q][0], yj[q][0]) : elem_mono(c, (int)(j0 - P.N));
+      double v1 = 0.0;
+      if (two)
+        v1 = (j0 + 1) < P.N ? elem_rbf<LOC>(P, c, j0 + 1, xj[q][1], yj[q][1]) : elem_mono(c, (int)(j0 + 1 - P.N));
+      if (W64) {
+        double* dst = out64 + rr * ld64 + j0;
+        if (two && (((uintptr_t)dst & 15) == 0)) {
+          vd2 v = {v0, v1};
+          __builtin_nontemporal_store(v, reinterpret_cast<vd2*>(dst));
+        } else {
+          __builtin_nontemporal_store(v0, dst);
+          if (two) __builtin_nontemporal_store(v1, dst + 1);
+        }
       }
-    }
-    if (W32) {
-      float* dst = out32 + rr * ld32 + j0;
-      if (two && (((uintptr_t)dst & 7) == 0)) {
-        vf2 v = {(float)v0, (float)v1};
-        __builtin_nontemporal_store(v, reinterpret_cast<vf2*>(dst));
-      } else {
-        __builtin_nontemporal_store((float)v0, dst);
-        if (two) __builtin_nontemporal_store((float)v1, dst + 1);
+      if (W32) {
+        float* dst = out32 + rr * ld32 + j0;
+        if (two && (((uintptr_t)dst & 7) == 0)) {
+          vf2 v = {(float)v0, (float)v1};
+          __builtin_nontemporal_store(v, reinterpret_cast<vf2*>(dst));
+        } else {
+          __builtin_nontemporal_store((float)v0, dst);
+          if (two) __builtin_nontemporal_store((float)v1, dst + 1);
+        }
       }
     }
   }
@@ -631,9 +680,82 @@ static int make_prob(const rbfx_problem* p, int matrix, ProbDev& P) {
   return RBFX_OK;
 }
 
+// The interior collocation rows of K (rows < Ni, every node in every
+// support): a kernel of their own, so that their entries compile to
+// straight-line code (no row-class switch, r = 0 and the zero diagonal as
+// selects) with a register budget of their own.  Same arithmetic as
+// elem_rbf / elem_mono for these rows.
+template <int KIND, bool FAST, bool W64, bool W32>
+__global__ __launch_bounds__(kAsmThreads) void assemble_interior_kernel(ProbDev P, int64_t row0, int64_t nrows,
+                                                                        double* out64, int64_t ld64, float* out32,
+                                                                        int64_t ld32) {
+  pin_kind<KIND, FAST>(P.k);
+  const int64_t j0 = (int64_t)blockIdx.x * (2 * kAsmThreads) + 2 * threadIdx.x;
+  const int64_t rb = (int64_t)blockIdx.y * kAsmRows;
+  if (j0 >= P.n) return;
+  const bool two = (j0 + 1) < P.n;
+  const bool rbf2 = (j0 + 1) < P.N;  // both columns are centres
+  const double xj0 = j0 < P.N ? P.x[j0] : 0.0, yj0 = j0 < P.N ? P.y[j0] : 0.0;
+  const double xj1 = rbf2 ? P.x[j0 + 1] : 0.0, yj1 = rbf2 ? P.y[j0 + 1] : 0.0;
+  const int64_t rend = rb + kAsmRows < nrows ? rb + kAsmRows : nrows;
+  for (int64_t rr = rb; rr < rend; ++rr) {
+    const int64_t r = row0 + rr;
+    const OpCoef oc = make_coef(P.rc, P.rrow, r);
+    const double xi = P.x[r], yi = P.y[r];
+    double v0, v1 = 0.0;
+    if (rbf2) {
+      const double dx0 = xi - xj0, dy0 = yi - yj0;
+      const double dx1 = xi - xj1, dy1 = yi - yj1;
+      const bool mid = __all(in_sqrt_mid(dist2(dx0, dy0)) && in_sqrt_mid(dist2(dx1, dy1)));
+      v0 = interior_entry<KIND, FAST>(P.k, oc, dx0, dy0, j0 == r, mid);
+      v1 = interior_entry<KIND, FAST>(P.k, oc, dx1, dy1, j0 + 1 == r, mid);
+    } else {
+      // the last centre and the monomial columns
+      const OpCoef mc = make_coef(P.mc, P.mrow, r);
+      v0 = j0 < P.N ? interior_entry<KIND, FAST>(P.k, oc, xi - xj0, yi - yj0, j0 == r, false)
+                    : mono_apply((int)(j0 - P.N), xi, yi, mc);
+      if (two) v1 = mono_apply((int)(j0 + 1 - P.N), xi, yi, mc);
+    }
+    if (W64) {
+      double* dst = out64 + rr * ld64 + j0;
+      if (two && (((uintptr_t)dst & 15) == 0)) {
+        vd2 v = {v0, v1};
+        __builtin_nontemporal_store(v, reinterpret_cast<vd2*>(dst));
+      } else {
+        __builtin_nontemporal_store(v0, dst);
+        if (two) __builtin_nontemporal_store(v1, dst + 1);
+      }
+    }
+    if (W32) {
+      float* dst = out32 + rr * ld32 + j0;
+      if (two && (((uintptr_t)dst & 7) == 0)) {
+        vf2 v = {(float)v0, (float)v1};
+        __builtin_nontemporal_store(v, reinterpret_cast<vf2*>(dst));
+      } else {
+        __builtin_nontemporal_store((float)v0, dst);
+        if (two) __builtin_nontemporal_store((float)v1, dst + 1);
+      }
+    }
+  }
+}
+
 }  // namespace rbfx
 
 using namespace rbfx;
+
+template <int KIND, bool FAST>
+static void launch_interior(dim3 grid, hipStream_t s, const ProbDev& P, int64_t row0, int64_t nrows,
+                            double* out64, int64_t ld64, float* out32, int64_t ld32) {
+  if (out64 && out32)
+    hipLaunchKernelGGL((assemble_interior_kernel<KIND, FAST, true, true>), grid, dim3(kAsmThreads), 0, s, P, row0,
+                       nrows, out64, ld64, out32, ld32);
+  else if (out64)
+    hipLaunchKernelGGL((assemble_interior_kernel<KIND, FAST, true, false>), grid, dim3(kAsmThreads), 0, s, P, row0,
+                       nrows, out64, ld64, out32, ld32);
+  else
+    hipLaunchKernelGGL((assemble_interior_kernel<KIND, FAST, false, true>), grid, dim3(kAsmThreads), 0, s, P, row0,
+                       nrows, out64, ld64, out32, ld32);
+}
 
 template <int KIND, bool FAST, bool LOC>
 static void launch_assemble(dim3 grid, hipStream_t s, const ProbDev& P, int64_t row0, int64_t nrows,
@@ -660,16 +782,29 @@ extern "C" int rbfx_assemble(const rbfx_problem* p, int matrix, int64_t row0, in
   RBFX_REQUIRE(!d_out64 || ld64 >= P.n, "ld64 < n");
   RBFX_REQUIRE(!d_out32 || ld32 >= P.n, "ld32 < n");
   if (nrows == 0) return RBFX_OK;
-  dim3 grid((unsigned)ceil_div(P.n, kAsmCols), (unsigned)ceil_div(nrows, kAsmRows));
-  RBFX_REQUIRE(grid.y <= 65535u * 1024u, "too many rows");
+  RBFX_REQUIRE(ceil_div(nrows, kAsmRows) <= 65535ll * 1024ll, "too many rows");
   hipStream_t s = as_stream(stream);
   const int pid = prof_start(d_out64 ? (d_out32 ? "assemble_f64_f32" : "assemble_f64") : "assemble_f32", s);
-  RBFX_KIND_SWITCH(P.k, {
-    if (P.local)
-      launch_assemble<KIND_, FAST_, true>(grid, s, P, row0, nrows, d_out64, ld64, d_out32, ld32);
-    else
-      launch_assemble<KIND_, FAST_, false>(grid, s, P, row0, nrows, d_out64, ld64, d_out32, ld32);
-  });
+  // rows [row0, row0 + ni) are interior collocation rows of K
+  const int64_t ni = (matrix == RBFX_MAT_K && !P.local) ? std::max<int64_t>(0, std::min(row0 + nrows, P.Ni) - row0) : 0;
+  if (ni > 0) {
+    dim3 gi((unsigned)ceil_div(P.n, 2 * kAsmThreads), (unsigned)ceil_div(ni, kAsmRows));
+    double* o64 = d_out64;
+    float* o32 = d_out32;
+    RBFX_KIND_SWITCH(P.k, { launch_interior<KIND_, FAST_>(gi, s, P, row0, ni, o64, ld64, o32, ld32); });
+  }
+  const int64_t r1 = row0 + ni, nr = nrows - ni;
+  if (nr > 0) {
+    dim3 grid((unsigned)ceil_div(P.n, kAsmCols), (unsigned)ceil_div(nr, kAsmRows));
+    double* o64 = d_out64 ? d_out64 + ni * ld64 : nullptr;
+    float* o32 = d_out32 ? d_out32 + ni * ld32 : nullptr;
+    RBFX_KIND_SWITCH(P.k, {
+      if (P.local)
+        launch_assemble<KIND_, FAST_, true>(grid, s, P, r1, nr, o64, ld64, o32, ld32);
+      else
+        launch_assemble<KIND_, FAST_, false>(grid, s, P, r1, nr, o64, ld64, o32, ld32);
+    });
+  }
   prof_stop(pid, s, 0.0, (double)nrows * P.n * ((d_out64 ? 8.0 : 0.0) + (d_out32 ? 4.0 : 0.0)));
   RBFX_CHECK_LAUNCH();
   return RBFX_OK;

@@ -108,7 +108,7 @@ __global__ __launch_bounds__(64) void cardinal_coeffs_kernel(CardArgs a) {
     const double ubx = __shfl(ux, b, 64), uby = __shfl(uy, b, 64);
     if (lane < k) {
       const double dx = xl - xb, dy = yl - yb;
-      A[lane][b] = lane == b ? 0.0 : radial_value(a.kern, sqrt(dx * dx + dy * dy));
+      A[lane][b] = lane == b ? 0.0 : radial_value(a.kern, sqrt(dist2(dx, dy)));
     } else if (lane < S) {
       A[lane][b] = mono_val(lane - k, ubx, uby);
     }

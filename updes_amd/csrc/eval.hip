@@ -38,7 +38,7 @@ __global__ __launch_bounds__(kEvThreads) void eval_kernel(Kern k, int64_t N, int
     for (int q = 0; q < kEvPts; ++q) {
       const double dx = xp[q] - xj, dy = yp[q] - yj;
       if (only_val) {
-        acc[q][RBFX_VAL] += lam * radial_value(k, sqrt(dx * dx + dy * dy));
+        acc[q][RBFX_VAL] += lam * radial_value(k, sqrt(dist2(dx, dy)));
       } else {
         double o[RBFX_NOPS];
         all_ops(k, dx, dy, o);
@@ -76,7 +76,7 @@ __global__ __launch_bounds__(kEvThreads) void eval_kernel(Kern k, int64_t N, int
 }
 
 __device__ inline double op_value(const Kern& k, int op, double dx, double dy) {
-  if (op == RBFX_VAL) return radial_value(k, sqrt(dx * dx + dy * dy));
+  if (op == RBFX_VAL) return radial_value(k, sqrt(dist2(dx, dy)));
   double o[RBFX_NOPS];
   all_ops(k, dx, dy, o);
   return o[op];

@@ -100,7 +100,9 @@ __device__ inline void pin_kind(Kern& k) {
   } while (0)
 
 // phi, f1 = phi'/r and g2 = phi'' - phi'/r (= r^2 f2) for r > 0.  Using g2
-// keeps the Laplacian division-free: Lap phi = g2 + 2 f1.
+// keeps the Laplacian division-free: Lap phi = g2 + 2 f1.  Every sum is an
+// explicit FMA or add, so each kernel that forms an entry gets the same bits
+// (the assembly, the matrix-free products and the residuals must agree).
 __device__ inline void radial(const Kern& k, double r, double& phi, double& f1, double& g2) {
   switch (k.kind) {
     case RBFX_POLYHARMONIC:
@@ -127,26 +129,26 @@ __device__ inline void radial(const Kern& k, double r, double& phi, double& f1, 
       const double lr = log(r);
       if (k.fast) {
         phi = r * r * lr;
-        f1 = 2.0 * lr + 1.0;
+        f1 = __builtin_fma(2.0, lr, 1.0);
         g2 = 2.0;
       } else {
         const double a = k.p;
         const double rm2 = pow(r, k.m - 2.0);
         phi = rm2 * r * r * lr;
-        f1 = rm2 * (2.0 * a * lr + 1.0);
-        g2 = rm2 * ((4.0 * a * a - 4.0 * a) * lr + 4.0 * a - 2.0);
+        f1 = rm2 * __builtin_fma(2.0 * a, lr, 1.0);
+        g2 = rm2 * __builtin_fma(4.0 * a * a - 4.0 * a, lr, 4.0 * a - 2.0);
       }
       break;
     }
     case RBFX_MULTIQUADRIC: {
-      const double s = sqrt(1.0 + k.e2 * r * r);
+      const double s = sqrt(__builtin_fma(k.e2 * r, r, 1.0));
       phi = s;
       f1 = k.e2 / s;
       g2 = -k.e2 * k.e2 * r * r / (s * s * s);
       break;
     }
     default: {  // RBFX_INV_MULTIQUADRIC
-      const double s = 1.0 / sqrt(1.0 + k.e2 * r * r);
+      const double s = 1.0 / sqrt(__builtin_fma(k.e2 * r, r, 1.0));
       phi = s;
       f1 = -k.e2 * s * s * s;
       g2 = 3.0 * k.e2 * k.e2 * r * r * s * s * s * s * s;
@@ -154,6 +156,9 @@ __device__ inline void radial(const Kern& k, double r, double& phi, double& f1, 
     }
   }
 }
+
+// Squared distance, the same bits in every kernel.
+__device__ inline double dist2(double dx, double dy) { return __builtin_fma(dx, dx, dy * dy); }
 
 __device__ inline double radial_value(const Kern& k, double r) {
   if (r == 0.0) return k.phi0;
@@ -165,9 +170,9 @@ __device__ inline double radial_value(const Kern& k, double r) {
     case RBFX_THIN_PLATE:
       return k.fast ? r * r * log(r) : pow(r, k.m) * log(r);
     case RBFX_MULTIQUADRIC:
-      return sqrt(1.0 + k.e2 * r * r);
+      return sqrt(__builtin_fma(k.e2 * r, r, 1.0));
     default:
-      return 1.0 / sqrt(1.0 + k.e2 * r * r);
+      return 1.0 / sqrt(__builtin_fma(k.e2 * r, r, 1.0));
   }
 }
 
@@ -180,11 +185,11 @@ __device__ inline double radial_f1(const Kern& k, double r) {
     case RBFX_GAUSSIAN:
       return -2.0 * k.e2 * exp(-k.e2 * r * r);
     case RBFX_THIN_PLATE:
-      return k.fast ? 2.0 * log(r) + 1.0 : pow(r, k.m - 2.0) * (2.0 * k.p * log(r) + 1.0);
+      return k.fast ? __builtin_fma(2.0, log(r), 1.0) : pow(r, k.m - 2.0) * __builtin_fma(2.0 * k.p, log(r), 1.0);
     case RBFX_MULTIQUADRIC:
-      return k.e2 / sqrt(1.0 + k.e2 * r * r);
+      return k.e2 / sqrt(__builtin_fma(k.e2 * r, r, 1.0));
     default: {
-      const double s = 1.0 / sqrt(1.0 + k.e2 * r * r);
+      const double s = 1.0 / sqrt(__builtin_fma(k.e2 * r, r, 1.0));
       return -k.e2 * s * s * s;
     }
   }
@@ -197,21 +202,26 @@ struct OpCoef {
   bool hess;  // any of dxx/dyy/dxy non-zero
 };
 
-__device__ inline double apply_op(const Kern& k, const OpCoef& c, double dx, double dy) {
-  const double r2 = dx * dx + dy * dy;
-  const double r = sqrt(r2);
-  if (r == 0.0) return c.v * k.phi0;
+// apply_op for r2 = dist2(dx, dy) > 0 and r = sqrt(r2), explicit FMAs.
+__device__ inline double apply_op_pos(const Kern& k, const OpCoef& c, double dx, double dy, double r2, double r) {
   double phi, f1, g2;
   radial(k, r, phi, f1, g2);
-  const double a1 = c.dx * dx + c.dy * dy + 2.0 * c.lap + c.dxx + c.dyy;
+  const double a1 = __builtin_fma(c.dx, dx, __builtin_fma(c.dy, dy, __builtin_fma(2.0, c.lap, c.dxx + c.dyy)));
   double hv = c.lap;
-  if (c.hess) hv += (c.dxx * dx * dx + c.dyy * dy * dy + c.dxy * dx * dy) / r2;
-  return c.v * phi + f1 * a1 + g2 * hv;
+  if (c.hess) hv += __builtin_fma(c.dxx * dx, dx, __builtin_fma(c.dyy * dy, dy, c.dxy * dx * dy)) / r2;
+  return __builtin_fma(c.v, phi, __builtin_fma(f1, a1, g2 * hv));
+}
+
+__device__ inline double apply_op(const Kern& k, const OpCoef& c, double dx, double dy) {
+  const double r2 = dist2(dx, dy);
+  const double r = sqrt(r2);
+  if (r == 0.0) return c.v * k.phi0;
+  return apply_op_pos(k, c, dx, dy, r2, r);
 }
 
 // All seven operator values of phi at d (evaluation kernels).
 __device__ inline void all_ops(const Kern& k, double dx, double dy, double out[RBFX_NOPS]) {
-  const double r2 = dx * dx + dy * dy;
+  const double r2 = dist2(dx, dy);
   const double r = sqrt(r2);
   if (r == 0.0) {
     out[0] = k.phi0;

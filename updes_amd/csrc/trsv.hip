@@ -211,6 +211,55 @@ __global__ __launch_bounds__(256) void sbinv_kernel(const TF* __restrict__ LU, i
   (void)NSUB;
 }
 
+// Condition estimate of every diagonal superblock T_KK of L (which = 0, unit
+// lower) and U (which = 1): max(||T||_inf ||X||_inf, ||T||_1 ||X||_1) with X
+// its 512 x 512 inverse.  The FP64 sweeps apply X only where this is at most
+// kCondRobust; above it they solve the block by substitution over its 64-row
+// sub-blocks (one refinement step with X reduces the block residual by about
+// kappa eps, so X serves a backward-stable solve only for kappa well below
+// eps^-1/2).
+constexpr double kCondRobust = 1e7;
+template <typename TF>
+__global__ __launch_bounds__(256) void sbcond_kernel(const TF* __restrict__ LU, int64_t lda, int64_t n,
+                                                     const double* __restrict__ dinv512, double* __restrict__ cond) {
+  __shared__ double red[4][256];
+  const int I = blockIdx.x >> 1, which = blockIdx.x & 1;
+  const int64_t o = (int64_t)I * SB;
+  const int sz = (int)imin64(SB, n - o);
+  const double* X = dinv512 + (int64_t)blockIdx.x * SB * SB;
+  double tr = 0, tc = 0, xr = 0, xc = 0;  // max row / column sums of |T| and |X|
+  for (int q = threadIdx.x; q < sz; q += 256) {
+    double a = 0, b = 0, cx = 0, dx = 0;
+    for (int k = 0; k < sz; ++k) {
+      // row q and column q of the triangle (unit diagonal for L)
+      const bool inr = which ? k >= q : k < q;
+      const bool inc = which ? k <= q : k > q;
+      if (inr) a += fabs((double)LU[(o + q) * lda + o + k]);
+      if (inc) b += fabs((double)LU[(o + k) * lda + o + q]);
+      cx += fabs(X[(int64_t)q * SB + k]);
+      dx += fabs(X[(int64_t)k * SB + q]);
+    }
+    if (!which) {
+      a += 1.0;
+      b += 1.0;
+    }
+    tr = fmax(tr, a);
+    tc = fmax(tc, b);
+    xr = fmax(xr, cx);
+    xc = fmax(xc, dx);
+  }
+  red[0][threadIdx.x] = tr;
+  red[1][threadIdx.x] = tc;
+  red[2][threadIdx.x] = xr;
+  red[3][threadIdx.x] = xc;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int t = 1; t < 256; ++t)
+      for (int e = 0; e < 4; ++e) red[e][0] = fmax(red[e][0], red[e][t]);
+    cond[blockIdx.x] = fmax(red[0][0] * red[2][0], red[1][0] * red[3][0]);
+  }
+}
+
 // ctl[0] ticket, ctl[1] superblocks solved, ctl[2] status of the solve; the
 // s-th processed superblock owns the 128-byte line ctl[kCtlLine (s + 1) ..]:
 // word e < 3 counts its chunks that have published stage e (0: accumulated
@@ -219,9 +268,11 @@ __global__ __launch_bounds__(256) void sbinv_kernel(const TF* __restrict__ LU, i
 // Superblocks complete in order (each folds its predecessor), so ctl[1] >= k
 // means superblocks 0 .. k-1 are solved.
 //
-// REFINE (FP64 U factors): the inverse of an ill-conditioned 512 x 512 block
-// of U applied to the accumulated values loses digits that a substitution
-// keeps (measured on the adjoint gradients).  One step of refinement inside
+// REFINE (FP64 factors, L and U): the inverse of an ill-conditioned
+// 512 x 512 diagonal block applied to the accumulated values loses digits
+// that a substitution keeps (measured on the adjoint gradients, and as a
+// backward error of 6e-9 against 8e-17 for substitution with the same
+// factors of a numerically singular system).  One step of refinement inside
 // the block restores them: x0 = D acc, r = acc - T_KK x0, x = x0 + D r, at two
 // more exchanges per superblock.  The FP32 factors of the preconditioner
 // need no refinement: an application error there only perturbs the
@@ -230,7 +281,9 @@ constexpr int kSweepThreads = 512;
 constexpr int kSweepWaves = kSweepThreads / 64;
 template <typename TF, bool FWD, bool TRANS, int WHICH, bool REFINE>
 __global__ __launch_bounds__(kSweepThreads) void sbsweep_kernel(const TF* __restrict__ F, int64_t lda, int64_t n,
-                                                                const double* __restrict__ D512, double* __restrict__ y,
+                                                                const double* __restrict__ D512,
+                                                                const double* __restrict__ dinv64,
+                                                                const double* __restrict__ cond, double* __restrict__ y,
                                                                 double* __restrict__ P, unsigned* __restrict__ ctl,
                                                                 uint64_t wait_ticks) {
   typedef double T;
@@ -321,8 +374,9 @@ __global__ __launch_bounds__(kSweepThreads) void sbsweep_kernel(const TF* __rest
   // Loads of the operands are issued before the wait that releases their
   // use, so the chain of the sweep does not wait on HBM behind the other
   // workgroups' folds.
-  // the factor's tile of column block oJ (szJ valid); diag: the triangle of
-  // the diagonal block (U_KK, or its transpose) only
+  // the factor's tile of column block oJ (szJ valid); diag: the stored
+  // triangle of the diagonal block only (U_KK with its diagonal, or the
+  // strictly lower part of the unit-lower L_KK; transposed as the sweep is)
   // Addresses are a wave-uniform base (scalar registers) plus one 32-bit
   // per-lane offset, so the NV loads in flight hold no vector address each.
   const int wu = __builtin_amdgcn_readfirstlane(w);
@@ -338,14 +392,16 @@ __global__ __launch_bounds__(kSweepThreads) void sbsweep_kernel(const TF* __rest
       for (int k = 0; k < RW; ++k) {
         const int r = RW * wu + k;
         const TF* row = F + (oI + rc0 + r) * lda + oJ;
-        const int dlim = rc0 + r - lane;  // diagonal: 64m >= dlim
+        const int dlim = rc0 + r - lane;  // the diagonal entry: 64m == dlim
         if (full) {
 #pragma unroll
           for (int m = 0; m < 8; ++m) v[8 * k + m] = row[(uint32_t)(lane + 64 * m)];
         } else if (r < nr) {
 #pragma unroll
-          for (int m = 0; m < 8; ++m)
-            v[8 * k + m] = (64 * m < clim && (!diag || 64 * m >= dlim)) ? row[(uint32_t)(lane + 64 * m)] : TF(0);
+          for (int m = 0; m < 8; ++m) {
+            const bool tri = WHICH == 1 ? 64 * m >= dlim : 64 * m < dlim;
+            v[8 * k + m] = (64 * m < clim && (!diag || tri)) ? row[(uint32_t)(lane + 64 * m)] : TF(0);
+          }
         } else {
 #pragma unroll
           for (int m = 0; m < 8; ++m) v[8 * k + m] = TF(0);
@@ -360,10 +416,12 @@ __global__ __launch_bounds__(kSweepThreads) void sbsweep_kernel(const TF* __rest
       } else {
         const int g0 = tgrp * NV;
         const int qlim = tcol < nr ? szJ - g0 : 0;  // rows g0 + q valid for q < qlim
-        const int dlim = rc0 + tcol - g0;           // diagonal: q <= dlim
+        const int dlim = rc0 + tcol - g0;           // the diagonal entry: q == dlim
 #pragma unroll
-        for (int q = 0; q < NV; ++q)
-          v[q] = (q < qlim && (!diag || q <= dlim)) ? (base + q * lda)[off] : TF(0);
+        for (int q = 0; q < NV; ++q) {
+          const bool tri = WHICH == 1 ? q <= dlim : q > dlim;
+          v[q] = (q < qlim && (!diag || tri)) ? (base + q * lda)[off] : TF(0);
+        }
       }
     }
   };
@@ -483,11 +541,47 @@ __global__ __launch_bounds__(kSweepThreads) void sbsweep_kernel(const TF* __rest
   T acc_r = tid < nr ? y[oI + rc0 + tid] - row_value(tid) : T(0);
   if (!exchange(P, 0, acc_r, xs)) return;
   // 3. x = rows of D applied to the superblock's accumulated values
+  T x;
+  if constexpr (REFINE) {
+    if (cond[I * 2 + WHICH] > kCondRobust) {
+      // ill-conditioned block: every chunk solves the whole block by
+      // substitution over its 64-row sub-blocks with their 64 x 64 inverses
+      // (xs: values still to solve, xs2: solution), then keeps its rows
+      const int nsub = (szI + TB - 1) / TB;
+      for (int st = 0; st < nsub; ++st) {
+        const int b = FWD ? st : nsub - 1 - st;
+        const int ob = b * TB;
+        const double* D64 = dinv64 + ((I * (SB / TB) + b) * 2 + WHICH) * (int64_t)(TB * TB);
+        if (tid < TB) {
+          T a = 0;
+          for (int c2 = 0; c2 < TB; ++c2)
+            a = __builtin_fma(TRANS ? D64[c2 * TB + tid] : D64[tid * TB + c2], xs[ob + c2], a);
+          xs2[ob + tid] = ob + tid < szI ? a : T(0);
+        }
+        __syncthreads();
+        const int cend = szI - ob < TB ? szI - ob : TB;
+        for (int i = tid; i < szI; i += kSweepThreads) {
+          if (FWD ? i >= ob + TB : i < ob) {
+            T a = 0;
+            for (int c2 = 0; c2 < cend; ++c2) {
+              const int64_t j = ob + c2;
+              const TF t = TRANS ? F[(oI + j) * lda + oI + i] : F[(oI + i) * lda + oI + j];
+              a = __builtin_fma((T)t, xs2[j], a);
+            }
+            xs[i] -= a;
+          }
+        }
+        __syncthreads();
+      }
+      x = tid < nr ? xs2[rc0 + tid] : T(0);
+      goto publish;
+    }
+  }
   apply(dv, xs);
   __syncthreads();
-  T x = tid < nr ? row_value(tid) : T(0);
+  x = tid < nr ? row_value(tid) : T(0);
   if constexpr (REFINE) {
-    // x0 exchanged; r = acc - U_KK x0 (or U_KK^T x0) on the chunk's rows;
+    // x0 exchanged; r = acc - T_KK x0 (or T_KK^T x0) on the chunk's rows;
     // exchanged; x += D r.  The diagonal tile loads during the first
     // exchange, D's rows again during the second.
     T* X0 = P + n;
@@ -503,12 +597,14 @@ __global__ __launch_bounds__(kSweepThreads) void sbsweep_kernel(const TF* __rest
     __syncthreads();
     reduce_rows(pr);
     __syncthreads();
-    const T r_r = tid < nr ? xs[rc0 + tid] - row_value(tid) : T(0);
+    // L_KK has a unit diagonal that is not stored: its x0 term
+    const T r_r = tid < nr ? xs[rc0 + tid] - row_value(tid) - (WHICH == 0 ? xs2[rc0 + tid] : T(0)) : T(0);
     if (!exchange(Rb, 2, r_r, xs2)) return;
     apply(dv, xs2);
     __syncthreads();
     if (tid < nr) x += row_value(tid);
   }
+publish:
   if (tid < nr) st_coh(y + oI + rc0 + tid, x);
   __builtin_amdgcn_s_waitcnt(0);
   __syncthreads();
@@ -526,9 +622,15 @@ __global__ void reset_sweep_ctl_kernel(unsigned* ctl, int64_t nsb) {
   if (i < 2 || (i >= kCtlLine && i < kCtlLine * (nsb + 1))) ctl[i] = 0u;
 }
 
+// 64 x 64 inverses of L and U, then the 512 x 512 ones, then the condition
+// estimates of the 512 x 512 blocks.
+static int64_t dinv64_elems(int64_t n) { return ceil_div(n, TB) * 2 * TB * TB; }
+
 template <typename T>
 static int launch_sbsweep(bool fwd, bool trans, int which, const T* F, int64_t lda, int64_t n,
-                          const double* D512, double* y, double* P, unsigned* ctl, hipStream_t s) {
+                          const double* dinv, double* y, double* P, unsigned* ctl, hipStream_t s) {
+  const double* D512 = dinv + dinv64_elems(n);
+  const double* cond = D512 + ceil_div(n, (int64_t)SB) * 2 * (int64_t)SB * SB;
   const int64_t nsb = ceil_div(n, (int64_t)SB);
   hipLaunchKernelGGL(reset_sweep_ctl_kernel, dim3((unsigned)ceil_div(kCtlLine * (nsb + 1), 256)), dim3(256), 0, s,
                      ctl, nsb);
@@ -536,11 +638,11 @@ static int launch_sbsweep(bool fwd, bool trans, int which, const T* F, int64_t l
   const uint64_t ticks = device_info().wait_ticks;
   const unsigned grid = (unsigned)(nsb * NCH);
   const int pid = prof_start(sizeof(T) == 8 ? "sweep_f64" : "sweep_f32", s);
-  // one refinement step in the U blocks of FP64 factors
+  // one refinement step in the diagonal blocks of FP64 factors
   constexpr bool R64 = sizeof(T) == 8;
 #define RBFX_SBSWEEP(F_, T_, W_)                                                                              \
-  hipLaunchKernelGGL((sbsweep_kernel<T, F_, T_, W_, R64 && W_ == 1>), dim3(grid), dim3(kSweepThreads), 0, s, F, \
-                     lda, n, D512, y, P, ctl, ticks)
+  hipLaunchKernelGGL((sbsweep_kernel<T, F_, T_, W_, R64>), dim3(grid), dim3(kSweepThreads), 0, s, F, \
+                     lda, n, D512, dinv, cond, y, P, ctl, ticks)
   if (fwd && !trans && which == 0) RBFX_SBSWEEP(true, false, 0);
   else if (!fwd && !trans && which == 1) RBFX_SBSWEEP(false, false, 1);
   else if (fwd && trans && which == 1) RBFX_SBSWEEP(true, true, 1);
@@ -567,7 +669,6 @@ static bool carve_getrs(void* work, size_t work_bytes, int64_t n, double*& y, do
   ctl = ws.take<unsigned>(getrs_ctl_words(n));
   return y && P && ctl;
 }
-static int64_t dinv64_elems(int64_t n) { return ceil_div(n, TB) * 2 * TB * TB; }
 
 // The solve itself on carved scratch: y (n), P (3n), ctl (getrs_ctl_words).
 // Every scratch word that must start at zero is cleared by a kernel of the
@@ -575,14 +676,13 @@ static int64_t dinv64_elems(int64_t n) { return ceil_div(n, TB) * 2 * TB * TB; }
 template <typename T>
 static int getrs_run(int trans, int64_t n, const T* LU, int64_t lda, const int64_t* perm, const double* dinv,
                      double* b, double* y, double* P, unsigned* ctl, hipStream_t s) {
-  const double* D512 = dinv + dinv64_elems(n);
   const unsigned g = (unsigned)ceil_div(n, 256);
   int st;
   if (!trans) {
     hipLaunchKernelGGL(gather_perm_kernel, dim3(g), dim3(256), 0, s, b, perm, n, y, ctl, 1);
     RBFX_CHECK_LAUNCH();
-    if ((st = launch_sbsweep<T>(true, false, 0, LU, lda, n, D512, y, P, ctl, s))) return st;
-    if ((st = launch_sbsweep<T>(false, false, 1, LU, lda, n, D512, y, P, ctl, s))) return st;
+    if ((st = launch_sbsweep<T>(true, false, 0, LU, lda, n, dinv, y, P, ctl, s))) return st;
+    if ((st = launch_sbsweep<T>(false, false, 1, LU, lda, n, dinv, y, P, ctl, s))) return st;
     hipLaunchKernelGGL(scatter_perm_kernel, dim3(g), dim3(256), 0, s, y, (const int64_t*)nullptr, n, b,
                        ctl);
     RBFX_CHECK_LAUNCH();
@@ -590,8 +690,8 @@ static int getrs_run(int trans, int64_t n, const T* LU, int64_t lda, const int64
     hipLaunchKernelGGL(gather_perm_kernel, dim3(g), dim3(256), 0, s, b, (const int64_t*)nullptr, n, y,
                        ctl, 1);
     RBFX_CHECK_LAUNCH();
-    if ((st = launch_sbsweep<T>(true, true, 1, LU, lda, n, D512, y, P, ctl, s))) return st;
-    if ((st = launch_sbsweep<T>(false, true, 0, LU, lda, n, D512, y, P, ctl, s))) return st;
+    if ((st = launch_sbsweep<T>(true, true, 1, LU, lda, n, dinv, y, P, ctl, s))) return st;
+    if ((st = launch_sbsweep<T>(false, true, 0, LU, lda, n, dinv, y, P, ctl, s))) return st;
     hipLaunchKernelGGL(scatter_perm_kernel, dim3(g), dim3(256), 0, s, y, perm, n, b, ctl);
     RBFX_CHECK_LAUNCH();
   }
@@ -714,8 +814,13 @@ static int prepare(int64_t n, const T* LU, int64_t lda, const int64_t* ipiv, int
   RBFX_HIP(hipMemcpyAsync(perm, pm.data(), n * sizeof(int64_t), hipMemcpyHostToDevice, s));
   hipLaunchKernelGGL((diag_inv_kernel<T>), dim3((unsigned)ceil_div(n, TB)), dim3(TB), 0, s, LU, lda, n, dinv);
   RBFX_CHECK_LAUNCH();
-  hipLaunchKernelGGL((sbinv_kernel<T>), dim3((unsigned)(2 * ceil_div(n, (int64_t)SB))), dim3(256), 0, s, LU, lda, n,
-                     (const double*)dinv, dinv + dinv64_elems(n));
+  const int64_t nsb = ceil_div(n, (int64_t)SB);
+  double* d512 = dinv + dinv64_elems(n);
+  hipLaunchKernelGGL((sbinv_kernel<T>), dim3((unsigned)(2 * nsb)), dim3(256), 0, s, LU, lda, n, (const double*)dinv,
+                     d512);
+  RBFX_CHECK_LAUNCH();
+  hipLaunchKernelGGL((sbcond_kernel<T>), dim3((unsigned)(2 * nsb)), dim3(256), 0, s, LU, lda, n, (const double*)d512,
+                     d512 + nsb * 2 * (int64_t)SB * SB);
   RBFX_CHECK_LAUNCH();
   RBFX_HIP(hipStreamSynchronize(s));  // pm must outlive the copy
   return RBFX_OK;
@@ -736,7 +841,7 @@ extern "C" int rbfx_getrs_multi_f64(int64_t n, const double* d_LU, int64_t lda, 
 }
 
 extern "C" int64_t rbfx_getrs_dinv_elems(int64_t n) {
-  return n <= 0 ? 0 : dinv64_elems(n) + ceil_div(n, (int64_t)SB) * 2 * (int64_t)SB * SB;
+  return n <= 0 ? 0 : dinv64_elems(n) + ceil_div(n, (int64_t)SB) * (2 * (int64_t)SB * SB + 2);
 }
 
 extern "C" size_t rbfx_getrs_workspace(int64_t n) {
