@@ -203,13 +203,19 @@ struct OpCoef {
 };
 
 // apply_op for r2 = dist2(dx, dy) > 0 and r = sqrt(r2), explicit FMAs.
-__device__ inline double apply_op_pos(const Kern& k, const OpCoef& c, double dx, double dy, double r2, double r) {
+// HESS: the second-derivative terms (c.hess), a compile-time choice for
+// callers that branch once per row.
+template <bool HESS>
+__device__ inline double apply_op_pos_t(const Kern& k, const OpCoef& c, double dx, double dy, double r2, double r) {
   double phi, f1, g2;
   radial(k, r, phi, f1, g2);
   const double a1 = __builtin_fma(c.dx, dx, __builtin_fma(c.dy, dy, __builtin_fma(2.0, c.lap, c.dxx + c.dyy)));
   double hv = c.lap;
-  if (c.hess) hv += __builtin_fma(c.dxx * dx, dx, __builtin_fma(c.dyy * dy, dy, c.dxy * dx * dy)) / r2;
+  if (HESS) hv += __builtin_fma(c.dxx * dx, dx, __builtin_fma(c.dyy * dy, dy, c.dxy * dx * dy)) / r2;
   return __builtin_fma(c.v, phi, __builtin_fma(f1, a1, g2 * hv));
+}
+__device__ inline double apply_op_pos(const Kern& k, const OpCoef& c, double dx, double dy, double r2, double r) {
+  return c.hess ? apply_op_pos_t<true>(k, c, dx, dy, r2, r) : apply_op_pos_t<false>(k, c, dx, dy, r2, r);
 }
 
 __device__ inline double apply_op(const Kern& k, const OpCoef& c, double dx, double dy) {
