@@ -268,15 +268,17 @@ __global__ __launch_bounds__(256) void sbcond_kernel(const TF* __restrict__ LU, 
 // Superblocks complete in order (each folds its predecessor), so ctl[1] >= k
 // means superblocks 0 .. k-1 are solved.
 //
-// REFINE (FP64 factors, L and U): the inverse of an ill-conditioned
-// 512 x 512 diagonal block applied to the accumulated values loses digits
-// that a substitution keeps (measured on the adjoint gradients, and as a
-// backward error of 6e-9 against 8e-17 for substitution with the same
-// factors of a numerically singular system).  One step of refinement inside
-// the block restores them: x0 = D acc, r = acc - T_KK x0, x = x0 + D r, at two
-// more exchanges per superblock.  The FP32 factors of the preconditioner
-// need no refinement: an application error there only perturbs the
-// preconditioner.
+// REFINE (FP64 factors): the inverse of an ill-conditioned 512 x 512
+// diagonal block applied to the accumulated values loses digits that a
+// substitution keeps (measured on the adjoint gradients, and as a backward
+// error of 6e-9 against 8e-17 for substitution with the same factors of a
+// numerically singular system).  In U blocks one step of refinement inside
+// the block restores them for moderately conditioned blocks: x0 = D acc,
+// r = acc - U_KK x0, x = x0 + D r, at two more exchanges per superblock.
+// Blocks of L or U whose condition estimate exceeds kCondRobust are solved
+// by substitution over their 64-row sub-blocks instead.  The FP32 factors of
+// the preconditioner need neither: an application error there only perturbs
+// the preconditioner.
 constexpr int kSweepThreads = 512;
 constexpr int kSweepWaves = kSweepThreads / 64;
 template <typename TF, bool FWD, bool TRANS, int WHICH, bool REFINE>
@@ -580,7 +582,9 @@ __global__ __launch_bounds__(kSweepThreads) void sbsweep_kernel(const TF* __rest
   apply(dv, xs);
   __syncthreads();
   x = tid < nr ? row_value(tid) : T(0);
-  if constexpr (REFINE) {
+  // the refinement step in U blocks; L blocks (unit diagonal, multipliers
+  // at most 1) measured no better with it
+  if constexpr (REFINE && WHICH == 1) {
     // x0 exchanged; r = acc - T_KK x0 (or T_KK^T x0) on the chunk's rows;
     // exchanged; x += D r.  The diagonal tile loads during the first
     // exchange, D's rows again during the second.
