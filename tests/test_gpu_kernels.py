@@ -167,6 +167,34 @@ def test_getrf_matches_lapack_pivots():
     assert np.abs(lu.a[:, :n].cpu().numpy() - lu_ref).max() < 1e-10
 
 
+def test_getrf_pivot_ties_take_the_lowest_row():
+    """The pivot rule in its documented form: partial pivoting on |value|
+    with the 20 low mantissa bits of FP64 values cleared, ties to the lowest
+    row.  An exact tie and a near-tie (relative difference 2^-40, which
+    LAPACK resolves to the larger value) both go to the lower row; the
+    factorisation stays backward stable."""
+    from updes_amd import engine as E
+    _lib()
+    n = 300
+    for near in (False, True):
+        a = _random_well(n, 17)
+        a[:, 0] *= 0.4 / np.abs(a[:, 0]).max()
+        a[3, 0] = 1.0
+        a[5, 0] = 1.0 + (2.0 ** -40 if near else 0.0)
+        m = torch.zeros((n, E._ld(n)), dtype=torch.float64)
+        m[:, :n] = torch.as_tensor(a)
+        lu = E.LU(m.cuda(), n)
+        ipiv = lu.ipiv.cpu().numpy()
+        assert ipiv[0] == 3, (near, ipiv[0])
+        assert sla.lu_factor(a)[1][0] == (5 if near else 3)
+        LU = lu.a[:, :n].cpu().numpy()
+        Lm = np.tril(LU, -1) + np.eye(n)
+        pa = a.copy()
+        for i in range(n):
+            pa[[i, ipiv[i]]] = pa[[ipiv[i], i]]
+        assert np.abs(Lm @ np.triu(LU) - pa).max() < 1e-13 * n * np.abs(a).max()
+
+
 def test_expired_sweep_wait_fails_fast_and_the_next_solve_recovers():
     """Failure handling of the triangular sweeps, forced from the host: with a
     wait bound of 0 the waits that are not satisfied at once expire, the
