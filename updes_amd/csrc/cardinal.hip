@@ -36,8 +36,9 @@ constexpr int kCardMaxM = 15;
 constexpr int kCardPitch = 65;  // LDS row pitch (doubles) of the local system
 constexpr int kFormRows = 16;   // rows of K~ per workgroup of the gather formation kernel
 constexpr int kFormCols = 256;  // columns per block of the union tables (rbfx_cardinal)
-constexpr int kFormTR = 4;      // rows staged in LDS per pass of the LDS formation kernel
-constexpr int kFormUMax = 2048; // union entries of one block staged in LDS
+constexpr int kFormTR = 2;      // rows staged in LDS per pass of the LDS formation kernel
+constexpr int kFormUMax = 2560; // union entries of one block staged in LDS
+constexpr int kFormSlots = 5;   // union entries per thread and chunk of the staging loads
 constexpr int kFormRB = 512;    // rows per workgroup of the LDS formation kernel
 constexpr int kRedBlocks = 128; // partial sums of the small reductions
 
@@ -291,23 +292,28 @@ __global__ __launch_bounds__(256) void cardinal_form_kernel(FormArgs f, int64_t 
 }
 
 // K~ = diag(rs) D C in FP32 with the K entries staged in LDS.  Workgroup
-// (rb, cb) owns the 256 columns of block cb and the kFormRB rows of range rb:
-// each thread keeps its column's coefficients and the positions of its
-// stencil in the block's union U (sorted distinct stencil ids of the block)
-// in registers, and every pass stages K[rows, U] for kFormTR rows in LDS
-// (U is mostly contiguous runs, so the loads coalesce) and forms those rows
-// from LDS.  A block whose union exceeds kFormUMax gathers from global
-// memory instead.  Linear block b runs on XCD b % 8: b -> row range
-// (b % 8) + 8 * (b / 8 / ncb), column block (b / 8) % ncb, so the workgroups
-// of one XCD share the K rows they read in its L2.
-template <int KMAX>
-__global__ __launch_bounds__(256) void cardinal_form_lds_kernel(FormArgs f, const int64_t* __restrict__ uptr,
+// (rb, cb) owns the kFormCols columns of block cb and the kFormRB rows of
+// range rb.  Each thread keeps its column's coefficients, the LDS offsets of
+// its stencil entries within a staged row, and its share of the block's union
+// U (sorted distinct stencil ids, mostly contiguous runs, so the staging
+// loads coalesce) in registers.  Every pass stages K[rows, U] for kFormTR
+// rows: all of a chunk's loads are issued before any of them is written to
+// LDS, so a pass waits on memory once per chunk, not once per load.  A block
+// whose union exceeds kFormUMax re-reads its stencil ids and gathers from
+// global memory instead (slow, correct).  Linear block b runs on XCD b % 8:
+// b -> row range (b % 8) + 8 * (b / 8 / ncb), column block (b / 8) % ncb, so
+// the workgroups of one XCD share the K rows they read in its L2.
+template <int KMAX, int MMAX>
+__global__ __launch_bounds__(kFormCols) void cardinal_form_lds_kernel(FormArgs f, const int64_t* __restrict__ uptr,
                                                                 const int32_t* __restrict__ uidx,
                                                                 const int16_t* __restrict__ upos, int64_t ncb,
                                                                 int64_t nrb) {
   __shared__ double Ks[kFormTR][kFormUMax];
-  __shared__ double Kp[kFormTR][kCardMaxM];
-  constexpr int kUPerThread = kFormUMax / 256;
+  __shared__ double Kp[kFormTR][MMAX];
+  constexpr int kSlots = kFormUMax / kFormCols;     // union entries per thread
+  constexpr int kChunks = kSlots / kFormSlots;      // staged one chunk at a time
+  constexpr int kChunk = kFormSlots * kFormCols;    // union entries per chunk
+  static_assert(kSlots * kFormCols == kFormUMax && kChunks * kFormSlots == kSlots, "union staging");
   const int64_t b = blockIdx.x;
   const int64_t q = b >> 3;
   const int64_t rb = (q / ncb) * 8 + (b & 7);
@@ -315,61 +321,90 @@ __global__ __launch_bounds__(256) void cardinal_form_lds_kernel(FormArgs f, cons
   if (rb >= nrb) return;
   const int tid = threadIdx.x;
   const int64_t u0 = uptr[cb];
-  const int64_t nu = uptr[cb + 1] - u0;
-  const bool lds = nu >= 0 && nu <= kFormUMax;  // uniform over the workgroup
+  const int64_t nu64 = uptr[cb + 1] - u0;
+  const bool lds = nu64 >= 0 && nu64 <= kFormUMax;  // uniform over the workgroup
+  const int nu = lds ? (int)nu64 : 0;
   const int64_t j = cb * kFormCols + tid;
   const bool act = j < f.N;
-  int32_t pj[KMAX];
+  const int k = (int)f.k, M = (int)f.M;
+  // coefficients and LDS byte offsets of this column's stencil entries
+  // (loads issued unconditionally from clamped indices, then checked)
+  uint32_t la[KMAX];
   double bj[KMAX];
-  double aj[kCardMaxM];
+  double aj[MMAX];
 #pragma unroll
   for (int t = 0; t < KMAX; ++t) {
-    const bool ok = act && t < f.k;
-    int32_t p = 0;
-    if (ok) {
-      if (lds) {
-        p = upos[j * f.k + t];
-        p = (p >= 0 && p < nu) ? p : 0;
-      } else {
-        p = (int32_t)checked_id(f.stencil[j * f.k + t], f.N, j, f.status);
-      }
-    }
-    pj[t] = p;
-    bj[t] = ok ? f.beta[j * f.k + t] : 0.0;
+    const bool ok = act && t < k;
+    const int64_t e = ok ? j * f.k + t : 0;
+    const int p = upos[e];
+    bj[t] = f.beta[e];
+    la[t] = (uint32_t)p;
+    if (!ok) bj[t] = 0.0;
+    if (!(ok && p >= 0 && p < nu)) la[t] = 0u;
   }
 #pragma unroll
-  for (int m = 0; m < kCardMaxM; ++m) aj[m] = (act && m < f.M) ? f.alpha[j * f.M + m] : 0.0;
-  // this thread's share of the union ids (staging loads)
-  int32_t myu[kUPerThread];
+  for (int t = 0; t < KMAX; ++t) la[t] *= (uint32_t)sizeof(double);
 #pragma unroll
-  for (int e = 0; e < kUPerThread; ++e) {
-    const int64_t u = tid + (int64_t)e * 256;
-    myu[e] = (lds && u < nu) ? (int32_t)checked_id(uidx[u0 + u], f.N, 0, f.status) : 0;
+  for (int m = 0; m < MMAX; ++m) {
+    const bool ok = act && m < M;
+    const double a = f.alpha[ok ? j * f.M + m : 0];
+    aj[m] = ok ? a : 0.0;
   }
+  // byte offsets within a K row of this thread's union entries
+  uint32_t uo[kSlots];
+#pragma unroll
+  for (int e = 0; e < kSlots; ++e) {
+    const int u = tid + e * kFormCols;
+    uo[e] = (uint32_t)uidx[u < nu ? u0 + u : (nu > 0 ? u0 : 0)];
+  }
+#pragma unroll
+  for (int e = 0; e < kSlots; ++e) {
+    const int u = tid + e * kFormCols;
+    const int32_t id = (int32_t)uo[e];
+    uo[e] = (u < nu ? (uint32_t)checked_id(id, f.N, 0, f.status) : 0u) * (uint32_t)sizeof(double);
+  }
+  // this thread's entry of the staged polynomial columns
+  const int prr = tid / MMAX, pm = tid - prr * MMAX;
+  const bool pw = prr < kFormTR && pm < M;
+  const int64_t pcol = M > 0 ? f.N + (pm < M ? pm : M - 1) : 0;
   const int64_t rbeg = rb * kFormRB;
   const int64_t rend = rbeg + kFormRB < f.N ? rbeg + kFormRB : f.N;
   for (int64_t r0 = rbeg; r0 < rend; r0 += kFormTR) {
     const int nr = (int)(rend - r0 < kFormTR ? rend - r0 : kFormTR);
     if (lds) {
       __syncthreads();  // the previous pass is done with Ks
-      for (int rr = 0; rr < nr; ++rr) {
-        const double* Ki = f.K + (r0 + rr) * f.ld;
+      const double kp = f.K[(r0 + (prr < nr ? prr : nr - 1)) * f.ld + pcol];
 #pragma unroll
-        for (int e = 0; e < kUPerThread; ++e) {
-          const int64_t u = tid + (int64_t)e * 256;
-          if (u < nu) Ks[rr][u] = Ki[myu[e]];
+      for (int c = 0; c < kChunks; ++c) {
+        if (c * kChunk < nu) {
+          double v[kFormTR][kFormSlots];
+#pragma unroll
+          for (int rr = 0; rr < kFormTR; ++rr) {
+            const char* Ki = reinterpret_cast<const char*>(f.K + (r0 + (rr < nr ? rr : nr - 1)) * f.ld);
+#pragma unroll
+            for (int s = 0; s < kFormSlots; ++s)
+              v[rr][s] = *reinterpret_cast<const double*>(Ki + uo[c * kFormSlots + s]);
+          }
+#pragma unroll
+          for (int rr = 0; rr < kFormTR; ++rr)
+#pragma unroll
+            for (int s = 0; s < kFormSlots; ++s) {
+              const int u = c * kChunk + s * kFormCols + tid;
+              if (u < nu) Ks[rr][u] = v[rr][s];
+            }
         }
-        if (tid < f.M) Kp[rr][tid] = Ki[f.N + tid];
       }
+      if (pw) Kp[prr][pm] = kp;
       __syncthreads();
       for (int rr = 0; rr < nr; ++rr) {
+        const char* row = reinterpret_cast<const char*>(&Ks[rr][0]);
         double acc = 0.0;
 #pragma unroll
         for (int t = 0; t < KMAX; ++t)
-          if (t < f.k) acc = __builtin_fma(Ks[rr][pj[t]], bj[t], acc);
+          if (t < k) acc = __builtin_fma(*reinterpret_cast<const double*>(row + la[t]), bj[t], acc);
 #pragma unroll
-        for (int m = 0; m < kCardMaxM; ++m)
-          if (m < f.M) acc = __builtin_fma(Kp[rr][m], aj[m], acc);
+        for (int m = 0; m < MMAX; ++m)
+          if (m < M) acc = __builtin_fma(Kp[rr][m], aj[m], acc);
         if (act) f.out[(r0 + rr) * f.ld32 + j] = (float)(f.rs[r0 + rr] * acc);
       }
     } else {
@@ -378,10 +413,13 @@ __global__ __launch_bounds__(256) void cardinal_form_lds_kernel(FormArgs f, cons
         double acc = 0.0;
 #pragma unroll
         for (int t = 0; t < KMAX; ++t)
-          if (t < f.k) acc = __builtin_fma(Ki[pj[t]], bj[t], acc);
+          if (t < k && act) {
+            const int64_t id = checked_id(f.stencil[j * f.k + t], f.N, j, f.status);
+            acc = __builtin_fma(Ki[id], bj[t], acc);
+          }
 #pragma unroll
-        for (int m = 0; m < kCardMaxM; ++m)
-          if (m < f.M) acc = __builtin_fma(Ki[f.N + m], aj[m], acc);
+        for (int m = 0; m < MMAX; ++m)
+          if (m < M) acc = __builtin_fma(Ki[f.N + m], aj[m], acc);
         if (act) f.out[(r0 + rr) * f.ld32 + j] = (float)(f.rs[r0 + rr] * acc);
       }
     }
@@ -750,9 +788,14 @@ extern "C" int rbfx_cardinal_form(const rbfx_cardinal* c, const double* d_K64, i
     const int64_t nrb = ceil_div(c->N, kFormRB);
     const int64_t blocks = ceil_div(nrb, 8) * 8 * ncb;
     RBFX_REQUIRE(blocks < (int64_t)UINT32_MAX, "cardinal_form grid too large");
-#define RBFX_FORM(KM)                                                                                         \
-  hipLaunchKernelGGL((cardinal_form_lds_kernel<KM>), dim3((unsigned)blocks), dim3(256), 0, s, f, c->d_union_ptr, \
-                     c->d_union_idx, c->d_union_pos, ncb, nrb)
+#define RBFX_FORM_M(KM, MM)                                                                                \
+  hipLaunchKernelGGL((cardinal_form_lds_kernel<KM, MM>), dim3((unsigned)blocks), dim3(kFormCols), 0, s, f, \
+                     c->d_union_ptr, c->d_union_idx, c->d_union_pos, ncb, nrb)
+#define RBFX_FORM(KM)      \
+  if (c->M <= 6)           \
+    RBFX_FORM_M(KM, 6);    \
+  else                     \
+    RBFX_FORM_M(KM, kCardMaxM)
     if (c->k <= 16)
       RBFX_FORM(16);
     else if (c->k <= 24)
@@ -762,6 +805,7 @@ extern "C" int rbfx_cardinal_form(const rbfx_cardinal* c, const double* d_K64, i
     else
       RBFX_FORM(48);
 #undef RBFX_FORM
+#undef RBFX_FORM_M
   } else {
     const int64_t nrt = ceil_div(c->N, kFormRows);
     const int64_t blocks = ceil_div(nrt, 8) * 8 * ncb;
