@@ -15,6 +15,34 @@ def _lib():
     return L
 
 
+@pytest.mark.parametrize("trans", [0, 1])
+@pytest.mark.parametrize("m,n,lda,alpha,beta", [(1, 1, 1, 1.0, 0.0), (5, 40006, 40006, -1.0, 1.0),
+                                                (7, 301, 301, 2.0, 0.5), (1000, 257, 264, 1.0, 0.0),
+                                                (3001, 3001, 3008, -1.0, 1.0), (2049, 130, 131, 1.0, 0.0),
+                                                (64, 129, 130, 1.0, -1.0)])
+def test_gemv_matches_numpy(trans, m, n, lda, alpha, beta):
+    """rbfx_gemv_f64, both orientations: the row-per-wave product and the
+    transposed product over row chunks (paired 16-byte columns where lda is
+    even, single columns otherwise, odd column counts, row counts that are
+    not multiples of the four waves), against NumPy in FP64."""
+    L = _lib()
+    rng = np.random.default_rng(m + 3 * n + lda)
+    A = rng.standard_normal((m, lda))
+    x = rng.standard_normal(m if trans else n)
+    y0 = rng.standard_normal(n if trans else m)
+    dA = torch.as_tensor(A).cuda()
+    dx = torch.as_tensor(x).cuda()
+    dy = torch.as_tensor(y0).cuda()
+    ws = L.workspace(L.lib().rbfx_gemv_workspace(trans, m, n), "cuda")
+    L.check(L.lib().rbfx_gemv_f64(trans, m, n, alpha, L.ptr(dA), lda, L.ptr(dx), beta, L.ptr(dy), L.ptr(ws),
+                                  ws.numel(), L.stream()))
+    Am = A[:, :n]
+    ref = alpha * ((Am.T @ x) if trans else (Am @ x)) + beta * y0
+    got = dy.cpu().numpy()
+    scale = np.abs(Am).sum(axis=0 if trans else 1).max() * np.abs(x).max() * abs(alpha) + np.abs(y0).max()
+    assert np.abs(got - ref).max() <= 1e-14 * scale * max(1, (m if trans else n) ** 0.5)
+
+
 @pytest.mark.parametrize("dtype", [torch.float64, torch.float32])
 @pytest.mark.parametrize("m,n,k,beta", [(1, 1, 1, 1.0), (130, 70, 17, 1.0), (300, 257, 64, 1.0),
                                          (512, 384, 256, 1.0), (300, 258, 64, 1.0), (130, 72, 16, 0.0),

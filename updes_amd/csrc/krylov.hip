@@ -74,18 +74,65 @@ __global__ __launch_bounds__(kGvThreads) void gemv_n_wave_kernel(int64_t m, int6
   if (lane == 0) y[i] = alpha * s + (beta != 0.0 ? beta * y[i] : 0.0);
 }
 
-// y[j] = alpha * sum_i A[i][j] x[i] + beta * y[j]: thread per column over a
-// chunk of rows; partial sums reduced by a second kernel when chunked.
+// y[j] = alpha * sum_i A[i][j] x[i] + beta * y[j], partial sums over row
+// chunks (reduced by gemv_t_finish_kernel).  Workgroup (column block of
+// kGvTCols, row chunk): lane l of wave w owns columns 128 b + 2 l and
+// 128 b + 2 l + 1 and the rows rs + w, rs + w + 4, ... of the chunk, with four
+// 16-byte row loads in flight; the four waves' sums meet in LDS.
+constexpr int kGvTCols = 128;
 __global__ __launch_bounds__(kGvThreads) void gemv_t_kernel(int64_t m, int64_t n, const double* __restrict__ A,
                                                             int64_t lda, const double* __restrict__ x,
                                                             int64_t rows_per_chunk, double* __restrict__ part) {
-  const int64_t j = (int64_t)blockIdx.x * kGvThreads + threadIdx.x;
-  if (j >= n) return;
+  typedef double vd2 __attribute__((ext_vector_type(2)));
+  __shared__ double red[kGvThreads / 64][kGvTCols];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int64_t c0 = (int64_t)blockIdx.x * kGvTCols + 2 * lane;
   const int64_t rs = (int64_t)blockIdx.y * rows_per_chunk;
   const int64_t re = rs + rows_per_chunk < m ? rs + rows_per_chunk : m;
-  double s = 0.0;
-  for (int64_t i = rs; i < re; ++i) s += A[i * lda + j] * x[i];
-  part[(int64_t)blockIdx.y * n + j] = s;
+  double s0 = 0.0, s1 = 0.0;
+  const bool vec = ((lda & 1) == 0) && ((reinterpret_cast<uintptr_t>(A) & 15) == 0);
+  if (vec && c0 + 1 < n) {
+    const double* a = A + c0;
+    int64_t i = rs + w;
+    for (; i + 12 < re; i += 16) {
+      const vd2 a0 = __builtin_nontemporal_load(reinterpret_cast<const vd2*>(a + i * lda));
+      const vd2 a1 = __builtin_nontemporal_load(reinterpret_cast<const vd2*>(a + (i + 4) * lda));
+      const vd2 a2 = __builtin_nontemporal_load(reinterpret_cast<const vd2*>(a + (i + 8) * lda));
+      const vd2 a3 = __builtin_nontemporal_load(reinterpret_cast<const vd2*>(a + (i + 12) * lda));
+      const double x0 = x[i], x1 = x[i + 4], x2 = x[i + 8], x3 = x[i + 12];
+      s0 = __builtin_fma(a0.x, x0, s0);
+      s1 = __builtin_fma(a0.y, x0, s1);
+      s0 = __builtin_fma(a1.x, x1, s0);
+      s1 = __builtin_fma(a1.y, x1, s1);
+      s0 = __builtin_fma(a2.x, x2, s0);
+      s1 = __builtin_fma(a2.y, x2, s1);
+      s0 = __builtin_fma(a3.x, x3, s0);
+      s1 = __builtin_fma(a3.y, x3, s1);
+    }
+    for (; i < re; i += 4) {
+      const vd2 a0 = __builtin_nontemporal_load(reinterpret_cast<const vd2*>(a + i * lda));
+      s0 = __builtin_fma(a0.x, x[i], s0);
+      s1 = __builtin_fma(a0.y, x[i], s1);
+    }
+  } else if (c0 < n) {
+    const bool two = c0 + 1 < n;
+    for (int64_t i = rs + w; i < re; i += 4) {
+      s0 = __builtin_fma(A[i * lda + c0], x[i], s0);
+      if (two) s1 = __builtin_fma(A[i * lda + c0 + 1], x[i], s1);
+    }
+  }
+  red[w][2 * lane] = s0;
+  red[w][2 * lane + 1] = s1;
+  __syncthreads();
+  if (threadIdx.x < kGvTCols) {
+    const int64_t j = (int64_t)blockIdx.x * kGvTCols + threadIdx.x;
+    if (j < n) {
+      double t = red[0][threadIdx.x];
+#pragma unroll
+      for (int q = 1; q < kGvThreads / 64; ++q) t += red[q][threadIdx.x];
+      part[(int64_t)blockIdx.y * n + j] = t;
+    }
+  }
 }
 
 __global__ void gemv_t_finish_kernel(int64_t n, int64_t chunks, const double* __restrict__ part,
@@ -97,9 +144,10 @@ __global__ void gemv_t_finish_kernel(int64_t n, int64_t chunks, const double* __
   y[j] = alpha * s + (beta != 0.0 ? beta * y[j] : 0.0);
 }
 
+// Row chunks of the transposed product: about eight workgroups per CU.
 static int64_t gemv_t_chunks(int64_t m, int64_t n) {
-  const int64_t colb = ceil_div(n, kGvThreads);
-  int64_t want = ceil_div(4 * (int64_t)device_info().num_cus, colb);
+  const int64_t colb = ceil_div(n, kGvTCols);
+  int64_t want = ceil_div(8 * (int64_t)device_info().num_cus, colb);
   want = std::max<int64_t>(1, std::min<int64_t>(want, ceil_div(m, 64)));
   return want;
 }
@@ -143,7 +191,7 @@ int gemv_launch(int trans, int64_t m, int64_t n, double alpha, const double* A, 
   const int64_t rpc = ceil_div(m, chunks);
   const int64_t used = ceil_div(m, rpc);
   RBFX_REQUIRE(part != nullptr, "gemv^T needs a workspace");
-  hipLaunchKernelGGL(gemv_t_kernel, dim3((unsigned)ceil_div(n, kGvThreads), (unsigned)used), dim3(kGvThreads),
+  hipLaunchKernelGGL(gemv_t_kernel, dim3((unsigned)ceil_div(n, kGvTCols), (unsigned)used), dim3(kGvThreads),
                      0, s, m, n, A, lda, x, rpc, part);
   RBFX_CHECK_LAUNCH();
   hipLaunchKernelGGL(gemv_t_finish_kernel, dim3((unsigned)ceil_div(n, 256)), dim3(256), 0, s, n, used, part,
