@@ -113,6 +113,28 @@ def test_local_matvecs_match_assembled(fs, k):
             assert err < 1e-13
 
 
+@pytest.mark.parametrize("kind,param", C.KINDS)
+@pytest.mark.parametrize("M", [0, 3, 6])
+def test_value_matvec_matches_assembled(kind, param, M):
+    """A x and A^T x with every node as support (the value-row kernel, whose
+    r comes from the square-root sequence without range scaling where the
+    wave allows, then the P^T rows) for every kernel family, with and
+    without monomials, against the assembled A."""
+    import updes_amd as u
+    from updes_amd import _lib as L, engine as E
+    uc = u.SquareCloud(Nx=29, Ny=27, facet_types=C.FACET_SETS[1], jitter=C.jitter(29, 27, 0.2, M),
+                       support_size="max")
+    dc = E.device_cloud(uc)
+    prob = E.Problem(dc, kind, param, M, E.OpDesc.builtin(L.LAP), None)
+    x = torch.randn(prob.n, generator=torch.Generator().manual_seed(M + 7 * kind), dtype=torch.float64).cuda()
+    Ad = E.assemble(prob, L.MAT_A, torch.float64)[:, :prob.n]
+    for trans in (False, True):
+        y = E.matvec(prob, x, L.MAT_A, trans=trans)
+        ref = (Ad.T if trans else Ad) @ x
+        err = (y - ref).abs().max().item() / ref.abs().max().item()
+        assert err < 1e-13, (trans, err)
+
+
 @pytest.mark.parametrize("fs", [0, 1, 2, 3, 5])
 @pytest.mark.parametrize("k", [10, 25])
 def test_local_pde_solver_matches_reference_algorithm(fs, k):

@@ -385,9 +385,9 @@ constexpr int kMvRows = 4;
 
 template <int KIND, bool FAST, bool LOC>
 __global__ __launch_bounds__(kMvThreads) void matvec_kernel(ProbDev P, const double* __restrict__ xv,
-                                                            double* __restrict__ yv) {
+                                                            double* __restrict__ yv, int64_t row_base) {
   pin_kind<KIND, FAST>(P.k);
-  const int64_t r0 = (int64_t)blockIdx.x * kMvRows;
+  const int64_t r0 = row_base + (int64_t)blockIdx.x * kMvRows;
   RowCtx c[kMvRows];
   double acc[kMvRows];
 #pragma unroll
@@ -620,6 +620,76 @@ __global__ void sum_chunks_kernel(const double* __restrict__ part, int64_t chunk
   double s = 0.0;
   for (int64_t c = 0; c < chunks; ++c) s += part[c * n + j];
   y[j] = s;
+}
+
+// y = A x over the value rows of A (rows < N, every node in every support):
+// kMvRows rows per workgroup, each thread strides over the centre columns,
+// then the monomial columns.  Where every r^2 of the wave is in sqrt_mid's
+// range r comes from sqrt_mid (the same bits as sqrt), so the entries are
+// elem_rbf's for these rows; the rows of P^T go through matvec_kernel.
+template <int KIND, bool FAST>
+__global__ __launch_bounds__(kMvThreads) void matvec_value_kernel(ProbDev P, const double* __restrict__ xv,
+                                                                  double* __restrict__ yv) {
+  pin_kind<KIND, FAST>(P.k);
+  const int64_t r0 = (int64_t)blockIdx.x * kMvRows;
+  int64_t ri[kMvRows];
+  double xi[kMvRows], yi[kMvRows], acc[kMvRows];
+#pragma unroll
+  for (int q = 0; q < kMvRows; ++q) {
+    ri[q] = r0 + q < P.N ? r0 + q : P.N - 1;
+    xi[q] = P.x[ri[q]];
+    yi[q] = P.y[ri[q]];
+    acc[q] = 0.0;
+  }
+  for (int64_t j = threadIdx.x; j < P.N; j += kMvThreads) {
+    const double w = xv[j], xj = P.x[j], yj = P.y[j];
+    double d2[kMvRows];
+    bool in = true;
+#pragma unroll
+    for (int q = 0; q < kMvRows; ++q) {
+      d2[q] = dist2(xi[q] - xj, yi[q] - yj);
+      in = in && in_sqrt_mid(d2[q]);
+    }
+    if (__all(in)) {
+#pragma unroll
+      for (int q = 0; q < kMvRows; ++q) {
+        const double r = d2[q] > 0.0 ? sqrt_mid(d2[q]) : 0.0;
+        const double e = j == ri[q] ? 0.0 : radial_value(P.k, r);
+        acc[q] = __builtin_fma(e, w, acc[q]);
+      }
+    } else {
+#pragma unroll
+      for (int q = 0; q < kMvRows; ++q) {
+        const double e = j == ri[q] ? 0.0 : radial_value(P.k, sqrt(d2[q]));
+        acc[q] = __builtin_fma(e, w, acc[q]);
+      }
+    }
+  }
+  if (threadIdx.x < P.M) {
+    const int m = threadIdx.x;
+    const double w = xv[P.N + m];
+#pragma unroll
+    for (int q = 0; q < kMvRows; ++q) {
+      double o[RBFX_NOPS];
+      mono_ops(m, xi[q], yi[q], o);
+      acc[q] = __builtin_fma(o[RBFX_VAL], w, acc[q]);
+    }
+  }
+  __shared__ double red[kMvRows][kMvThreads / 64];
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+#pragma unroll
+  for (int q = 0; q < kMvRows; ++q) {
+    double v = acc[q];
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+    if (lane == 0) red[q][wid] = v;
+  }
+  __syncthreads();
+  if (threadIdx.x < kMvRows) {
+    const int q = threadIdx.x;
+    double t = 0.0;
+    for (int w = 0; w < kMvThreads / 64; ++w) t += red[q][w];
+    if (r0 + q < P.N) yv[r0 + q] = t;
+  }
 }
 
 static int make_prob(const rbfx_problem* p, int matrix, ProbDev& P) {
@@ -914,9 +984,18 @@ extern "C" int rbfx_matvec(const rbfx_problem* p, int matrix, int trans, const d
       if (P.local && !trans)
         hipLaunchKernelGGL((matvec_local_kernel<KIND_, FAST_>), grid_loc, dim3(kLocThreads), 0, s, P, d_x, d_y);
       else if (P.local)
-        hipLaunchKernelGGL((matvec_kernel<KIND_, FAST_, true>), grid, dim3(kMvThreads), 0, s, P, d_x, d_y);
-      else
-        hipLaunchKernelGGL((matvec_kernel<KIND_, FAST_, false>), grid, dim3(kMvThreads), 0, s, P, d_x, d_y);
+        hipLaunchKernelGGL((matvec_kernel<KIND_, FAST_, true>), grid, dim3(kMvThreads), 0, s, P, d_x, d_y,
+                           (int64_t)0);
+      else if (matrix == RBFX_MAT_A) {
+        // the value rows, then the rows of P^T
+        hipLaunchKernelGGL((matvec_value_kernel<KIND_, FAST_>), dim3((unsigned)ceil_div(P.N, kMvRows)),
+                           dim3(kMvThreads), 0, s, P, d_x, d_y);
+        if (P.M > 0)
+          hipLaunchKernelGGL((matvec_kernel<KIND_, FAST_, false>), dim3((unsigned)ceil_div(P.M, kMvRows)),
+                             dim3(kMvThreads), 0, s, P, d_x, d_y, P.N);
+      } else
+        hipLaunchKernelGGL((matvec_kernel<KIND_, FAST_, false>), grid, dim3(kMvThreads), 0, s, P, d_x, d_y,
+                           (int64_t)0);
     });
     prof_stop(pid, s, 0.0, 16.0 * P.n);
     RBFX_CHECK_LAUNCH();
