@@ -44,17 +44,38 @@ def test_cardinal_coefficients_match_numpy(kind, param, M):
     assert (np.abs(alpha - alpha_ref) / sa).max() <= 1e-8
 
 
-@pytest.mark.parametrize("kind,param,nx,union", [(R.POLYHARMONIC, 1.0, 12, True), (R.GAUSSIAN, 5.0, 12, True),
-                                                  (R.THIN_PLATE, 1.0, 12, True), (R.POLYHARMONIC, 1.0, 12, False),
-                                                  (R.POLYHARMONIC, 1.0, 40, True)])
-def test_cardinal_form_matches_numpy(kind, param, nx, union):
-    """Both formation kernels: K rows staged in LDS over the per-block stencil
-    unions (several 256-column blocks and row ranges at 40 x 40), and gathered
-    from global memory."""
+def _widen_unions(basis, N):
+    """Union tables whose every block lists all N node ids (a valid superset
+    of each block's stencil ids, position = id), so that no block fits the
+    LDS staging and the kernels take their global-memory paths."""
+    ncb = -(-N // 256)
+    st = basis.stencil.reshape(N, basis.k)
+    ptr = torch.arange(ncb + 1, dtype=torch.int64) * N
+    idx = torch.arange(N, dtype=torch.int32).repeat(ncb)
+    basis.union = (ptr.cuda(), idx.cuda(), st.to(torch.int16).contiguous())
+
+
+@pytest.mark.parametrize("kind,param,nx,union,k,M", [
+    (R.POLYHARMONIC, 1.0, 12, True, 20, 6), (R.GAUSSIAN, 5.0, 12, True, 20, 6), (R.THIN_PLATE, 1.0, 12, True, 20, 6),
+    (R.POLYHARMONIC, 1.0, 12, False, 20, 6), (R.POLYHARMONIC, 1.0, 40, True, 20, 6),
+    (R.POLYHARMONIC, 1.0, 40, True, 15, 6), (R.GAUSSIAN, 5.0, 12, True, 15, 3), (R.POLYHARMONIC, 1.0, 40, True, 7, 6),
+    (R.POLYHARMONIC, 1.0, 40, True, 15, 10), (R.POLYHARMONIC, 1.0, 40, "wide", 15, 6),
+    (R.POLYHARMONIC, 1.0, 52, "wide", 20, 6)])
+def test_cardinal_form_matches_numpy(kind, param, nx, union, k, M):
+    """Every formation kernel: stencils of at most 24 entries with at most 6
+    monomials through the LDS ring (producer waves streaming the rows of
+    K[:, U], consumer waves contracting them; k = 7 and 15 give 16 and 24
+    entries with the 9 special points), larger stencils or more monomials
+    through the batched LDS staging (k = 20: 29 entries; M = 10), both over
+    several 256-column blocks and row ranges at 40 x 40; blocks whose union
+    does not fit LDS ("wide": every id listed) from global memory in either
+    kernel; and the gather kernel without union tables."""
     from updes_amd import engine as E, _lib as L
-    M = 6
-    oc, uc, prob, basis = _setup(kind, param, M, nx=nx, ny=nx - 1, facets=C.FACET_SETS[2])
-    basis.use_union = union
+    oc, uc, prob, basis = _setup(kind, param, M, nx=nx, ny=nx - 1, facets=C.FACET_SETS[2], k=k)
+    if union == "wide":
+        _widen_unions(basis, oc.N)
+    else:
+        basis.use_union = union
     K64 = E.assemble(prob, L.MAT_K, torch.float64)
     rs = torch.empty(oc.N, dtype=torch.float64, device="cuda")
     Kt = basis.form(K64, rs)[:, :oc.N].cpu().numpy()

@@ -39,7 +39,15 @@ constexpr int kFormCols = 256;  // columns per block of the union tables (rbfx_c
 constexpr int kFormTR = 2;      // rows staged in LDS per pass of the LDS formation kernel
 constexpr int kFormUMax = 2560; // union entries of one block staged in LDS
 constexpr int kFormSlots = 5;   // union entries per thread and chunk of the staging loads
-constexpr int kFormRB = 512;    // rows per workgroup of the LDS formation kernel
+constexpr int kFormRB = 512;    // rows per workgroup of the LDS formation kernels
+// ring formation kernel (k <= 24, M <= 6): rows of K[:, U] streamed into an
+// LDS ring by producer waves, kRingDepth rows ahead of the consumer waves
+constexpr int kRingUMax = 1280;                    // union entries + monomial columns per ring slot
+constexpr int kRingDepth = 6;                      // rows in flight
+constexpr int kRingSlots = kRingDepth + 2;         // a slot is refilled two rows after its last read
+constexpr int kRingLoads = 2 * kRingUMax / kFormCols;  // 4-byte loads per producer lane and row
+static_assert(kRingLoads * kFormCols == 2 * kRingUMax && kRingLoads * kRingDepth <= 63, "ring loads");
+static_assert((kRingSlots & (kRingSlots - 1)) == 0, "ring slots: a power of two");
 constexpr int kRedBlocks = 128; // partial sums of the small reductions
 
 __device__ inline double wave_sum_d(double v) {
@@ -426,6 +434,151 @@ __global__ __launch_bounds__(kFormCols) void cardinal_form_lds_kernel(FormArgs f
   }
 }
 
+// K~ = diag(rs) D C with the rows of K[:, U] (U: the block's union, then the
+// M monomial columns) streamed through a ring of kRingSlots LDS slots.
+// Workgroup (rb, cb) of 2 x kFormCols threads: the first kFormCols are
+// consumers (thread = column of block cb: coefficients and LDS offsets of
+// its stencil in registers), the others producers that only issue
+// direct-to-LDS 4-byte loads (lane l of a producer wave loads half l & 1 of
+// one union entry, so 64 lanes fill 32 consecutive entries of a slot).
+// Producers run kRingDepth rows ahead and wait on their own load counter
+// for the row the consumers read next; one barrier per row hands it over.
+// Producers never store and consumers never issue the ring's loads, so each
+// wave's counter covers exactly its own kind of work.  A block whose union
+// does not fit a slot is formed by the consumers from global memory.
+// XCD-aware block order as in cardinal_form_lds_kernel.
+template <int KMAX, int MMAX>
+__global__ __launch_bounds__(2 * kFormCols) __attribute__((amdgpu_waves_per_eu(4, 4))) void cardinal_form_ring_kernel(FormArgs f, const int64_t* __restrict__ uptr,
+                                                                     const int32_t* __restrict__ uidx,
+                                                                     const int16_t* __restrict__ upos, int64_t ncb,
+                                                                     int64_t nrb, const double* __restrict__ rsv) {
+  __shared__ __align__(16) double ring[kRingSlots][kRingUMax];
+  const int64_t b = blockIdx.x;
+  const int64_t q = b >> 3;
+  const int64_t rb = (q / ncb) * 8 + (b & 7);
+  const int64_t cb = q % ncb;
+  if (rb >= nrb) return;
+  const int tid = threadIdx.x;
+  const int k = (int)f.k, M = (int)f.M;
+  const int64_t u0 = uptr[cb];
+  const int64_t nu64 = uptr[cb + 1] - u0;
+  const bool fits = nu64 >= 0 && nu64 + M <= kRingUMax;  // uniform over the workgroup
+  const int nu = fits ? (int)nu64 : 0;
+  const int64_t rbeg = rb * kFormRB;
+  const int64_t rend = rbeg + kFormRB < f.N ? rbeg + kFormRB : f.N;
+  const int nrows = (int)(rend - rbeg);
+
+  if (tid >= kFormCols) {
+    if (!fits) return;
+    // ---- producer ----
+    const int p = tid - kFormCols, pw = p >> 6, lane = p & 63;
+    constexpr int kPW = kFormCols / 64;  // producer waves
+    uint32_t go[kRingLoads];  // byte offsets within a K row of this lane's 4-byte words
+#pragma unroll
+    for (int e = 0; e < kRingLoads; ++e) {
+      const int u = (e * kPW + pw) * 32 + (lane >> 1);
+      const int32_t raw = uidx[u < nu ? u0 + u : (nu > 0 ? u0 : 0)];
+      int64_t id = 0;
+      if (u < nu)
+        id = checked_id(raw, f.N, 0, f.status);
+      else if (u < nu + M)
+        id = f.N + (u - nu);
+      go[e] = (uint32_t)id * (uint32_t)sizeof(double) + 4u * (uint32_t)(lane & 1);
+    }
+    auto issue = [&](int i) {  // row rbeg + i (clamped: every row issues the same loads) into its slot
+      const int64_t row = rbeg + (i < nrows ? i : nrows - 1);
+      const char* Kr = reinterpret_cast<const char*>(f.K + row * f.ld);
+      char* slot = reinterpret_cast<char*>(&ring[i & (kRingSlots - 1)][0]);
+#pragma unroll
+      for (int e = 0; e < kRingLoads; ++e)
+        __builtin_amdgcn_global_load_lds((const void*)(Kr + go[e]),
+                                         (__attribute__((address_space(3))) void*)(slot + (e * kPW + pw) * 256), 4,
+                                         0, 0);
+    };
+    for (int d = 0; d < kRingDepth; ++d) issue(d);
+    // s_waitcnt vmcnt(kRingLoads * kRingDepth): vmcnt in bits [3:0] and [15:14]
+    constexpr int kVm = kRingLoads * kRingDepth;
+    constexpr int kWait = (kVm & 15) | ((kVm >> 4) << 14) | (7 << 4) | (15 << 8);
+    for (int i = 0; i < nrows; ++i) {
+      issue(i + kRingDepth);
+      __builtin_amdgcn_s_waitcnt(kWait);  // row i has landed (kRingDepth rows issued after it)
+      __builtin_amdgcn_s_barrier();
+    }
+    __builtin_amdgcn_s_waitcnt(0);  // no load may land after the workgroup ends
+    return;
+  }
+
+  // ---- consumer ----
+  const int64_t j = cb * kFormCols + tid;
+  const bool act = j < f.N;
+  // this column's coefficients and stencil positions (a column past N reads
+  // column 0's and discards them); the loads are issued before any is used
+  const double* pb = f.beta + (act ? j * f.k : 0);
+  const int16_t* pu = upos + (act ? j * f.k : 0);
+  const double* pa = f.alpha + (act ? j * f.M : 0);
+  uint32_t la[KMAX];
+  double bj[KMAX];
+  double aj[MMAX];
+#pragma unroll
+  for (int t = 0; t < KMAX; ++t) {
+    bj[t] = 0.0;
+    la[t] = 0u;
+    if (t < k) {
+      bj[t] = pb[t];
+      la[t] = (uint32_t)(int32_t)pu[t];
+    }
+  }
+#pragma unroll
+  for (int m = 0; m < MMAX; ++m) aj[m] = m < M ? pa[m] : 0.0;
+#pragma unroll
+  for (int t = 0; t < KMAX; ++t) {
+    const int32_t pp = (int32_t)la[t];
+    la[t] = (act && t < k && pp >= 0 && pp < nu) ? (uint32_t)pp * (uint32_t)sizeof(double) : 0u;
+    if (!act) bj[t] = 0.0;
+  }
+#pragma unroll
+  for (int m = 0; m < MMAX; ++m)
+    if (!act) aj[m] = 0.0;
+  if (!fits) {
+    for (int64_t r = rbeg; r < rend; ++r) {
+      const double* Ki = f.K + r * f.ld;
+      double acc = 0.0;
+#pragma unroll
+      for (int t = 0; t < KMAX; ++t)
+        if (t < k && act) {
+          const int64_t id = checked_id(f.stencil[j * f.k + t], f.N, j, f.status);
+          acc = __builtin_fma(Ki[id], bj[t], acc);
+        }
+#pragma unroll
+      for (int m = 0; m < MMAX; ++m)
+        if (m < M) acc = __builtin_fma(Ki[f.N + m], aj[m], acc);
+      if (act) f.out[r * f.ld32 + j] = (float)(f.rs[r] * acc);
+    }
+    return;
+  }
+  // the monomial columns' LDS offsets; unused stencil and monomial slots
+  // read entry 0 (a K value, finite) with a zero coefficient, so the row's
+  // LDS reads need no branches and are issued together
+  uint32_t lm[MMAX];
+#pragma unroll
+  for (int m = 0; m < MMAX; ++m) lm[m] = (uint32_t)(m < M ? nu + m : 0) * (uint32_t)sizeof(double);
+  // row scales through the scalar cache (read-only here), one row ahead, so
+  // that the loop has no vector load to wait for and its stores never stall it
+  double rs_next = rsv[rbeg];
+  for (int i = 0; i < nrows; ++i) {
+    const double rs_i = rs_next;
+    rs_next = rsv[rbeg + (i + 1 < nrows ? i + 1 : i)];
+    __builtin_amdgcn_s_barrier();  // the producers' loads of row i have landed
+    const char* row = reinterpret_cast<const char*>(&ring[i & (kRingSlots - 1)][0]);
+    double acc = 0.0;
+#pragma unroll
+    for (int t = 0; t < KMAX; ++t) acc = __builtin_fma(*reinterpret_cast<const double*>(row + la[t]), bj[t], acc);
+#pragma unroll
+    for (int m = 0; m < MMAX; ++m) acc = __builtin_fma(*reinterpret_cast<const double*>(row + lm[m]), aj[m], acc);
+    if (act) f.out[(rbeg + i) * f.ld32 + j] = (float)(rs_i * acc);
+  }
+}
+
 // ---- the preconditioner's vector kernels ----------------------------------
 struct VecArgs {
   int64_t N, M, k;
@@ -796,7 +949,14 @@ extern "C" int rbfx_cardinal_form(const rbfx_cardinal* c, const double* d_K64, i
     RBFX_FORM_M(KM, 6);    \
   else                     \
     RBFX_FORM_M(KM, kCardMaxM)
-    if (c->k <= 16)
+#define RBFX_RING(KM)                                                                                       \
+  hipLaunchKernelGGL((cardinal_form_ring_kernel<KM, 6>), dim3((unsigned)blocks), dim3(2 * kFormCols), 0, s, f, \
+                     c->d_union_ptr, c->d_union_idx, c->d_union_pos, ncb, nrb, (const double*)c->d_rs)
+    if (c->k <= 16 && c->M <= 6)
+      RBFX_RING(16);
+    else if (c->k <= 24 && c->M <= 6)
+      RBFX_RING(24);
+    else if (c->k <= 16)
       RBFX_FORM(16);
     else if (c->k <= 24)
       RBFX_FORM(24);
@@ -804,6 +964,7 @@ extern "C" int rbfx_cardinal_form(const rbfx_cardinal* c, const double* d_K64, i
       RBFX_FORM(32);
     else
       RBFX_FORM(48);
+#undef RBFX_RING
 #undef RBFX_FORM
 #undef RBFX_FORM_M
   } else {
