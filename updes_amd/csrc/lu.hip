@@ -230,7 +230,7 @@ constexpr int kGemmStages = 4;
 
 // BT x BT block tile (BT = 128, or 64 for small FP64 products), 2 x 2 waves
 // of (BT/2) x (BT/2), each TW x TW MFMA 16x16 tiles.
-template <typename T, int USE, int BT>
+template <typename T, int USE, int BT, int STAGES = kGemmStages>
 __device__ __forceinline__ void gemm_dma_body(int64_t m, int64_t n, int64_t k, T alpha, const T* __restrict__ A,
                                               int64_t lda, const T* __restrict__ B, int64_t ldb, T beta,
                                               T* __restrict__ C, int64_t ldc, int64_t tiles_m, int64_t tiles_n) {
@@ -246,7 +246,7 @@ __device__ __forceinline__ void gemm_dma_body(int64_t m, int64_t n, int64_t k, T
   static_assert(TW % E == 0 || E == 2, "E adjacent n-tiles per 16-byte B read");
   static_assert(TW >= E, "E adjacent n-tiles per 16-byte B read");
   typedef T vec_t __attribute__((ext_vector_type(E)));
-  __shared__ __align__(16) T sm[kGemmStages][2][NCH * E];
+  __shared__ __align__(16) T sm[STAGES][2][NCH * E];
 
   // tile order as in gemm_kernel (groups of 8 tile rows, XCD-contiguous)
   const int64_t T_ = tiles_m * tiles_n;
@@ -313,14 +313,14 @@ __device__ __forceinline__ void gemm_dma_body(int64_t m, int64_t n, int64_t k, T
 
   const int64_t nt = k / BK;
 #pragma unroll
-  for (int t = 0; t < kGemmStages - 1; ++t)
+  for (int t = 0; t < STAGES - 1; ++t)
     if (t < nt) issue((int64_t)t * BK, t);
 
   for (int64_t t = 0; t < nt; ++t) {
     // tile t has landed once at most (tiles issued after it) x 2 RC loads of
     // this thread are outstanding; the barrier makes every thread's part
     // visible and ends all reads of the stage refilled below
-    const int64_t ahead = nt - 1 - t < kGemmStages - 2 ? nt - 1 - t : kGemmStages - 2;
+    const int64_t ahead = nt - 1 - t < STAGES - 2 ? nt - 1 - t : STAGES - 2;
     if (ahead >= 2)
       __builtin_amdgcn_s_waitcnt(0xF70 | (4 * RC));
     else if (ahead == 1)
@@ -331,8 +331,8 @@ __device__ __forceinline__ void gemm_dma_body(int64_t m, int64_t n, int64_t k, T
     // tiles still in flight (its release fence covers the LDS-DMA).  Every
     // LDS read of this wave has completed (the MFMAs consumed them).
     __asm__ volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
-    if (t + kGemmStages - 1 < nt) issue((t + kGemmStages - 1) * BK, (int)((t + kGemmStages - 1) % kGemmStages));
-    const int st = (int)(t % kGemmStages);
+    if (t + STAGES - 1 < nt) issue((t + STAGES - 1) * BK, (int)((t + STAGES - 1) % STAGES));
+    const int st = (int)(t % STAGES);
     const T* As = sm[st][0];
     const T* Bs = sm[st][1];
     vec_t a[TW], b[TW / E][E];
@@ -372,11 +372,11 @@ __device__ __forceinline__ void gemm_dma_body(int64_t m, int64_t n, int64_t k, T
     }
 }
 
-template <typename T, int USE>
+template <typename T, int USE, int STAGES = kGemmStages>
 __global__ __launch_bounds__(GTHREADS) __attribute__((amdgpu_waves_per_eu(2, 2))) void gemm_dma_kernel(
     int64_t m, int64_t n, int64_t k, T alpha, const T* __restrict__ A, int64_t lda, const T* __restrict__ B,
     int64_t ldb, T beta, T* __restrict__ C, int64_t ldc, int64_t tiles_m, int64_t tiles_n) {
-  gemm_dma_body<T, USE, 128>(m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, tiles_m, tiles_n);
+  gemm_dma_body<T, USE, 128, STAGES>(m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, tiles_m, tiles_n);
 }
 
 // 64 x 64 tiles for FP64 products too small to fill the GPU with 128 x 128
@@ -391,9 +391,12 @@ __global__ __launch_bounds__(GTHREADS) void gemm_dma64_kernel(int64_t m, int64_t
   gemm_dma_body<double, USE, 64>(m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, tiles_m, tiles_n);
 }
 
+// small_lds: an FP32 product of the panel stream, which runs beside the
+// trailing update: a two-stage ring (32 KB of LDS), so that its workgroups
+// fit beside two update workgroups on a CU instead of waiting for one to end.
 template <typename T, int USE = 0>
 static int gemm_launch(int64_t m, int64_t n, int64_t k, T alpha, const T* A, int64_t lda, const T* B,
-                       int64_t ldb, T beta, T* C, int64_t ldc, hipStream_t s) {
+                       int64_t ldb, T beta, T* C, int64_t ldc, hipStream_t s, bool small_lds = false) {
   if (m <= 0 || n <= 0) return RBFX_OK;
   const int64_t tm = ceil_div(m, GBM), tn = ceil_div(n, GBN);
   const int64_t tiles = tm * tn;
@@ -408,7 +411,10 @@ static int gemm_launch(int64_t m, int64_t n, int64_t k, T alpha, const T* A, int
     hipLaunchKernelGGL((gemm_dma64_kernel<USE>), dim3((unsigned)grid64), dim3(GTHREADS), 0, s, m, n, k,
                        (double)alpha, reinterpret_cast<const double*>(A), lda, reinterpret_cast<const double*>(B),
                        ldb, (double)beta, reinterpret_cast<double*>(C), ldc, tm64, tn64);
-  } else if (dma)
+  } else if (dma && small_lds && sizeof(T) == 4)
+    hipLaunchKernelGGL((gemm_dma_kernel<T, USE, 2>), dim3((unsigned)grid), dim3(GTHREADS), 0, s, m, n, k, alpha, A,
+                       lda, B, ldb, beta, C, ldc, tm, tn);
+  else if (dma)
     hipLaunchKernelGGL((gemm_dma_kernel<T, USE>), dim3((unsigned)grid), dim3(GTHREADS), 0, s, m, n, k, alpha, A,
                        lda, B, ldb, beta, C, ldc, tm, tn);
   else
@@ -427,7 +433,7 @@ static int gemm(int64_t m, int64_t n, int64_t k, T alpha, const T* A, int64_t ld
   const int pid = prof_start(in_panel ? (sizeof(T) == 8 ? "gemm_panel_f64" : "gemm_panel_f32")
                                       : (sizeof(T) == 8 ? "gemm_f64" : "gemm_f32"),
                              s);
-  const int st = in_panel ? gemm_launch<T, 3>(m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, s)
+  const int st = in_panel ? gemm_launch<T, 3>(m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, s, true)
                           : gemm_launch<T, 0>(m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, s);
   prof_stop(pid, s, 2.0 * m * n * k, (double)sizeof(T) * (2.0 * m * n + (double)m * k + (double)k * n));
   return st;
@@ -1051,6 +1057,7 @@ struct LuCtx {
   int wleaf = kPanelWMax;  // leaf width of the recursion
   int64_t pc0 = 0;  // outer panel being factored: columns [pc0, pc0 + pw)
   int pw = 0;
+  bool on_panel_stream = false;  // the panel stream's context (runs beside the trailing update)
 };
 
 // Upper bound on the workgroups of one leaf launch (0: the CU count).
@@ -1190,7 +1197,8 @@ static int trsm(LuCtx<T>& c, int64_t r0, int W, const T* X, int64_t ldx, int64_t
     hipLaunchKernelGGL((copy_block_kernel<T>), dim3((unsigned)std::min<int64_t>(ceil_div(w, 256), 64), W),
                        dim3(256), 0, c.s, B + off, c.lda, W, w, c.sc.swap_tmp, w);
     RBFX_CHECK_LAUNCH();
-    if ((st = gemm_launch<T, 1>(W, w, W, T(1), X, ldx, c.sc.swap_tmp, w, T(0), B + off, c.lda, c.s)))
+    if ((st = gemm_launch<T, 1>(W, w, W, T(1), X, ldx, c.sc.swap_tmp, w, T(0), B + off, c.lda, c.s,
+                                c.on_panel_stream)))
       return st;
   }
   prof_stop(pid, c.s, 2.0 * W * W * ncols, 3.0 * sizeof(T) * W * ncols);
@@ -1228,8 +1236,10 @@ static int panel(LuCtx<T>& c, int64_t r0, int W, T* X, int64_t ldx) {
   RBFX_REQUIRE((int64_t)W1 * W2 <= c.sc.swap_tmp_elems, "inverse staging too small");
   T* Tm = c.sc.swap_tmp;
   const int pid = prof_start(sizeof(T) == 8 ? "trtri_f64" : "trtri_f32", c.s);
-  if ((st = gemm_launch<T, 2>(W2, W1, W1, T(1), L21, c.lda, X, ldx, T(0), Tm, W1, c.s))) return st;
-  st = gemm_launch<T, 2>(W2, W1, W2, T(-1), X22, ldx, Tm, W1, T(0), X + (int64_t)W1 * ldx, ldx, c.s);
+  if ((st = gemm_launch<T, 2>(W2, W1, W1, T(1), L21, c.lda, X, ldx, T(0), Tm, W1, c.s, c.on_panel_stream)))
+    return st;
+  st = gemm_launch<T, 2>(W2, W1, W2, T(-1), X22, ldx, Tm, W1, T(0), X + (int64_t)W1 * ldx, ldx, c.s,
+                         c.on_panel_stream);
   prof_stop(pid, c.s, 2.0 * W2 * W1 * (double)(W1 + W2), (double)sizeof(T) * 3.0 * W1 * W2);
   return st;
 }
@@ -1365,6 +1375,7 @@ static int getrf(int64_t n, T* A, int64_t lda, int64_t* ipiv, void* work, size_t
   // panel context: same matrix and pivot records, own stream and scratch
   LuCtx<T> cp = c;
   cp.s = ps->s;
+  cp.on_panel_stream = true;
   cp.sc = sc_panel;
   hipEvent_t ready = ps->ev[0], done = ps->ev[1];
   // Panel stream, per step: factor the panel (which also leaves the inverse
