@@ -346,6 +346,12 @@ int rbfx_gmres_ir_cardinal(int trans, const rbfx_cardinal* c, const double* d_K6
 size_t rbfx_frobenius_workspace(int64_t m);
 int rbfx_frobenius_f64(int64_t m, int64_t n, const double* d_A, int64_t lda, const double* d_row_scale,
                        int64_t n_scaled, double* h_norm, void* d_work, size_t work_bytes, void* stream);
+/* Both norms of rbfx_frobenius_f64 from one read of A: with the row scales
+ * (h_norm_scaled) and without (h_norm); the same workspace.  No counterpart
+ * in the reference (the norms of GMRES-IR's backward-error test). */
+int rbfx_frobenius_pair_f64(int64_t m, int64_t n, const double* d_A, int64_t lda, const double* d_row_scale,
+                            int64_t n_scaled, double* h_norm_scaled, double* h_norm, void* d_work,
+                            size_t work_bytes, void* stream);
 
 /* ---- per-launch timing ----------------------------------------------------
  * Between rbfx_profile_begin() and rbfx_profile_end() every launch of the

@@ -43,6 +43,26 @@ def test_gemv_matches_numpy(trans, m, n, lda, alpha, beta):
     assert np.abs(got - ref).max() <= 1e-14 * scale * max(1, (m if trans else n) ** 0.5)
 
 
+@pytest.mark.parametrize("m,n,lda,nrs", [(1, 1, 1, 1), (300, 257, 264, 200), (5000, 301, 301, 5000), (64, 64, 64, 0)])
+def test_frobenius_norms_match_numpy(m, n, lda, nrs):
+    """rbfx_frobenius_f64 (rows i < n_scaled scaled) and the one-pass pair
+    (scaled, unscaled) against NumPy."""
+    L = _lib()
+    from updes_amd import engine as E
+    rng = np.random.default_rng(m + n)
+    A = rng.standard_normal((m, lda))
+    rs = rng.uniform(0.5, 2.0, nrs)
+    dA = torch.as_tensor(A).cuda()
+    drs = torch.as_tensor(rs).cuda()
+    scale = np.ones(m)
+    scale[:nrs] = rs
+    ref_s = np.linalg.norm(A[:, :n] * scale[:, None])
+    ref = np.linalg.norm(A[:, :n])
+    got_s, got = E.frobenius_pair(dA, m, n, drs)
+    assert abs(got_s - ref_s) <= 1e-13 * ref_s and abs(got - ref) <= 1e-13 * ref
+    assert E.frobenius(dA, m, n, drs) == got_s and E.frobenius(dA, m, n) == got
+
+
 @pytest.mark.parametrize("dtype", [torch.float64, torch.float32])
 @pytest.mark.parametrize("m,n,k,beta", [(1, 1, 1, 1.0), (130, 70, 17, 1.0), (300, 257, 64, 1.0),
                                          (512, 384, 256, 1.0), (300, 258, 64, 1.0), (130, 72, 16, 0.0),

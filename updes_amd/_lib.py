@@ -115,6 +115,8 @@ _SIGS = {
                                               _p, _p, _p, _d, _i32, _i32, _p, _sz, ctypes.POINTER(IrStats), _p]),
     "rbfx_frobenius_workspace": (_sz, [_i64]),
     "rbfx_frobenius_f64": (ctypes.c_int, [_i64, _i64, _p, _i64, _p, _i64, ctypes.POINTER(_d), _p, _sz, _p]),
+    "rbfx_frobenius_pair_f64": (ctypes.c_int, [_i64, _i64, _p, _i64, _p, _i64, ctypes.POINTER(_d),
+                                               ctypes.POINTER(_d), _p, _sz, _p]),
     "rbfx_profile_begin": (ctypes.c_int, []),
     "rbfx_profile_end": (ctypes.c_int, [ctypes.POINTER(ProfEntry), ctypes.c_int, ctypes.POINTER(ctypes.c_int)]),
     "rbfx_gemm_f64": (ctypes.c_int, [_i64, _i64, _i64, _d, _p, _i64, _p, _i64, _d, _p, _i64, _p]),

@@ -245,10 +245,14 @@ __global__ void fill_kernel(int64_t n, double v, double* __restrict__ y) {
 
 // Frobenius norm partials: sum of squares of rows [blockIdx.x*rows, ...),
 // row i scaled by rs[i] when i < nrs.
+// Partial sums of squares over row blocks: part[b] with rows i < nrs scaled
+// by rs[i] (rs may be null), and, when part1 is given, part1[b] unscaled
+// from the same loads.
 __global__ __launch_bounds__(256) void fro_partial_kernel(int64_t m, int64_t n, const double* __restrict__ A,
                                                           int64_t lda, int64_t rows, const double* __restrict__ rs,
-                                                          int64_t nrs, double* __restrict__ part) {
-  double s = 0.0;
+                                                          int64_t nrs, double* __restrict__ part,
+                                                          double* __restrict__ part1) {
+  double s = 0.0, s1 = 0.0;
   const int64_t r0 = (int64_t)blockIdx.x * rows;
   const int64_t r1 = r0 + rows < m ? r0 + rows : m;
   for (int64_t i = r0; i < r1; ++i) {
@@ -259,12 +263,22 @@ __global__ __launch_bounds__(256) void fro_partial_kernel(int64_t m, int64_t n, 
       t += v * v;
     }
     s += f * f * t;
+    s1 += t;
   }
-  __shared__ double red[4];
-  for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off, 64);
-  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
+  __shared__ double red[2][4];
+  for (int off = 32; off > 0; off >>= 1) {
+    s += __shfl_down(s, off, 64);
+    s1 += __shfl_down(s1, off, 64);
+  }
+  if ((threadIdx.x & 63) == 0) {
+    red[0][threadIdx.x >> 6] = s;
+    red[1][threadIdx.x >> 6] = s1;
+  }
   __syncthreads();
-  if (threadIdx.x == 0) part[blockIdx.x] = red[0] + red[1] + red[2] + red[3];
+  if (threadIdx.x == 0) {
+    part[blockIdx.x] = red[0][0] + red[0][1] + red[0][2] + red[0][3];
+    if (part1) part1[blockIdx.x] = red[1][0] + red[1][1] + red[1][2] + red[1][3];
+  }
 }
 
 static int dot(int64_t n, const double* a, const double* b, double* part, double* out, hipStream_t s) {
@@ -293,30 +307,52 @@ extern "C" int rbfx_gemv_f64(int trans, int64_t m, int64_t n, double alpha, cons
   return gemv(trans, m, n, alpha, d_A, lda, d_x, beta, d_y, static_cast<double*>(d_work), as_stream(stream));
 }
 
+static int64_t fro_blocks(int64_t m) { return std::max<int64_t>(1, std::min<int64_t>(m, 4096)); }
+
 extern "C" size_t rbfx_frobenius_workspace(int64_t m) {
-  return ws_round(sizeof(double) * (size_t)std::max<int64_t>(1, std::min<int64_t>(m, 4096)) + 64);
+  return 2 * ws_round(sizeof(double) * (size_t)fro_blocks(m)) + 64;  // room for rbfx_frobenius_pair_f64
+}
+
+// Both norms (scaled, unscaled) from one read of A; norm1 may be null.
+static int frobenius(int64_t m, int64_t n, const double* A, int64_t lda, const double* rs, int64_t nrs,
+                     double* norm, double* norm1, void* work, size_t work_bytes, hipStream_t s) {
+  RBFX_REQUIRE(m > 0 && n > 0 && lda >= n && A && norm, "bad frobenius args");
+  RBFX_REQUIRE(work && work_bytes >= rbfx_frobenius_workspace(m), "frobenius workspace too small");
+  RBFX_REQUIRE(!rs || (nrs >= 0 && nrs <= m), "bad row scale count");
+  const int64_t rows = ceil_div(m, fro_blocks(m));
+  const int64_t used = ceil_div(m, rows);
+  Workspace ws(work, work_bytes);
+  double* part = ws.take<double>(fro_blocks(m));
+  double* part1 = ws.take<double>(fro_blocks(m));
+  RBFX_REQUIRE(part && part1, "frobenius workspace carve failed");
+  hipLaunchKernelGGL(fro_partial_kernel, dim3((unsigned)used), dim3(256), 0, s, m, n, A, lda, rows, rs, nrs, part,
+                     norm1 ? part1 : nullptr);
+  RBFX_CHECK_LAUNCH();
+  std::vector<double> hp(2 * used);
+  RBFX_HIP(hipMemcpyAsync(hp.data(), part, used * sizeof(double), hipMemcpyDeviceToHost, s));
+  if (norm1) RBFX_HIP(hipMemcpyAsync(hp.data() + used, part1, used * sizeof(double), hipMemcpyDeviceToHost, s));
+  RBFX_HIP(hipStreamSynchronize(s));
+  double t = 0.0, t1 = 0.0;
+  for (int64_t b = 0; b < used; ++b) {
+    t += hp[b];
+    t1 += hp[used + b];
+  }
+  *norm = sqrt(t);
+  if (norm1) *norm1 = sqrt(t1);
+  return RBFX_OK;
 }
 
 extern "C" int rbfx_frobenius_f64(int64_t m, int64_t n, const double* d_A, int64_t lda, const double* d_row_scale,
                                   int64_t n_scaled, double* h_norm, void* d_work, size_t work_bytes, void* stream) {
-  RBFX_REQUIRE(m > 0 && n > 0 && lda >= n && d_A && h_norm, "bad frobenius args");
-  RBFX_REQUIRE(d_work && work_bytes >= rbfx_frobenius_workspace(m), "frobenius workspace too small");
-  hipStream_t s = as_stream(stream);
-  const int64_t blocks = std::min<int64_t>(m, 4096);
-  const int64_t rows = ceil_div(m, blocks);
-  const int64_t used = ceil_div(m, rows);
-  double* part = static_cast<double*>(d_work);
-  RBFX_REQUIRE(!d_row_scale || (n_scaled >= 0 && n_scaled <= m), "bad row scale count");
-  hipLaunchKernelGGL(fro_partial_kernel, dim3((unsigned)used), dim3(256), 0, s, m, n, d_A, lda, rows, d_row_scale,
-                     n_scaled, part);
-  RBFX_CHECK_LAUNCH();
-  std::vector<double> hp(used);
-  RBFX_HIP(hipMemcpyAsync(hp.data(), part, used * sizeof(double), hipMemcpyDeviceToHost, s));
-  RBFX_HIP(hipStreamSynchronize(s));
-  double t = 0.0;
-  for (double v : hp) t += v;
-  *h_norm = sqrt(t);
-  return RBFX_OK;
+  return frobenius(m, n, d_A, lda, d_row_scale, n_scaled, h_norm, nullptr, d_work, work_bytes, as_stream(stream));
+}
+
+extern "C" int rbfx_frobenius_pair_f64(int64_t m, int64_t n, const double* d_A, int64_t lda,
+                                       const double* d_row_scale, int64_t n_scaled, double* h_norm_scaled,
+                                       double* h_norm, void* d_work, size_t work_bytes, void* stream) {
+  RBFX_REQUIRE(h_norm, "bad frobenius args");
+  return frobenius(m, n, d_A, lda, d_row_scale, n_scaled, h_norm_scaled, h_norm, d_work, work_bytes,
+                   as_stream(stream));
 }
 
 namespace rbfx {

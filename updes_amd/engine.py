@@ -520,6 +520,18 @@ def frobenius(mat, rows, cols, row_scale=None):
     return out.value
 
 
+def frobenius_pair(mat, rows, cols, row_scale):
+    """(scaled, unscaled) Frobenius norms of mat[:rows, :cols] from one read:
+    rows scaled by row_scale (a prefix), and without scaling."""
+    lib = L.lib()
+    ws = L.workspace(lib.rbfx_frobenius_workspace(rows), mat.device)
+    scaled, plain = ctypes.c_double(0.0), ctypes.c_double(0.0)
+    L.check(lib.rbfx_frobenius_pair_f64(rows, cols, L.ptr(mat), mat.shape[1], L.ptr(row_scale), row_scale.numel(),
+                                        ctypes.byref(scaled), ctypes.byref(plain), L.ptr(ws), ws.numel(),
+                                        L.stream()))
+    return scaled.value, plain.value
+
+
 class KFactor:
     """Factorisation of the collocation matrix K for one (cloud, rbf, operator).
 
@@ -557,7 +569,7 @@ class KFactor:
             self.rs = torch.empty(N, dtype=torch.float64, device=self.K64.device)
             kt = self.basis.form(self.K64, self.rs)
             # norms of the row-scaled forward system and of the transposed one
-            self.knorm = (frobenius(self.K64, self.n, self.n, self.rs), frobenius(self.K64, self.n, self.n))
+            self.knorm = frobenius_pair(self.K64, self.n, self.n, self.rs)
             self.lu32 = LU(kt, N)
         else:
             raise ValueError("precision must be 'fp64' or 'fp32_ir'")
