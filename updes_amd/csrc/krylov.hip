@@ -437,11 +437,12 @@ int fgmres_ir(PairOp& pre, ResidualOp& res, int64_t n, const double* d_b, double
                loc.inner_iters);
       tline = buf;
     }
-    double eta = INFINITY;
+    double eta = INFINITY, xn = -1.0;
     if (rel == rel && rel > o.tol && cycle > 0 && o.knorm > 0.0) {
       double xn2 = 0.0;
       if ((st = host_dot(d_x, d_x, xn2))) return st;
-      eta = rn / (o.knorm * sqrt(xn2) + bn);
+      xn = sqrt(xn2);
+      eta = rn / (o.knorm * xn + bn);
       if (trace) {
         char buf[40];
         snprintf(buf, sizeof(buf), " (eta %.1e)", eta);
@@ -460,7 +461,13 @@ int fgmres_ir(PairOp& pre, ResidualOp& res, int64_t n, const double* d_b, double
     hipLaunchKernelGGL(scale_copy_kernel, dim3(g), dim3(256), 0, s, n, 1.0 / rn, r, V);
     RBFX_CHECK_LAUNCH();
     ls.start(rn);
-    const double inner_tol = std::max(0.1 * o.tol * bn, 1e-10 * rn);
+    // The cycle's target: a residual at which the next cycle's test passes
+    // (relative residual tol, or, once ||x|| is known, a backward error of
+    // kStopEta with this ||x||), with a margin; but no deeper than one cycle
+    // reaches in FP64 (1e-10 of its starting residual).
+    double target = 0.1 * o.tol * bn;
+    if (xn >= 0.0) target = std::max(target, 0.5 * kStopEta * (o.knorm * xn + bn));
+    const double inner_tol = std::max(target, 1e-10 * rn);
     int kdone = 0;
     bool early_stall = false;
     mon.est_met = false;
