@@ -465,9 +465,12 @@ int fgmres_ir(PairOp& pre, ResidualOp& res, int64_t n, const double* d_b, double
     // (relative residual tol, or, once ||x|| is known, a backward error of
     // kStopEta with this ||x||), with a margin; but no deeper than one cycle
     // reaches in FP64 (1e-10 of its starting residual).
+    // The first cycle may go to 1e-12 of its residual: it often meets the
+    // tolerance itself, and the backward-error target (from ||Z y|| of its
+    // current iterate, checked from 1e-8 on) stops it where it cannot.
     double target = 0.1 * o.tol * bn;
     if (xn >= 0.0) target = std::max(target, 0.5 * kStopEta * (o.knorm * xn + bn));
-    const double inner_tol = std::max(target, 1e-10 * rn);
+    const double inner_tol = std::max(target, (cycle == 0 ? 1e-12 : 1e-10) * rn);
     int kdone = 0;
     bool early_stall = false;
     mon.est_met = false;
@@ -499,7 +502,18 @@ int fgmres_ir(PairOp& pre, ResidualOp& res, int64_t n, const double* d_b, double
         snprintf(buf, sizeof(buf), " %.1e", est / bn);
         tline += buf;
       }
-      if (est <= inner_tol || hn == 0.0) {
+      double cycle_tol = inner_tol;
+      if (cycle == 0 && o.knorm > 0.0 && est > inner_tol && est <= 1e-8 * rn) {
+        // ||x|| of the cycle's current iterate x = Z y (the first cycle starts
+        // from x = 0), for the backward-error target
+        const double* yk = ls.solve(k + 1);
+        RBFX_HIP(hipMemcpyAsync(hd, yk, (k + 1) * sizeof(double), hipMemcpyHostToDevice, s));
+        if ((st = gemv(1, k + 1, n, 1.0, Z, n, hd, 0.0, u, part, s))) return st;
+        double xk2 = 0.0;
+        if ((st = host_dot(u, u, xk2))) return st;  // synchronises: yk outlives the copy
+        cycle_tol = std::max(inner_tol, 0.5 * kStopEta * (o.knorm * sqrt(xk2) + bn));
+      }
+      if (est <= cycle_tol || hn == 0.0) {
         mon.est_met = true;
         break;
       }
