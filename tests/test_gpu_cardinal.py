@@ -32,8 +32,12 @@ def _setup(kind, param, M, nx=12, ny=11, facets=None, seed=2, k=20):
 
 @pytest.mark.parametrize("kind,param", C.KINDS)
 @pytest.mark.parametrize("M", [3, 6])
-def test_cardinal_coefficients_match_numpy(kind, param, M):
-    oc, uc, prob, basis = _setup(kind, param, M)
+@pytest.mark.parametrize("k", [20, 15])
+def test_cardinal_coefficients_match_numpy(kind, param, M, k):
+    """Local cardinal coefficients against NumPy, for local systems of more
+    than 32 rows (k = 20: 29 stencil entries with the special points) and of
+    at most 32 (k = 15: 24 entries), which use a smaller LDS layout."""
+    oc, uc, prob, basis = _setup(kind, param, M, k=k)
     st = basis.stencil.cpu().numpy().reshape(oc.N, basis.k)
     beta_ref, alpha_ref = CR.coefficients(oc.sorted_nodes, st, kind, param, M)
     beta = basis.beta.cpu().numpy().reshape(oc.N, basis.k)

@@ -33,7 +33,6 @@ namespace rbfx {
 
 constexpr int kCardMaxK = 48;
 constexpr int kCardMaxM = 15;
-constexpr int kCardPitch = 65;  // LDS row pitch (doubles) of the local system
 constexpr int kFormRows = 16;   // rows of K~ per workgroup of the gather formation kernel
 constexpr int kFormCols = 256;  // columns per block of the union tables (rbfx_cardinal)
 constexpr int kFormTR = 2;      // rows staged in LDS per pass of the LDS formation kernel
@@ -92,12 +91,16 @@ struct CardArgs {
 // (x - x_j)/s, s the stencil radius), project beta' onto the null space of
 // P_S^T (two passes against an orthonormal basis of P_S, so that the moment
 // conditions hold to rounding), and expand the local polynomial a' into the
-// global monomials.
+// global monomials.  MS bounds the local system's size k + M (32 for the
+// usual stencils of at most 26 entries with at most 6 monomials: 12 KB of
+// LDS, so about three times as many nodes in flight per CU as with 64).
+template <int MS>
 __global__ __launch_bounds__(64) void cardinal_coeffs_kernel(CardArgs a) {
-  __shared__ double A[64][kCardPitch];
+  constexpr int kQ = MS == 64 ? kCardMaxM : 6;  // monomials of the projection basis
+  __shared__ double A[MS][MS + 1];
   __shared__ double rhs[64];
   __shared__ double sol[64];
-  __shared__ double Qs[kCardMaxM][64];
+  __shared__ double Qs[kQ][64];
   const int64_t j = blockIdx.x;
   const int lane = threadIdx.x;
   const int k = (int)a.k, M = (int)a.M, S = k + M;
@@ -916,7 +919,10 @@ extern "C" int rbfx_cardinal_coeffs(const rbfx_problem* p, const rbfx_cardinal* 
   CardArgs a{c->N, c->M, c->k, c->d_x, c->d_y, c->d_stencil, c->d_beta, c->d_alpha, make_kern(p->kernel, p->param),
              status};
   const int pid = prof_start("cardinal_coeffs", s);
-  hipLaunchKernelGGL(cardinal_coeffs_kernel, dim3((unsigned)c->N), dim3(64), 0, s, a);
+  if (c->k + c->M <= 32 && c->M <= 6)
+    hipLaunchKernelGGL(cardinal_coeffs_kernel<32>, dim3((unsigned)c->N), dim3(64), 0, s, a);
+  else
+    hipLaunchKernelGGL(cardinal_coeffs_kernel<64>, dim3((unsigned)c->N), dim3(64), 0, s, a);
   RBFX_CHECK_LAUNCH();
   prof_stop(pid, s, 0.0, 0.0);
   return read_status(status, s, "cardinal coefficients");
