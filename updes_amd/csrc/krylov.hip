@@ -45,7 +45,8 @@ __global__ __launch_bounds__(kGvThreads) void gemv_n_block_kernel(int64_t m, int
   }
 }
 
-// Same, one wave per row (many rows): 2 doubles per lane per step.
+// Same, one wave per row (many rows): 16-byte non-temporal loads, four in
+// flight per lane.
 __global__ __launch_bounds__(kGvThreads) void gemv_n_wave_kernel(int64_t m, int64_t n, double alpha,
                                                                  const double* __restrict__ A, int64_t lda,
                                                                  const double* __restrict__ x, double beta,
@@ -61,11 +62,31 @@ __global__ __launch_bounds__(kGvThreads) void gemv_n_wave_kernel(int64_t m, int6
     typedef double vd2 __attribute__((ext_vector_type(2)));
     const vd2* r2 = reinterpret_cast<const vd2*>(row);
     const vd2* x2 = reinterpret_cast<const vd2*>(x);
-    for (int64_t j = lane; j < n2; j += 64) {
+    // four 16-byte loads in flight per lane
+    double s1 = 0.0, s2 = 0.0, s3 = 0.0;
+    int64_t j = lane;
+    for (; j + 192 < n2; j += 256) {
+      const vd2 a0 = __builtin_nontemporal_load(r2 + j);
+      const vd2 a1 = __builtin_nontemporal_load(r2 + j + 64);
+      const vd2 a2 = __builtin_nontemporal_load(r2 + j + 128);
+      const vd2 a3 = __builtin_nontemporal_load(r2 + j + 192);
+      const vd2 b0 = x2[j], b1 = x2[j + 64], b2 = x2[j + 128], b3 = x2[j + 192];
+      s = __builtin_fma(a0.x, b0.x, s);
+      s = __builtin_fma(a0.y, b0.y, s);
+      s1 = __builtin_fma(a1.x, b1.x, s1);
+      s1 = __builtin_fma(a1.y, b1.y, s1);
+      s2 = __builtin_fma(a2.x, b2.x, s2);
+      s2 = __builtin_fma(a2.y, b2.y, s2);
+      s3 = __builtin_fma(a3.x, b3.x, s3);
+      s3 = __builtin_fma(a3.y, b3.y, s3);
+    }
+    for (; j < n2; j += 64) {
       const vd2 a = __builtin_nontemporal_load(r2 + j);
       const vd2 b = x2[j];
-      s += a.x * b.x + a.y * b.y;
+      s = __builtin_fma(a.x, b.x, s);
+      s = __builtin_fma(a.y, b.y, s);
     }
+    s += (s1 + s2) + s3;
     if ((n & 1) && lane == 0) s += row[n - 1] * x[n - 1];
   } else {
     for (int64_t j = lane; j < n; j += 64) s += row[j] * x[j];
@@ -77,7 +98,7 @@ __global__ __launch_bounds__(kGvThreads) void gemv_n_wave_kernel(int64_t m, int6
 // y[j] = alpha * sum_i A[i][j] x[i] + beta * y[j], partial sums over row
 // chunks (reduced by gemv_t_finish_kernel).  Workgroup (column block of
 // kGvTCols, row chunk): lane l of wave w owns columns 128 b + 2 l and
-// 128 b + 2 l + 1 and the rows rs + w, rs + w + 4, ... of the chunk, with four
+// 128 b + 2 l + 1 and the rows rs + w, rs + w + 4, ... of the chunk, with eight
 // 16-byte row loads in flight; the four waves' sums meet in LDS.
 constexpr int kGvTCols = 128;
 __global__ __launch_bounds__(kGvThreads) void gemv_t_kernel(int64_t m, int64_t n, const double* __restrict__ A,
@@ -94,20 +115,19 @@ __global__ __launch_bounds__(kGvThreads) void gemv_t_kernel(int64_t m, int64_t n
   if (vec && c0 + 1 < n) {
     const double* a = A + c0;
     int64_t i = rs + w;
-    for (; i + 12 < re; i += 16) {
-      const vd2 a0 = __builtin_nontemporal_load(reinterpret_cast<const vd2*>(a + i * lda));
-      const vd2 a1 = __builtin_nontemporal_load(reinterpret_cast<const vd2*>(a + (i + 4) * lda));
-      const vd2 a2 = __builtin_nontemporal_load(reinterpret_cast<const vd2*>(a + (i + 8) * lda));
-      const vd2 a3 = __builtin_nontemporal_load(reinterpret_cast<const vd2*>(a + (i + 12) * lda));
-      const double x0 = x[i], x1 = x[i + 4], x2 = x[i + 8], x3 = x[i + 12];
-      s0 = __builtin_fma(a0.x, x0, s0);
-      s1 = __builtin_fma(a0.y, x0, s1);
-      s0 = __builtin_fma(a1.x, x1, s0);
-      s1 = __builtin_fma(a1.y, x1, s1);
-      s0 = __builtin_fma(a2.x, x2, s0);
-      s1 = __builtin_fma(a2.y, x2, s1);
-      s0 = __builtin_fma(a3.x, x3, s0);
-      s1 = __builtin_fma(a3.y, x3, s1);
+    for (; i + 28 < re; i += 32) {
+      vd2 av[8];
+      double xv[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        av[u] = __builtin_nontemporal_load(reinterpret_cast<const vd2*>(a + (i + 4 * u) * lda));
+        xv[u] = x[i + 4 * u];
+      }
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        s0 = __builtin_fma(av[u].x, xv[u], s0);
+        s1 = __builtin_fma(av[u].y, xv[u], s1);
+      }
     }
     for (; i < re; i += 4) {
       const vd2 a0 = __builtin_nontemporal_load(reinterpret_cast<const vd2*>(a + i * lda));
