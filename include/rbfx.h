@@ -390,8 +390,9 @@ int rbfx_set_panel_groups(int max_groups);
 
 /* ---- tuning / test hook -----------------------------------------------------
  * Panels of rbfx_getrf_* with at least `rows` rows use 16-wide leaves instead
- * of 64-wide ones (process-wide; 0 = the default, 28,000 rows; a negative
- * value disables them).  No counterpart in the reference.                  */
+ * of 64-wide ones (process-wide; 0 = the default: 28,000 rows for FP64, none
+ * for FP32; a negative value disables them).  No counterpart in the
+ * reference.                                                               */
 int rbfx_set_narrow_leaf_rows(int64_t rows);
 
 /* ---- test hook --------------------------------------------------------------

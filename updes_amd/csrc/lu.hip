@@ -465,7 +465,11 @@ constexpr int kPanelWMax = 64;
 // workgroups (2 x 216 VGPRs per SIMD), where a 64-wide one (164) displaces
 // one of them.  More leaves and splits per panel, hidden under the update
 // while it is long: panels of at least kNarrowLeafRows rows (measured best
-// at n = 40,006 among 20k..36k).
+// at n = 40,006 among 20k..36k).  FP32 only uses them on request: a 64-wide
+// FP32 leaf (95 VGPRs) already fits beside two update workgroups, and the
+// FP32 update is too short to hide the narrow leaves' extra launches
+// (40k mixed step: 615 ms without, 621 ms from 28,000 rows, 625 ms from
+// 20,000).
 constexpr int kPanelWNarrow = 16;
 constexpr int64_t kNarrowLeafRows = 28000;
 static std::atomic<int64_t> g_narrow_leaf_rows{0};
@@ -1389,7 +1393,7 @@ static int getrf(int64_t n, T* A, int64_t lda, int64_t* ipiv, void* work, size_t
     cp.pc0 = r0;
     cp.pw = W;
     const int64_t nl = g_narrow_leaf_rows.load(std::memory_order_relaxed);
-    const int64_t narrow_rows = nl == 0 ? kNarrowLeafRows : nl;
+    const int64_t narrow_rows = nl != 0 ? nl : sizeof(T) == 8 ? kNarrowLeafRows : -1;
     cp.wleaf = beside_update && narrow_rows > 0 && n - r0 >= narrow_rows ? kPanelWNarrow : kPanelWMax;
     if ((e = panel(cp, r0, W, out.tri_inv, (int64_t)kNB))) return e;
     prof_stop(pid, cp.s, (double)(n - r0) * W * W, 0.0);
