@@ -381,7 +381,8 @@ int rbfx_gemm_f32(int64_t m, int64_t n, int64_t k, float alpha, const float* d_A
 
 /* ---- tuning / test hook -----------------------------------------------------
  * Caps the workgroups of one panel-leaf launch of rbfx_getrf_* (process-wide;
- * 0 = the device's CU count, the default).  A leaf keeps its rows in
+ * 0 = the default: the device's CU count for FP64, a quarter of it for FP32,
+ * or as many as three rows per thread need).  A leaf keeps its rows in
  * registers, one to three rows per thread of 256-thread workgroups, so a cap
  * below ceil(m/256) moves it to two or three rows per thread; a cap that
  * would need more than three fails the factorisation with an error.  No

@@ -1072,7 +1072,15 @@ template <typename T>
 static int leaf(LuCtx<T>& c, int64_t r0, int w) {
   const DeviceInfo& di = device_info();
   const int64_t m = c.n - r0;
-  const int cap = g_panel_groups.load(std::memory_order_relaxed);
+  int cap = g_panel_groups.load(std::memory_order_relaxed);
+  if (cap <= 0 && sizeof(T) == 4) {
+    // FP32: at most a quarter of the CUs (more only where three rows per
+    // thread do not reach m).  Fewer workgroups, each with up to three rows
+    // per thread, exchange faster beside the trailing update: 40k mixed step
+    // 616 ms with up to 256 workgroups, 601 / 596 / 593 ms with 96 / 80 / 64.
+    const int64_t need = ceil_div(m, (int64_t)kPanelThreads * kPanelMaxRows);
+    cap = (int)std::max<int64_t>(di.num_cus / 4, need);
+  }
   const int64_t gmax =
       std::min<int64_t>(cap > 0 ? std::min(cap, di.num_cus) : di.num_cus, kKeyMaxGroups - 1);
   // As few workgroups as one row per thread allows (every column's exchange
