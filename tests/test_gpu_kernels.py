@@ -200,6 +200,31 @@ def test_getrf_multirow_leaves(dtype, n, groups, narrow):
     assert rel < (1e-13 if dtype == torch.float64 else 1e-6) * n, rel
 
 
+def test_getrf_f32_leaf_geometry_does_not_change_the_factors():
+    """The FP32 leaves run on at most a quarter of the CUs by default (two or
+    three rows per thread on tall panels); the factors and pivots are the
+    same, bit for bit, as with one row per thread on up to 256 workgroups:
+    every row's arithmetic is the same whichever workgroup holds it."""
+    from updes_amd import engine as E
+    L = _lib()
+    n = 17000  # the first panels have more than 16,384 rows: two rows per thread
+    g = torch.Generator().manual_seed(17)
+    a = torch.randn(n, n, generator=g, dtype=torch.float32) + 4.0 * torch.eye(n, dtype=torch.float32)
+    m = torch.zeros((n, E._ld(n)), dtype=torch.float32)
+    m[:, :n] = a
+    default = E.LU(m.cuda(), n)
+    torch.cuda.synchronize()
+    L.check(L.lib().rbfx_set_panel_groups(256))
+    try:
+        wide = E.LU(m.cuda(), n)
+        torch.cuda.synchronize()
+    finally:
+        L.check(L.lib().rbfx_set_panel_groups(0))
+    assert default.singular == 0 and wide.singular == 0
+    assert torch.equal(default.ipiv, wide.ipiv)
+    assert torch.equal(default.a[:, :n], wide.a[:, :n])
+
+
 def test_getrf_matches_lapack_pivots():
     """Same pivot sequence as LAPACK getrf on a matrix without ties."""
     from updes_amd import engine as E
