@@ -531,7 +531,11 @@ int fgmres_ir(PairOp& pre, ResidualOp& res, int64_t n, const double* d_b, double
         if ((st = gemv(1, k + 1, n, 1.0, Z, n, hd, 0.0, u, part, s))) return st;
         double xk2 = 0.0;
         if ((st = host_dot(u, u, xk2))) return st;  // synchronises: yk outlives the copy
-        cycle_tol = std::max(inner_tol, 0.5 * kStopEta * (o.knorm * sqrt(xk2) + bn));
+        // the backward-error target, or the rounding floor of this cycle's
+        // residual (2^-53 ||K||_F ||x||), below which more iterations lower
+        // the estimate but not the true residual
+        const double xk = sqrt(xk2);
+        cycle_tol = std::max({inner_tol, 0.5 * kStopEta * (o.knorm * xk + bn), 0x1p-53 * o.knorm * xk});
       }
       if (est <= cycle_tol || hn == 0.0) {
         mon.est_met = true;
